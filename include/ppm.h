@@ -1,0 +1,311 @@
+/*
+ * ppm.h — C ABI of the MI355X (gfx950) progressive-photon-mapping (PPM-PA)
+ * hot path.
+ *
+ * The reference (eijian/ppmpa, a Rust crate) has no FFI layer: its GPU-side
+ * replacement sits behind the crate's public call surface, which the CLIs in
+ * src/bin/ppmpa.rs, pm.rs and rt.rs call directly.  Every entry point below names the reference
+ * function it replaces.  Calls are batch-granular (one call per pass or per
+ * batch of queries), never one call per photon.
+ *
+ * Conventions
+ *  - Every call returns a ppm_status (0 = ok, < 0 = error); nothing aborts.
+ *    ppm_last_error() returns a text description of the last failure on a
+ *    context.  (The reference panics instead: photonmap.rs:26,71 unwrap on
+ *    non-finite kd coordinates; algebra.rs:220, camera.rs:167 on zero-length
+ *    normalisation; photonmap.rs:62-67 on malformed map text.)
+ *  - The caller owns host buffers.  Device buffers are owned by the object
+ *    that allocated them and are freed by the matching *_destroy.
+ *  - All calls on one context are ordered on that context's HIP stream and
+ *    are not thread-safe; use one context per host thread.
+ *  - Geometry and colours cross the boundary as double; the device keeps
+ *    traversal, intersection and gather in float and accumulates the
+ *    per-pixel radiance and the radius schedule in double.
+ */
+#ifndef PPM_H
+#define PPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ppm_status;
+
+#define PPM_OK            0
+#define PPM_E_INVALID    -1 /* bad argument or handle */
+#define PPM_E_NONFINITE  -2 /* NaN/inf coordinate (photonmap.rs:26 panics) */
+#define PPM_E_OOM        -3 /* device or host allocation failed */
+#define PPM_E_HIP        -4 /* a HIP runtime call failed */
+#define PPM_E_RANGE      -5 /* a size beyond a buffer's capacity */
+#define PPM_E_PARSE      -6 /* malformed photon-map text */
+#define PPM_E_NODEVICE   -7 /* no usable gfx950 device */
+
+/* Shapes — geometry.rs:66-90 (Shape enum). */
+#define PPM_SHAPE_POINT          0
+#define PPM_SHAPE_PLAIN          1 /* infinite plane n.x + dist = 0 */
+#define PPM_SHAPE_SPHERE         2
+#define PPM_SHAPE_POLYGON        3 /* triangle p0, p0+dir1, p0+dir2 */
+#define PPM_SHAPE_PARALLELOGRAM  4
+
+/* Surfaces — surface.rs:16-41 (Surface enum). */
+#define PPM_SURF_NOTHING  0
+#define PPM_SURF_SIMPLE   1
+#define PPM_SURF_TS       2
+#define PPM_SURF_DISNEY   3
+#define PPM_SURF_BRADY    4
+
+/* Lights — light.rs:16-39 (Light enum). */
+#define PPM_LIGHT_POINT          0
+#define PPM_LIGHT_PARALLELOGRAM  1
+#define PPM_LIGHT_SUN            2
+
+/* Photon filters — optics.rs:17-21 (PhotonFilter enum), tracer.rs:197-216. */
+#define PPM_FILTER_NONE   0
+#define PPM_FILTER_CONE   1
+#define PPM_FILTER_GAUSS  2
+
+/* Wavelength channel codes — physics.rs:17-21. */
+#define PPM_WL_RED    0
+#define PPM_WL_GREEN  1
+#define PPM_WL_BLUE   2
+
+typedef struct ppm_ctx      ppm_ctx;
+typedef struct ppm_scene    ppm_scene;
+typedef struct ppm_photons  ppm_photons;
+typedef struct ppm_map      ppm_map;
+typedef struct ppm_session  ppm_session;
+
+/* One object = shape + material index (object.rs:10-13). */
+typedef struct {
+  int32_t kind;       /* PPM_SHAPE_* */
+  int32_t material;   /* index into ppm_scene_desc.materials */
+  double  pos[3];     /* POINT position, SPHERE centre, POLYGON/PARALLELOGRAM p0 */
+  double  nvec[3];    /* PLAIN, POLYGON, PARALLELOGRAM unit normal */
+  double  dir1[3];    /* POLYGON/PARALLELOGRAM edge p1 - p0 */
+  double  dir2[3];    /* POLYGON/PARALLELOGRAM edge p2 - p0 */
+  double  dist;       /* PLAIN offset */
+  double  radius;     /* SPHERE radius */
+} ppm_shape_desc;
+
+/* Material (material.rs:11-16) with its surface model (surface.rs:16-41). */
+typedef struct {
+  double  emittance[3];
+  double  transmittance[3];
+  double  ior[3];
+  int32_t surface;      /* PPM_SURF_* */
+  int32_t reserved;
+  double  color1[3];    /* Simple: reflectance;    TS: albedo_diff */
+  double  color2[3];    /* Simple: specular_refl;  TS: albedo_spec */
+  double  diffuseness;  /* Simple: diffuseness;    TS: scatterness */
+  double  metalness;
+  double  roughness;
+} ppm_material_desc;
+
+/* Light (light.rs:16-39).  color is used as given (the reference normalises
+ * it when the scene is built, scene.rs:24). */
+typedef struct {
+  int32_t kind;       /* PPM_LIGHT_* */
+  int32_t reserved;
+  double  color[3];
+  double  flux;
+  double  pos[3];
+  double  nvec[3];
+  double  dir1[3];
+  double  dir2[3];
+  double  dir[3];     /* SUN direction of travel */
+} ppm_light_desc;
+
+/* Camera (camera.rs:23-50): the derived screen vectors of read_camera
+ * (camera.rs:244-256); ppm_camera_setup() computes them from the
+ * user-facing parameters. */
+typedef struct {
+  int32_t xreso, yreso;
+  int32_t progressive, antialias, blur;
+  int32_t pfilter;      /* PPM_FILTER_* */
+  double  eye_pos[3];
+  double  origin[3];
+  double  esx[3], esy[3];
+  double  eex[3], eey[3];
+} ppm_camera_desc;
+
+typedef struct {
+  int32_t n_shapes, n_materials, n_lights, reserved;
+  const ppm_shape_desc*    shapes;
+  const ppm_material_desc* materials;
+  const ppm_light_desc*    lights;
+  ppm_camera_desc          camera;
+  /* Quantisation box of the photon-map Morton keys.  Photons outside it are
+   * clamped to its faces (the tree stays exact; only its tightness suffers). */
+  double bounds_min[3], bounds_max[3];
+} ppm_scene_desc;
+
+/* PPM-PA pass schedule (util/iterator.rb:26-38). */
+typedef struct {
+  uint64_t seed;
+  uint64_t n_photon;     /* photons per pass (ppmpa.rs:30 'nphoton') */
+  uint32_t first_pass;   /* 1-based index of the first pass to run */
+  uint32_t reserved0;
+  double   radius0;      /* r_1, the radius of pass 1 (not squared) */
+  double   alpha;        /* iterator.rb:17 ALPHA */
+  int32_t  use_classic;  /* ppmpa.rs:16 DEF_USECLASSIC; "-nc" clears it */
+  int32_t  max_depth;    /* tracer.rs:27 MAX_TRACE */
+  int32_t  pfilter;      /* PPM_FILTER_* used by the gather */
+  int32_t  flags;        /* PPM_SESSION_* */
+} ppm_pass_cfg;
+
+#define PPM_SESSION_STAGE_TIMING  1 /* hipEvents around every stage (no graph) */
+#define PPM_SESSION_NO_GRAPH      2 /* launch the pass kernels directly */
+
+typedef struct {
+  double   ms_photon_trace;  /* emission + bouncing (H1, H2) */
+  double   ms_map_build;     /* Morton keys + radix sort + tree (H4) */
+  double   ms_eye_trace;     /* eye rays + direct light (H7) */
+  double   ms_gather;        /* the gather kernel alone: radius query + radiance estimate (H5, H6) */
+  double   ms_accumulate;    /* gather-query Morton sort + FP64 per-pixel accumulation (H8) */
+  double   ms_total;         /* all passes, device time */
+  uint64_t photons_emitted;  /* summed over passes */
+  uint64_t photons_stored;   /* summed over passes */
+  uint64_t gather_queries;   /* gather queries (all eye-tree nodes), summed over passes */
+  uint32_t passes;
+  uint32_t reserved;
+} ppm_timings;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Open a context on HIP device `device`; hip_stream may be NULL (the context
+ * then creates its own default-priority stream). */
+ppm_status  ppm_create(ppm_ctx** out, int device, void* hip_stream);
+void        ppm_destroy(ppm_ctx* ctx);
+const char* ppm_last_error(const ppm_ctx* ctx);
+const char* ppm_status_string(ppm_status s);
+/* Block until all work queued on the context's stream is done. */
+ppm_status  ppm_sync(ppm_ctx* ctx);
+
+/* camera.rs:201-299 read_camera: fill the derived screen vectors of `cam`
+ * from the camera parameters (focal_len in metres, as camera.rs:235). */
+ppm_status  ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso,
+                             const double eye[3], const double target[3],
+                             const double upper[3], double focus,
+                             double focal_len, double f_number);
+
+/* ---- scene -------------------------------------------------------------- */
+
+/* Upload shapes, materials, lights and camera (scene.rs:20 read_scene,
+ * camera.rs:201 read_camera). */
+ppm_status  ppm_scene_create(ppm_ctx* ctx, const ppm_scene_desc* desc,
+                             ppm_scene** out);
+void        ppm_scene_destroy(ppm_scene* scene);
+
+/* ---- photons (H1, H2) ---------------------------------------------------- */
+
+/* A device-resident photon set of fixed capacity. */
+ppm_status  ppm_photons_create(ppm_ctx* ctx, uint64_t capacity,
+                               ppm_photons** out);
+void        ppm_photons_destroy(ppm_photons* ph);
+
+/* Σ_lights Σ_i trace_photon(uc, M_AIR, objs, 0, light.generate_photon())
+ * (tracer.rs:31, light.rs:67, ppmpa.rs:86-92): emit the photons of one pass
+ * and store their diffuse hits into `out` (overwriting it).  Per-light counts
+ * are round(flux_l / (Σflux / n_photon)) (ppmpa.rs:30-31,70-72).  Photon i of
+ * pass p draws its random numbers from the counter-based stream
+ * (seed, p, i); max_depth is MAX_TRACE (tracer.rs:27). */
+ppm_status  ppm_trace_photons(ppm_ctx* ctx, const ppm_scene* scene,
+                              uint64_t seed, uint32_t pass, uint64_t n_photon,
+                              int32_t use_classic, int32_t max_depth,
+                              ppm_photons* out);
+
+/* Stored-photon count (synchronises the stream). */
+ppm_status  ppm_photons_count(ppm_ctx* ctx, const ppm_photons* ph, uint64_t* n);
+
+/* Load a photon set from host memory: the content of the text map that
+ * read_map parses (photonmap.rs:31-74).  pos and dir are n x 3 doubles, wl is
+ * n channel codes.  Directions are re-normalised (geometry.rs:41-48 via
+ * photonmap.rs:68).  The double positions are kept beside the float copy so
+ * that radius queries can be decided exactly in double (refine64). */
+ppm_status  ppm_photons_upload(ppm_ctx* ctx, ppm_photons* ph, uint64_t n,
+                               const double* pos, const double* dir,
+                               const uint8_t* wl);
+
+/* Copy a photon set back to host memory (float values widened to double).
+ * meta may be NULL; otherwise it receives the packed record tag:
+ * bits 0-1 channel, bits 2-6 bounce depth, bits 7-31 photon index. */
+ppm_status  ppm_photons_download(ppm_ctx* ctx, const ppm_photons* ph,
+                                 uint64_t cap, double* pos, double* dir,
+                                 uint8_t* wl, uint32_t* meta, uint64_t* n);
+
+/* ---- photon map (H4, H5, H6) --------------------------------------------- */
+
+ppm_status  ppm_map_create(ppm_ctx* ctx, uint64_t capacity, ppm_map** out);
+void        ppm_map_destroy(ppm_map* map);
+
+/* build_photonmap (photonmap.rs:23-29): Morton keys within the scene's
+ * bounds, radix sort, and a balanced tree of 64-photon leaves.  power is the
+ * per-photon power (PhotonMap.power).  scene may be NULL: the bounds are then
+ * those recorded with the photon set (the bounding box of an uploaded set, or
+ * the bounds of the scene that traced it). */
+ppm_status  ppm_map_build(ppm_ctx* ctx, const ppm_scene* scene,
+                          const ppm_photons* ph, double power, ppm_map* map);
+
+/* KdTree::within(q, r2, squared_euclidean) (tracer.rs:180): for each of the
+ * nq query points (nq x 3 doubles) the indices (into the photon set as it
+ * was given to ppm_map_build) of the photons with squared distance <= r2.
+ * offsets_out has nq + 1 entries; idx_out receives at most idx_cap indices,
+ * each query's set sorted ascending by index.  refine64 = 0 decides in float
+ * ((dx*dx + dy*dy) + dz*dz, no contraction); refine64 = 1 decides in double
+ * as the reference does (needs a set loaded by ppm_photons_upload).
+ * Returns PPM_E_RANGE (and the needed total in offsets_out[nq]) when
+ * idx_cap is too small. */
+ppm_status  ppm_radius_query(ppm_ctx* ctx, const ppm_map* map,
+                             const double* q_xyz, uint64_t nq, double r2,
+                             int32_t refine64, uint64_t idx_cap,
+                             uint32_t* idx_out, uint64_t* offsets_out);
+
+/* estimate_radiance (tracer.rs:179-195) for nq (position, normal) pairs:
+ * (1/(pi r2)) Σ filter(d2, r2) * power * max(0, -n.dir) into the photon's
+ * channel (optics.rs:224-233).  rgb_out is nq x 3 doubles. */
+ppm_status  ppm_gather(ppm_ctx* ctx, const ppm_map* map, const double* q_xyz,
+                       const double* q_nrm, uint64_t nq, double r2,
+                       int32_t pfilter, int32_t refine64, double* rgb_out);
+
+/* ---- eye pass (H7) ------------------------------------------------------ */
+
+/* One pass of per-pixel trace_ray (tracer.rs:129-177) over the camera's
+ * screen (camera.rs:151-168, ppmpa.rs:576-577): rgb_out is yreso x xreso x 3
+ * doubles, row-major (y, then x) as the reference prints it. */
+ppm_status  ppm_trace_eye(ppm_ctx* ctx, const ppm_scene* scene,
+                          const ppm_map* map, uint64_t seed, uint32_t pass,
+                          double r2, int32_t use_classic, int32_t max_depth,
+                          int32_t pfilter, double* rgb_out);
+
+/* ---- PPM-PA passes (H8) -------------------------------------------------- */
+
+/* util/iterator.rb + util/averager2.rb in one device-resident loop: pass i
+ * uses r_i^2 = r_1^2 * prod_{k=1}^{i-1} (k + alpha) / (k + 1) (kept in
+ * double on the device) and the per-pixel radiance is summed in double. */
+ppm_status  ppm_session_create(ppm_ctx* ctx, const ppm_scene* scene,
+                               const ppm_pass_cfg* cfg, ppm_session** out);
+void        ppm_session_destroy(ppm_session* s);
+/* Queue n_pass further passes (asynchronous, stream-ordered). */
+ppm_status  ppm_session_run(ppm_session* s, uint32_t n_pass);
+/* Per-pixel mean over the passes run so far (yreso x xreso x 3 doubles,
+ * row-major) and, when var_out is not NULL, the unbiased per-pixel variance
+ * of the per-pass values.  Synchronises. */
+ppm_status  ppm_session_image(ppm_session* s, double* mean_out,
+                              double* var_out, uint32_t* passes_done);
+/* r_i^2 of the most recent pass, read back from the device (synchronises). */
+ppm_status  ppm_session_radius2(ppm_session* s, double* r2_next);
+ppm_status  ppm_session_timings(ppm_session* s, ppm_timings* t);
+
+/* create + run(cfg->n_pass) + image + timings + destroy. */
+ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
+                           const ppm_pass_cfg* cfg, uint32_t n_pass,
+                           double* mean_out, double* var_out, ppm_timings* t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPM_H */

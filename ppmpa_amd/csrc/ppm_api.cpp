@@ -1,0 +1,1067 @@
+// ppm_api.cpp — the C ABI of include/ppm.h over the gfx950 kernels.
+//
+// Host responsibilities: validate inputs (the reference panics where this
+// returns a status), lay scenes out for the device (ppm_layout.hpp), size
+// buffers, order launches on the context's stream and capture one PPM-PA
+// pass into a hipGraph that is replayed for every pass.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ppm.h"
+#include "ppm_launch.hpp"
+#include "ppm_layout.hpp"
+
+using namespace ppm;
+
+struct ppm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int n_cu = 256;
+  std::string err;
+};
+
+struct ppm_scene {
+  ppm_ctx* ctx = nullptr;
+  SceneBuf buf;
+  std::vector<DevLight> lights;  // host copy (photon counts are filled per call)
+  double total_flux = 0.0;
+  double bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
+};
+
+struct ppm_photons {
+  ppm_ctx* ctx = nullptr;
+  uint64_t cap = 0;
+  PhotonBuf pb;
+  PassState* st = nullptr;
+  bool has64 = false;
+  bool has_bounds = false;
+  double bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0};
+  double power = 1.0;
+};
+
+struct ppm_map {
+  ppm_ctx* ctx = nullptr;
+  MapBuf m;
+  PassState* st = nullptr;
+  double power = 1.0;
+  bool has64 = false;
+  bool built = false;
+};
+
+struct ppm_session {
+  ppm_ctx* ctx = nullptr;
+  const ppm_scene* scene = nullptr;
+  ppm_pass_cfg cfg{};
+  SceneBuf sbuf;
+  DevLight* lights = nullptr;
+  PhotonBuf pb;
+  MapBuf map;
+  EyeBuf eb;
+  double* sum = nullptr;
+  double* sumsq = nullptr;
+  PassState* st = nullptr;
+  double power = 0.0;
+  uint32_t passes_done = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[10] = {};
+  ppm_timings t{};
+};
+
+namespace {
+
+ppm_status fail(ppm_ctx* c, ppm_status s, const std::string& msg) {
+  if (c) c->err = msg;
+  return s;
+}
+
+#define HIP_TRY(ctx, expr)                                                                  \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP,                 \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <typename T>
+void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+float4 f4(const double v[3], float w = 0.0f) { return make_float4((float)v[0], (float)v[1], (float)v[2], w); }
+
+bool finite3(const double v[3]) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - b[1] * a[2];
+  o[1] = a[2] * b[0] - b[2] * a[0];
+  o[2] = a[0] * b[1] - b[0] * a[1];
+}
+bool norm3(const double a[3], double o[3]) {
+  double n = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  if (n == 0.0) return false;
+  double inv = 1.0 / n;
+  o[0] = a[0] * inv;
+  o[1] = a[1] * inv;
+  o[2] = a[2] * inv;
+  return true;
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint32_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// per-light photon counts (ppmpa.rs:30-31,65-72): power = Σ flux / n; n_l = round(flux_l / power)
+uint64_t assign_photons(std::vector<DevLight>& L, double total_flux, uint64_t n_photon, double* power) {
+  double pw = total_flux / (double)n_photon;
+  *power = pw;
+  uint64_t first = 0;
+  for (auto& l : L) {
+    double nl = std::round((double)l.flux / pw);
+    uint64_t c = (nl > 0 && std::isfinite(nl)) ? (uint64_t)nl : 0;
+    l.first_photon = (uint32_t)first;
+    l.n_photon = (uint32_t)c;
+    first += c;
+  }
+  return first;
+}
+
+ppm_status check_cfg_depth(ppm_ctx* c, int max_depth) {
+  if (max_depth < 1 || max_depth > MAX_DEPTH_LIMIT)
+    return fail(c, PPM_E_INVALID, "max_depth must be in [1, 16]");
+  return PPM_OK;
+}
+
+hipError_t map_alloc(MapBuf& m, uint64_t cap, int n_cu) {
+  m.cap = cap;
+  uint64_t leaves = (cap + LEAF - 1) / LEAF;
+  m.lpad_max = next_pow2(leaves > 0 ? leaves : 1);
+  m.sort_blocks = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * 4,
+                                                                                                (cap + 2047) / 2048)));
+  hipError_t e;
+  if ((e = dalloc(&m.sorted, cap)) != hipSuccess) return e;
+  if ((e = dalloc(&m.src_idx, cap)) != hipSuccess) return e;
+  if ((e = dalloc(&m.nodes, 2ull * m.lpad_max)) != hipSuccess) return e;
+  for (int i = 0; i < 2; ++i) {
+    if ((e = dalloc(&m.keys[i], cap)) != hipSuccess) return e;
+    if ((e = dalloc(&m.vals[i], cap)) != hipSuccess) return e;
+  }
+  if ((e = dalloc(&m.ghist, 256ull * m.sort_blocks)) != hipSuccess) return e;
+  if ((e = dalloc(&m.totals, 256)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+void map_free(MapBuf& m) {
+  dfree(m.sorted);
+  dfree(m.sorted64);
+  dfree(m.src_idx);
+  dfree(m.nodes);
+  for (int i = 0; i < 2; ++i) {
+    dfree(m.keys[i]);
+    dfree(m.vals[i]);
+  }
+  dfree(m.ghist);
+  dfree(m.totals);
+}
+
+hipError_t photons_alloc(PhotonBuf& pb, uint64_t cap, uint32_t max_regions) {
+  pb.cap = cap;
+  pb.max_regions = max_regions;
+  hipError_t e;
+  if ((e = dalloc(&pb.rec, cap)) != hipSuccess) return e;
+  if ((e = dalloc(&pb.count, max_regions)) != hipSuccess) return e;
+  if ((e = dalloc(&pb.offset, max_regions)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+void photons_free(PhotonBuf& pb) {
+  dfree(pb.rec);
+  dfree(pb.rec64);
+  dfree(pb.count);
+  dfree(pb.offset);
+}
+
+// Region layout of a traced pass: one region per tracing wave.
+bool photons_layout(PhotonBuf& pb, uint64_t n_photon, int max_depth, int n_cu) {
+  const uint32_t w = trace_waves(n_photon, n_cu);
+  const uint64_t per = (n_photon + w - 1) / w;
+  const uint64_t rc = per * (uint64_t)max_depth;
+  if (w > pb.max_regions || rc > 0xFFFFFFFFull || (uint64_t)w * rc > pb.cap) return false;
+  pb.n_regions = w;
+  pb.region_cap = (uint32_t)rc;
+  return true;
+}
+
+uint64_t photons_needed(uint64_t n_photon, int max_depth, int n_cu) {
+  const uint32_t w = trace_waves(n_photon, n_cu);
+  const uint64_t per = (n_photon + w - 1) / w;
+  return (uint64_t)w * per * (uint64_t)max_depth;
+}
+
+hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) {
+  const uint32_t tiles = (uint32_t)cam.tiles_x * (uint32_t)((cam.yreso + 7) / 8);
+  eb.q_primary = tiles * 64u;
+  eb.n_tiles = tiles;
+  // deeper queries kept per pixel: a glass sphere yields about one per depth
+  eb.sec_max = (uint32_t)std::max(1, std::min(2 * max_depth, 32));
+  eb.n_pix = (uint32_t)cam.xreso * (uint32_t)cam.yreso;
+  eb.ovf_cap = 2u * eb.n_pix + 1024u;
+  const uint64_t inplace = (uint64_t)eb.q_primary * (1u + eb.sec_max);
+  eb.q_sort_blocks = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, (uint64_t)n_cu * 4));
+  hipError_t e;
+  if ((e = dalloc(&eb.q, inplace + eb.ovf_cap)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.sec_count, eb.q_primary)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.pix, eb.n_pix)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.tile_count, tiles)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.tile_off, tiles)) != hipSuccess) return e;
+  for (int i = 0; i < 2; ++i) {
+    if ((e = dalloc(&eb.qkeys[i], inplace)) != hipSuccess) return e;
+    if ((e = dalloc(&eb.qvals[i], inplace)) != hipSuccess) return e;
+  }
+  if ((e = dalloc(&eb.qghist, 256ull * eb.q_sort_blocks)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.qtotals, 256)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.res, inplace)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+void eye_free(EyeBuf& eb) {
+  dfree(eb.q);
+  dfree(eb.sec_count);
+  dfree(eb.pix);
+  dfree(eb.tile_count);
+  dfree(eb.tile_off);
+  for (int i = 0; i < 2; ++i) {
+    dfree(eb.qkeys[i]);
+    dfree(eb.qvals[i]);
+  }
+  dfree(eb.qghist);
+  dfree(eb.qtotals);
+  dfree(eb.res);
+}
+
+PassState host_state() {
+  PassState s;
+  std::memset(&s, 0, sizeof(s));
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ppm_status_string(ppm_status s) {
+  switch (s) {
+    case PPM_OK: return "ok";
+    case PPM_E_INVALID: return "invalid argument";
+    case PPM_E_NONFINITE: return "non-finite coordinate";
+    case PPM_E_OOM: return "out of memory";
+    case PPM_E_HIP: return "HIP runtime error";
+    case PPM_E_RANGE: return "size beyond capacity";
+    case PPM_E_PARSE: return "malformed photon map";
+    case PPM_E_NODEVICE: return "no usable device";
+    default: return "unknown status";
+  }
+}
+
+ppm_status ppm_create(ppm_ctx** out, int device, void* hip_stream) {
+  if (!out) return PPM_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return PPM_E_NODEVICE;
+  if (device < 0 || device >= n) return PPM_E_NODEVICE;
+  if (hipSetDevice(device) != hipSuccess) return PPM_E_NODEVICE;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PPM_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PPM_E_NODEVICE;
+  ppm_ctx* c = new ppm_ctx();
+  c->device = device;
+  c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  if (hip_stream) {
+    c->stream = (hipStream_t)hip_stream;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      return PPM_E_HIP;
+    }
+    c->own_stream = true;
+  }
+  *out = c;
+  return PPM_OK;
+}
+
+void ppm_destroy(ppm_ctx* c) {
+  if (!c) return;
+  if (c->own_stream && c->stream) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+  }
+  delete c;
+}
+
+const char* ppm_last_error(const ppm_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+ppm_status ppm_sync(ppm_ctx* c) {
+  if (!c) return PPM_E_INVALID;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return PPM_OK;
+}
+
+ppm_status ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso, const double eye[3],
+                            const double target[3], const double upper[3], double focus, double focal_len,
+                            double f_number) {
+  if (!cam || !eye || !target || !upper || xreso <= 0 || yreso <= 0 || focal_len == 0.0 || f_number == 0.0)
+    return PPM_E_INVALID;
+  // camera.rs:244-256
+  double d[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
+  double ez[3], ex[3], ey[3], t[3];
+  if (!norm3(d, ez)) return PPM_E_INVALID;
+  cross3(upper, ez, t);
+  if (!norm3(t, ex)) return PPM_E_INVALID;
+  cross3(ex, ez, t);
+  if (!norm3(t, ey)) return PPM_E_INVALID;
+  const double step = (focus * 0.035 / focal_len) / (double)xreso;
+  const double ea = focal_len / f_number;
+  const double lx = (double)(xreso / 2), ly = (double)(yreso / 2);
+  cam->xreso = xreso;
+  cam->yreso = yreso;
+  for (int i = 0; i < 3; ++i) {
+    cam->eye_pos[i] = eye[i];
+    cam->esx[i] = step * ex[i];
+    cam->esy[i] = step * ey[i];
+    cam->eex[i] = ea * ex[i];
+    cam->eey[i] = ea * ey[i];
+  }
+  for (int i = 0; i < 3; ++i) cam->origin[i] = focus * ez[i] - (lx - 0.5) * cam->esx[i] - (ly - 0.5) * cam->esy[i];
+  return PPM_OK;
+}
+
+ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out) {
+  if (!c || !d || !out) return fail(c, PPM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (d->n_shapes < 0 || d->n_materials < 0 || d->n_lights < 0 || d->n_materials > 2046)
+    return fail(c, PPM_E_INVALID, "bad counts (at most 2046 materials)");
+  if ((d->n_shapes && !d->shapes) || (d->n_materials && !d->materials) || (d->n_lights && !d->lights))
+    return fail(c, PPM_E_INVALID, "null array");
+  if (d->camera.xreso <= 0 || d->camera.yreso <= 0) return fail(c, PPM_E_INVALID, "bad camera resolution");
+  if ((uint64_t)((d->camera.xreso + 7) / 8) * (uint64_t)((d->camera.yreso + 7) / 8) > 65536)
+    return fail(c, PPM_E_RANGE, "camera resolution beyond 65536 8x8 tiles");
+  std::vector<DevShape> shapes(d->n_shapes);
+  for (int i = 0; i < d->n_shapes; ++i) {
+    const ppm_shape_desc& s = d->shapes[i];
+    DevShape& o = shapes[i];
+    std::memset(&o, 0, sizeof(o));
+    if (s.kind < 0 || s.kind > 4) return fail(c, PPM_E_INVALID, "bad shape kind");
+    if (s.material < 0 || s.material >= d->n_materials) return fail(c, PPM_E_INVALID, "shape material out of range");
+    if (!finite3(s.pos) || !finite3(s.nvec) || !finite3(s.dir1) || !finite3(s.dir2) || !std::isfinite(s.dist) ||
+        !std::isfinite(s.radius))
+      return fail(c, PPM_E_NONFINITE, "non-finite shape parameter");
+    o.kind = (uint32_t)s.kind;
+    o.mat = (uint32_t)s.material;
+    o.dist = (float)s.dist;
+    o.radius = (float)s.radius;
+    o.p = f4(s.pos);
+    o.d1 = f4(s.dir1);
+    o.d2 = f4(s.dir2);
+    double n[3] = {s.nvec[0], s.nvec[1], s.nvec[2]};
+    if ((s.kind == PPM_SHAPE_POLYGON || s.kind == PPM_SHAPE_PARALLELOGRAM) && n[0] == 0 && n[1] == 0 && n[2] == 0) {
+      // Shape::new_polygon / new_parallelogram (geometry.rs:93-115)
+      double cr[3];
+      cross3(s.dir1, s.dir2, cr);
+      if (!norm3(cr, n)) return fail(c, PPM_E_INVALID, "degenerate polygon");
+    }
+    o.n = f4(n);
+  }
+  std::vector<DevMaterial> mats(d->n_materials + 1);
+  for (int i = 0; i <= d->n_materials; ++i) {
+    DevMaterial& o = mats[i];
+    std::memset(&o, 0, sizeof(o));
+    if (i == d->n_materials) {  // M_AIR (scene.rs:13-18)
+      o.surface = SURF_NOTHING;
+      o.ior = make_float4(1.f, 1.f, 1.f, 0.f);
+      o.ior_avg = 1.0f;
+      continue;
+    }
+    const ppm_material_desc& m = d->materials[i];
+    if (m.surface < 0 || m.surface > 4) return fail(c, PPM_E_INVALID, "bad surface kind");
+    o.surface = (uint32_t)m.surface;
+    o.diff = (float)m.diffuseness;
+    o.metal = (float)m.metalness;
+    o.rough = (float)m.roughness;
+    o.density_pow = (float)(1.0 / (std::pow(10.0, 5.0 * (1.0 - std::sqrt(m.roughness))) + 1.0));
+    o.alpha = (float)(m.roughness * m.roughness * m.roughness * m.roughness);
+    o.ior_avg = (float)((m.ior[0] + m.ior[1] + m.ior[2]) / 3.0);
+    o.em = f4(m.emittance);
+    o.ior = f4(m.ior);
+    o.c1 = f4(m.color1);
+    o.c2 = f4(m.color2);
+  }
+  ppm_scene* sc = new ppm_scene();
+  sc->ctx = c;
+  sc->lights.resize(d->n_lights);
+  for (int i = 0; i < d->n_lights; ++i) {
+    const ppm_light_desc& l = d->lights[i];
+    DevLight& o = sc->lights[i];
+    std::memset(&o, 0, sizeof(o));
+    if (l.kind < 0 || l.kind > 2 || !(l.flux >= 0.0)) {
+      delete sc;
+      return fail(c, PPM_E_INVALID, "bad light");
+    }
+    o.kind = (uint32_t)l.kind;
+    o.flux = (float)l.flux;
+    o.color = f4(l.color);
+    o.pos = f4(l.pos);
+    o.nvec = f4(l.nvec);
+    o.d1 = f4(l.dir1);
+    o.d2 = f4(l.dir2);
+    double dn[3];
+    o.dir = norm3(l.dir, dn) ? f4(dn) : f4(l.dir);
+    sc->total_flux += l.flux;
+  }
+  SceneBuf& b = sc->buf;
+  b.n_shapes = (uint32_t)d->n_shapes;
+  b.n_mats = (uint32_t)d->n_materials;
+  b.n_lights = (uint32_t)d->n_lights;
+  const ppm_camera_desc& cd = d->camera;
+  DevCamera& cam = b.cam;
+  std::memset(&cam, 0, sizeof(cam));
+  cam.xreso = cd.xreso;
+  cam.yreso = cd.yreso;
+  cam.progressive = cd.progressive;
+  cam.antialias = cd.antialias;
+  cam.blur = cd.blur;
+  cam.pfilter = cd.pfilter;
+  cam.tiles_x = (cd.xreso + 7) / 8;
+  cam.eye = f4(cd.eye_pos);
+  cam.origin = f4(cd.origin);
+  cam.esx = f4(cd.esx);
+  cam.esy = f4(cd.esy);
+  cam.eex = f4(cd.eex);
+  cam.eey = f4(cd.eey);
+  for (int i = 0; i < 3; ++i) {
+    sc->bmin[i] = d->bounds_min[i];
+    sc->bmax[i] = d->bounds_max[i];
+    if (!(sc->bmax[i] > sc->bmin[i]) || !std::isfinite(sc->bmin[i]) || !std::isfinite(sc->bmax[i])) {
+      delete sc;
+      return fail(c, PPM_E_INVALID, "bad photon-map bounds");
+    }
+  }
+  b.bmin = f4(sc->bmin);
+  b.bmax = f4(sc->bmax);
+  hipError_t e = hipSuccess;
+  if ((e = dalloc(&b.shapes, shapes.size())) == hipSuccess && (e = dalloc(&b.mats, mats.size())) == hipSuccess &&
+      (e = dalloc(&b.lights, sc->lights.size())) == hipSuccess) {
+    if (!shapes.empty()) e = hipMemcpy(b.shapes, shapes.data(), sizeof(DevShape) * shapes.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b.mats, mats.data(), sizeof(DevMaterial) * mats.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !sc->lights.empty())
+      e = hipMemcpy(b.lights, sc->lights.data(), sizeof(DevLight) * sc->lights.size(), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    ppm_scene_destroy(sc);
+    return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  }
+  *out = sc;
+  return PPM_OK;
+}
+
+void ppm_scene_destroy(ppm_scene* sc) {
+  if (!sc) return;
+  if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
+  dfree(sc->buf.shapes);
+  dfree(sc->buf.mats);
+  dfree(sc->buf.lights);
+  delete sc;
+}
+
+// ---------------------------------------------------------------------------
+// photons
+
+ppm_status ppm_photons_create(ppm_ctx* c, uint64_t capacity, ppm_photons** out) {
+  if (!c || !out) return fail(c, PPM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (capacity == 0 || capacity > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "capacity must be in [1, 2^31)");
+  ppm_photons* p = new ppm_photons();
+  p->ctx = c;
+  p->cap = capacity;
+  hipError_t e = photons_alloc(p->pb, capacity, 8192);
+  if (e == hipSuccess) e = dalloc(&p->st, 1);
+  if (e != hipSuccess) {
+    ppm_photons_destroy(p);
+    return fail(c, PPM_E_OOM, hipGetErrorString(e));
+  }
+  PassState s = host_state();
+  uint32_t zero = 0;
+  if ((e = hipMemcpy(p->st, &s, sizeof(s), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->pb.count, &zero, sizeof(zero), hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(p->pb.offset, &zero, sizeof(zero), hipMemcpyHostToDevice)) != hipSuccess) {
+    ppm_photons_destroy(p);
+    return fail(c, PPM_E_HIP, hipGetErrorString(e));
+  }
+  p->pb.n_regions = 1;
+  p->pb.region_cap = (uint32_t)capacity;
+  *out = p;
+  return PPM_OK;
+}
+
+void ppm_photons_destroy(ppm_photons* p) {
+  if (!p) return;
+  if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+  photons_free(p->pb);
+  dfree(p->st);
+  delete p;
+}
+
+ppm_status ppm_trace_photons(ppm_ctx* c, const ppm_scene* sc, uint64_t seed, uint32_t pass, uint64_t n_photon,
+                             int32_t use_classic, int32_t max_depth, ppm_photons* out) {
+  if (!c || !sc || !out) return fail(c, PPM_E_INVALID, "null argument");
+  if (ppm_status s = check_cfg_depth(c, max_depth)) return s;
+  if (n_photon == 0 || n_photon > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "n_photon must be in [1, 2^31)");
+  if (sc->lights.empty() || !(sc->total_flux > 0.0)) return fail(c, PPM_E_INVALID, "scene has no emitting light");
+  ppm_scene* s = const_cast<ppm_scene*>(sc);
+  double power = 0;
+  uint64_t total = assign_photons(s->lights, s->total_flux, n_photon, &power);
+  if (total == 0 || total > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "per-light photon counts out of range");
+  if (!photons_layout(out->pb, total, max_depth, c->n_cu))
+    return fail(c, PPM_E_RANGE, "photon set capacity too small: need " +
+                                    std::to_string(photons_needed(total, max_depth, c->n_cu)));
+  HIP_TRY(c, hipMemcpyAsync(s->buf.lights, s->lights.data(), sizeof(DevLight) * s->lights.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  SceneBuf b = s->buf;
+  b.n_emit_total = (uint32_t)total;
+  dfree(out->pb.rec64);
+  out->has64 = false;
+  out->power = power;
+  out->has_bounds = true;
+  std::memcpy(out->bmin, s->bmin, sizeof(out->bmin));
+  std::memcpy(out->bmax, s->bmax, sizeof(out->bmax));
+  PassState hs = host_state();
+  hs.pass = pass;
+  hs.power = power;
+  HIP_TRY(c, hipMemcpyAsync(out->st, &hs, sizeof(hs), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, launch_trace_photons(c->stream, b, out->st, out->pb, seed, use_classic, max_depth));
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, out->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (h.overflow) return fail(c, PPM_E_RANGE, "photon region capacity exceeded");
+  return PPM_OK;
+}
+
+ppm_status ppm_photons_count(ppm_ctx* c, const ppm_photons* p, uint64_t* n) {
+  if (!c || !p || !n) return fail(c, PPM_E_INVALID, "null argument");
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, p->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *n = h.n_stored;
+  return PPM_OK;
+}
+
+ppm_status ppm_photons_upload(ppm_ctx* c, ppm_photons* p, uint64_t n, const double* pos, const double* dir,
+                              const uint8_t* wl) {
+  if (!c || !p || (n && (!pos || !dir || !wl))) return fail(c, PPM_E_INVALID, "null argument");
+  if (n > p->cap) return fail(c, PPM_E_RANGE, "photon set capacity exceeded");
+  std::vector<PhotonRec> recs(n);
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* pp = pos + 3 * i;
+    const double* dd = dir + 3 * i;
+    if (!finite3(pp) || !finite3(dd)) return fail(c, PPM_E_NONFINITE, "non-finite photon " + std::to_string(i));
+    if (wl[i] > 2) return fail(c, PPM_E_INVALID, "bad channel code at photon " + std::to_string(i));
+    double dn[3];
+    if (!norm3(dd, dn)) return fail(c, PPM_E_INVALID, "zero direction at photon " + std::to_string(i));
+    recs[i].pm = make_float4((float)pp[0], (float)pp[1], (float)pp[2],
+                             __builtin_bit_cast(float, pack_meta(wl[i], 0, (uint32_t)i)));
+    recs[i].dm = make_float4((float)dn[0], (float)dn[1], (float)dn[2], 0.0f);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], pp[k]);
+      hi[k] = std::max(hi[k], pp[k]);
+    }
+  }
+  if (!p->pb.rec64) HIP_TRY(c, dalloc(&p->pb.rec64, 3 * p->cap));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (n) {
+    HIP_TRY(c, hipMemcpy(p->pb.rec, recs.data(), sizeof(PhotonRec) * n, hipMemcpyHostToDevice));
+    HIP_TRY(c, hipMemcpy(p->pb.rec64, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  }
+  uint32_t cnt = (uint32_t)n, zero = 0;
+  HIP_TRY(c, hipMemcpy(p->pb.count, &cnt, sizeof(cnt), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(p->pb.offset, &zero, sizeof(zero), hipMemcpyHostToDevice));
+  p->pb.n_regions = 1;
+  p->pb.region_cap = (uint32_t)p->cap;
+  PassState s = host_state();
+  s.n_stored = (uint32_t)n;
+  HIP_TRY(c, hipMemcpy(p->st, &s, sizeof(s), hipMemcpyHostToDevice));
+  p->has64 = true;
+  p->has_bounds = true;
+  for (int k = 0; k < 3; ++k) {
+    if (n == 0) {
+      lo[k] = 0.0;
+      hi[k] = 1.0;
+    }
+    double pad = std::max(1e-6, (hi[k] - lo[k]) * 1e-6);
+    p->bmin[k] = lo[k] - pad;
+    p->bmax[k] = hi[k] + pad;
+  }
+  return PPM_OK;
+}
+
+ppm_status ppm_photons_download(ppm_ctx* c, const ppm_photons* p, uint64_t cap, double* pos, double* dir,
+                                uint8_t* wl, uint32_t* meta, uint64_t* n) {
+  if (!c || !p || !n) return fail(c, PPM_E_INVALID, "null argument");
+  uint64_t cnt = 0;
+  if (ppm_status s = ppm_photons_count(c, p, &cnt)) return s;
+  *n = cnt;
+  if (cnt > cap) return fail(c, PPM_E_RANGE, "output capacity too small");
+  if (cnt && (!pos || !dir || !wl)) return fail(c, PPM_E_INVALID, "null output array");
+  std::vector<PhotonRec> recs(cnt);
+  if (cnt) {
+    PhotonRec* dense = nullptr;
+    HIP_TRY(c, dalloc(&dense, cnt));
+    hipError_t e = launch_compact(c->stream, p->pb, p->st, dense);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = hipMemcpy(recs.data(), dense, sizeof(PhotonRec) * cnt, hipMemcpyDeviceToHost);
+    dfree(dense);
+    if (e != hipSuccess) return fail(c, PPM_E_HIP, hipGetErrorString(e));
+  }
+  for (uint64_t i = 0; i < cnt; ++i) {
+    uint32_t m = __builtin_bit_cast(uint32_t, recs[i].pm.w);
+    pos[3 * i] = recs[i].pm.x;
+    pos[3 * i + 1] = recs[i].pm.y;
+    pos[3 * i + 2] = recs[i].pm.z;
+    dir[3 * i] = recs[i].dm.x;
+    dir[3 * i + 1] = recs[i].dm.y;
+    dir[3 * i + 2] = recs[i].dm.z;
+    wl[i] = (uint8_t)(m & 3u);
+    if (meta) meta[i] = m;
+  }
+  return PPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// map
+
+ppm_status ppm_map_create(ppm_ctx* c, uint64_t capacity, ppm_map** out) {
+  if (!c || !out) return fail(c, PPM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (capacity == 0 || capacity > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "capacity must be in [1, 2^31)");
+  ppm_map* m = new ppm_map();
+  m->ctx = c;
+  hipError_t e = map_alloc(m->m, capacity, c->n_cu);
+  if (e == hipSuccess) e = dalloc(&m->st, 1);
+  if (e != hipSuccess) {
+    ppm_map_destroy(m);
+    return fail(c, PPM_E_OOM, hipGetErrorString(e));
+  }
+  *out = m;
+  return PPM_OK;
+}
+
+void ppm_map_destroy(ppm_map* m) {
+  if (!m) return;
+  if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
+  map_free(m->m);
+  dfree(m->st);
+  delete m;
+}
+
+ppm_status ppm_map_build(ppm_ctx* c, const ppm_scene* sc, const ppm_photons* ph, double power, ppm_map* m) {
+  if (!c || !ph || !m) return fail(c, PPM_E_INVALID, "null argument");
+  if (ph->cap > m->m.cap) return fail(c, PPM_E_RANGE, "map capacity smaller than the photon set's");
+  const double* lo = sc ? sc->bmin : ph->bmin;
+  const double* hi = sc ? sc->bmax : ph->bmax;
+  if (!sc && !ph->has_bounds) return fail(c, PPM_E_INVALID, "no bounds for the Morton keys");
+  if (ph->has64 && !m->m.sorted64) HIP_TRY(c, dalloc(&m->m.sorted64, 3 * m->m.cap));
+  HIP_TRY(c, hipMemcpyAsync(m->st, ph->st, sizeof(PassState), hipMemcpyDeviceToDevice, c->stream));
+  PhotonBuf in = ph->pb;
+  if (!ph->has64) in.rec64 = nullptr;
+  HIP_TRY(c, launch_map_build(c->stream, m->m, m->st, in, f4(lo), f4(hi)));
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, m->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  m->power = power;
+  m->has64 = ph->has64;
+  m->built = true;
+  if (h.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
+  return PPM_OK;
+}
+
+static ppm_status run_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq,
+                            double r2, int refine64, int mode, int pfilter, std::vector<uint32_t>* counts,
+                            const std::vector<uint64_t>* offsets, uint32_t* idx_out, uint64_t idx_total,
+                            std::vector<float4>* rgb) {
+  std::vector<float4> qf(nq), qn;
+  for (uint64_t i = 0; i < nq; ++i) {
+    if (!finite3(q_xyz + 3 * i)) return fail(c, PPM_E_NONFINITE, "non-finite query point");
+    qf[i] = make_float4((float)q_xyz[3 * i], (float)q_xyz[3 * i + 1], (float)q_xyz[3 * i + 2], 0.0f);
+  }
+  if (q_nrm) {
+    qn.resize(nq);
+    for (uint64_t i = 0; i < nq; ++i)
+      qn[i] = make_float4((float)q_nrm[3 * i], (float)q_nrm[3 * i + 1], (float)q_nrm[3 * i + 2], 0.0f);
+  }
+  float4 *dqf = nullptr, *dqn = nullptr, *drgb = nullptr;
+  double* dq64 = nullptr;
+  uint32_t *dcnt = nullptr, *didx = nullptr;
+  uint64_t* doff = nullptr;
+  hipError_t e = hipSuccess;
+  auto cleanup = [&]() {
+    dfree(dqf); dfree(dqn); dfree(drgb); dfree(dq64); dfree(dcnt); dfree(didx); dfree(doff);
+  };
+  do {
+    if ((e = dalloc(&dqf, nq)) != hipSuccess) break;
+    if ((e = hipMemcpy(dqf, qf.data(), sizeof(float4) * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if (refine64) {
+      if ((e = dalloc(&dq64, 3 * nq)) != hipSuccess) break;
+      if ((e = hipMemcpy(dq64, q_xyz, sizeof(double) * 3 * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+    }
+    if (q_nrm) {
+      if ((e = dalloc(&dqn, nq)) != hipSuccess) break;
+      if ((e = hipMemcpy(dqn, qn.data(), sizeof(float4) * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+    }
+    if (mode == 0 && (e = dalloc(&dcnt, nq)) != hipSuccess) break;
+    if (mode == 1) {
+      if ((e = dalloc(&doff, nq)) != hipSuccess) break;
+      if ((e = hipMemcpy(doff, offsets->data(), sizeof(uint64_t) * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+      if ((e = dalloc(&didx, idx_total)) != hipSuccess) break;
+    }
+    if (mode == 2 && (e = dalloc(&drgb, nq)) != hipSuccess) break;
+    if ((e = launch_query(c->stream, m->m, m->st, dqf, dq64, dqn, (uint32_t)nq, r2, refine64, mode, pfilter, dcnt,
+                          doff, didx, drgb, c->n_cu)) != hipSuccess)
+      break;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
+    if (mode == 0) {
+      counts->resize(nq);
+      e = hipMemcpy(counts->data(), dcnt, sizeof(uint32_t) * nq, hipMemcpyDeviceToHost);
+    } else if (mode == 1) {
+      if (idx_total) e = hipMemcpy(idx_out, didx, sizeof(uint32_t) * idx_total, hipMemcpyDeviceToHost);
+    } else {
+      rgb->resize(nq);
+      e = hipMemcpy(rgb->data(), drgb, sizeof(float4) * nq, hipMemcpyDeviceToHost);
+    }
+  } while (0);
+  cleanup();
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  return PPM_OK;
+}
+
+ppm_status ppm_radius_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, uint64_t nq, double r2,
+                            int32_t refine64, uint64_t idx_cap, uint32_t* idx_out, uint64_t* offsets_out) {
+  if (!c || !m || (nq && !q_xyz) || !offsets_out) return fail(c, PPM_E_INVALID, "null argument");
+  if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
+  if (refine64 && !m->has64) return fail(c, PPM_E_INVALID, "refine64 needs a set loaded by ppm_photons_upload");
+  if (!(r2 >= 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
+  if (nq > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "too many queries");
+  offsets_out[0] = 0;
+  if (nq == 0) return PPM_OK;
+  std::vector<uint32_t> counts;
+  if (ppm_status s = run_query(c, m, q_xyz, nullptr, nq, r2, refine64, 0, 0, &counts, nullptr, nullptr, 0, nullptr))
+    return s;
+  std::vector<uint64_t> off(nq);
+  uint64_t tot = 0;
+  for (uint64_t i = 0; i < nq; ++i) {
+    off[i] = tot;
+    offsets_out[i] = tot;
+    tot += counts[i];
+  }
+  offsets_out[nq] = tot;
+  if (tot > idx_cap) return fail(c, PPM_E_RANGE, "idx_cap too small: need " + std::to_string(tot));
+  if (tot && !idx_out) return fail(c, PPM_E_INVALID, "null idx_out");
+  if (ppm_status s = run_query(c, m, q_xyz, nullptr, nq, r2, refine64, 1, 0, nullptr, &off, idx_out, tot, nullptr))
+    return s;
+  for (uint64_t i = 0; i < nq; ++i) std::sort(idx_out + off[i], idx_out + off[i] + counts[i]);
+  return PPM_OK;
+}
+
+ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq, double r2,
+                      int32_t pfilter, int32_t refine64, double* rgb_out) {
+  if (!c || !m || (nq && (!q_xyz || !q_nrm || !rgb_out))) return fail(c, PPM_E_INVALID, "null argument");
+  if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
+  if (refine64 && !m->has64) return fail(c, PPM_E_INVALID, "refine64 needs a set loaded by ppm_photons_upload");
+  if (!(r2 > 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
+  if (pfilter < 0 || pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
+  if (nq > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "too many queries");
+  if (nq == 0) return PPM_OK;
+  std::vector<float4> rgb;
+  if (ppm_status s = run_query(c, m, q_xyz, q_nrm, nq, r2, refine64, 2, pfilter, nullptr, nullptr, nullptr, 0, &rgb))
+    return s;
+  // tracer.rs:191-193: Σ (wt * power) * max(0, -cos) then * (1/pi / r2)
+  const double k = m->power * (1.0 / M_PI) / r2;
+  for (uint64_t i = 0; i < nq; ++i) {
+    rgb_out[3 * i] = (double)rgb[i].x * k;
+    rgb_out[3 * i + 1] = (double)rgb[i].y * k;
+    rgb_out[3 * i + 2] = (double)rgb[i].z * k;
+  }
+  return PPM_OK;
+}
+
+ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint64_t seed, uint32_t pass, double r2,
+                         int32_t use_classic, int32_t max_depth, int32_t pfilter, double* rgb_out) {
+  if (!c || !sc || !m || !rgb_out) return fail(c, PPM_E_INVALID, "null argument");
+  if (ppm_status s = check_cfg_depth(c, max_depth)) return s;
+  if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
+  if (!(r2 > 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
+  if (pfilter < 0 || pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
+  EyeBuf eb;
+  PassState* st = nullptr;
+  hipError_t e = hipSuccess;
+  std::vector<float4> h;
+  PassState hs;
+  do {
+    if ((e = eye_alloc(eb, sc->buf.cam, max_depth, c->n_cu)) != hipSuccess) break;
+    if ((e = dalloc(&st, 1)) != hipSuccess) break;
+    if ((e = hipMemcpyAsync(&hs, m->st, sizeof(PassState), hipMemcpyDeviceToHost, c->stream)) != hipSuccess) break;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
+    hs.pass = pass;
+    hs.r2 = r2;
+    hs.power = m->power;
+    hs.r2f = (float)r2;
+    hs.scale = (float)(m->power / (M_PI * r2));
+    hs.n_queries = 0;
+    hs.overflow = 0;
+    if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth)) != hipSuccess) break;
+    if ((e = launch_gather(c->stream, m->m, st, eb, sc->buf.cam, f4(sc->bmin), f4(sc->bmax), pfilter, c->n_cu,
+                           nullptr, nullptr)) != hipSuccess)
+      break;
+    h.resize(eb.n_pix);
+    if ((e = hipMemcpyAsync(h.data(), eb.pix, sizeof(float4) * eb.n_pix, hipMemcpyDeviceToHost, c->stream)) !=
+        hipSuccess)
+      break;
+    if ((e = hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, c->stream)) != hipSuccess) break;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
+  } while (0);
+  eye_free(eb);
+  dfree(st);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  if (hs.overflow) return fail(c, PPM_E_RANGE, "gather query buffer overflow");
+  for (size_t p = 0; p < h.size(); ++p) {
+    rgb_out[3 * p] = h[p].x;
+    rgb_out[3 * p + 1] = h[p].y;
+    rgb_out[3 * p + 2] = h[p].z;
+  }
+  return PPM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// session
+
+void ppm_session_destroy(ppm_session* s) {
+  if (!s) return;
+  if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (auto& e : s->ev)
+    if (e) (void)hipEventDestroy(e);
+  dfree(s->lights);
+  photons_free(s->pb);
+  map_free(s->map);
+  eye_free(s->eb);
+  dfree(s->sum);
+  dfree(s->sumsq);
+  dfree(s->st);
+  delete s;
+}
+
+ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, ppm_session** out) {
+  if (!c || !sc || !cfg || !out) return fail(c, PPM_E_INVALID, "null argument");
+  *out = nullptr;
+  if (ppm_status s = check_cfg_depth(c, cfg->max_depth)) return s;
+  if (cfg->n_photon == 0 || cfg->n_photon > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "n_photon out of range");
+  if (!(cfg->radius0 > 0.0) || !std::isfinite(cfg->radius0) || !std::isfinite(cfg->alpha))
+    return fail(c, PPM_E_INVALID, "bad radius0/alpha");
+  if (cfg->pfilter < 0 || cfg->pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
+  if (sc->lights.empty() || !(sc->total_flux > 0.0)) return fail(c, PPM_E_INVALID, "scene has no emitting light");
+  ppm_session* s = new ppm_session();
+  s->ctx = c;
+  s->scene = sc;
+  s->cfg = *cfg;
+  if (s->cfg.first_pass == 0) s->cfg.first_pass = 1;
+  std::vector<DevLight> L = sc->lights;
+  uint64_t total = assign_photons(L, sc->total_flux, cfg->n_photon, &s->power);
+  if (total == 0 || total > 0x7FFFFFFFull) {
+    delete s;
+    return fail(c, PPM_E_RANGE, "per-light photon counts out of range");
+  }
+  const uint64_t cap = photons_needed(total, cfg->max_depth, c->n_cu);
+  if (cap > 0x7FFFFFFFull) {
+    delete s;
+    return fail(c, PPM_E_RANGE, "n_photon * max_depth must stay below 2^31");
+  }
+  s->sbuf = sc->buf;
+  s->sbuf.n_emit_total = (uint32_t)total;
+  hipError_t e = hipSuccess;
+  do {
+    if ((e = dalloc(&s->lights, L.size())) != hipSuccess) break;
+    if ((e = hipMemcpy(s->lights, L.data(), sizeof(DevLight) * L.size(), hipMemcpyHostToDevice)) != hipSuccess) break;
+    s->sbuf.lights = s->lights;
+    if ((e = photons_alloc(s->pb, cap, trace_waves(total, c->n_cu))) != hipSuccess) break;
+    photons_layout(s->pb, total, cfg->max_depth, c->n_cu);
+    if ((e = map_alloc(s->map, cap, c->n_cu)) != hipSuccess) break;
+    if ((e = eye_alloc(s->eb, sc->buf.cam, cfg->max_depth, c->n_cu)) != hipSuccess) break;
+    const uint64_t n3 = 3ull * s->eb.n_pix;
+    if ((e = dalloc(&s->sum, n3)) != hipSuccess) break;
+    if ((e = dalloc(&s->sumsq, n3)) != hipSuccess) break;
+    if ((e = hipMemset(s->sum, 0, sizeof(double) * n3)) != hipSuccess) break;
+    if ((e = hipMemset(s->sumsq, 0, sizeof(double) * n3)) != hipSuccess) break;
+    if ((e = dalloc(&s->st, 1)) != hipSuccess) break;
+    PassState h = host_state();
+    if ((e = hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) break;
+    for (auto& ev : s->ev)
+      if ((e = hipEventCreate(&ev)) != hipSuccess) break;
+  } while (0);
+  if (e != hipSuccess) {
+    ppm_session_destroy(s);
+    return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  }
+  *out = s;
+  return PPM_OK;
+}
+
+// Enqueue the kernels of one pass; ev != nullptr records stage events.
+static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
+  const ppm_pass_cfg& cfg = s->cfg;
+  const ppm_scene* sc = s->scene;
+  hipError_t e;
+  const float4 bmin = make_float4((float)sc->bmin[0], (float)sc->bmin[1], (float)sc->bmin[2], 0.f);
+  const float4 bmax = make_float4((float)sc->bmax[0], (float)sc->bmax[1], (float)sc->bmax[2], 0.f);
+  if (ev && (e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
+  if ((e = launch_pass_begin(st, s->st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
+    return e;
+  if ((e = launch_trace_photons(st, s->sbuf, s->st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
+    return e;
+  if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
+  if ((e = launch_map_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
+  if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
+  if ((e = launch_gather(st, s->map, s->st, s->eb, sc->buf.cam, bmin, bmax, cfg.pfilter, s->ctx->n_cu, s->sum,
+                         s->sumsq, ev ? ev + 4 : nullptr)) != hipSuccess)
+    return e;
+  if ((e = launch_pass_end(st, s->st)) != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[6], st)) != hipSuccess) return e;
+  return hipSuccess;
+}
+
+ppm_status ppm_session_run(ppm_session* s, uint32_t n_pass) {
+  if (!s) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  hipStream_t st = c->stream;
+  const bool timing = (s->cfg.flags & PPM_SESSION_STAGE_TIMING) != 0;
+  const bool graph = !timing && (s->cfg.flags & PPM_SESSION_NO_GRAPH) == 0;
+  if (timing) {
+    for (uint32_t i = 0; i < n_pass; ++i) {
+      HIP_TRY(c, enqueue_pass(s, st, s->ev));
+      HIP_TRY(c, hipEventSynchronize(s->ev[6]));
+      float ms[6];
+      for (int k = 0; k < 6; ++k) HIP_TRY(c, hipEventElapsedTime(&ms[k], s->ev[k], s->ev[k + 1]));
+      s->t.ms_photon_trace += ms[0];
+      s->t.ms_map_build += ms[1];
+      s->t.ms_eye_trace += ms[2];
+      s->t.ms_gather += ms[4];
+      s->t.ms_accumulate += ms[3] + ms[5];
+      s->t.ms_total += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+    }
+  } else {
+    if (graph && !s->exec) {
+      hipStream_t cap;
+      HIP_TRY(c, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        hipError_t e2 = enqueue_pass(s, cap, nullptr);
+        e = hipStreamEndCapture(cap, &s->graph);
+        if (e2 != hipSuccess) e = e2;
+      }
+      if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+      (void)hipStreamDestroy(cap);
+      if (e != hipSuccess) return fail(c, PPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    HIP_TRY(c, hipEventRecord(s->ev[8], st));
+    for (uint32_t i = 0; i < n_pass; ++i) {
+      if (graph)
+        HIP_TRY(c, hipGraphLaunch(s->exec, st));
+      else
+        HIP_TRY(c, enqueue_pass(s, st, nullptr));
+    }
+    HIP_TRY(c, hipEventRecord(s->ev[9], st));
+    HIP_TRY(c, hipEventSynchronize(s->ev[9]));
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, s->ev[8], s->ev[9]));
+    s->t.ms_total += ms;
+  }
+  s->passes_done += n_pass;
+  s->t.passes = s->passes_done;
+  return PPM_OK;
+}
+
+ppm_status ppm_session_timings(ppm_session* s, ppm_timings* t) {
+  if (!s || !t) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *t = s->t;
+  t->photons_stored = h.total_stored;
+  t->gather_queries = h.total_queries + (uint64_t)s->passes_done * 0;
+  t->photons_emitted = (uint64_t)s->sbuf.n_emit_total * s->passes_done;
+  t->passes = s->passes_done;
+  return PPM_OK;
+}
+
+ppm_status ppm_session_radius2(ppm_session* s, double* r2) {
+  if (!s || !r2) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *r2 = h.r2;
+  return PPM_OK;
+}
+
+ppm_status ppm_session_image(ppm_session* s, double* mean_out, double* var_out, uint32_t* passes_done) {
+  if (!s) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (passes_done) *passes_done = s->passes_done;
+  if (h.overflow) return fail(c, PPM_E_RANGE, "capacity exceeded during a pass (flags " + std::to_string(h.overflow) + ")");
+  if (h.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
+  const uint64_t n3 = 3ull * s->eb.n_pix;
+  std::vector<double> sum(n3), sq(n3);
+  HIP_TRY(c, hipMemcpy(sum.data(), s->sum, sizeof(double) * n3, hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(sq.data(), s->sumsq, sizeof(double) * n3, hipMemcpyDeviceToHost));
+  const double n = (double)s->passes_done;
+  for (uint64_t i = 0; i < n3; ++i) {
+    if (mean_out) mean_out[i] = n > 0 ? sum[i] / n : 0.0;
+    if (var_out) var_out[i] = n > 1 ? std::max(0.0, (sq[i] - sum[i] * sum[i] / n) / (n - 1.0)) : 0.0;
+  }
+  return PPM_OK;
+}
+
+ppm_status ppm_run_passes(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, uint32_t n_pass, double* mean_out,
+                          double* var_out, ppm_timings* t) {
+  ppm_session* s = nullptr;
+  if (ppm_status st = ppm_session_create(c, sc, cfg, &s)) return st;
+  ppm_status st = ppm_session_run(s, n_pass);
+  if (st == PPM_OK) st = ppm_session_image(s, mean_out, var_out, nullptr);
+  if (st == PPM_OK && t) st = ppm_session_timings(s, t);
+  ppm_session_destroy(s);
+  return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,553 @@
+// ppm_dev.hpp — device-side types and arithmetic of the PPM-PA hot path
+// (float traversal/intersection/shading, gfx950).
+//
+// Every function restates the reference function it cites; the reference is
+// FP64 Rust, this is FP32 HIP.  Where FP32 could produce a result that FP64
+// produces only on a measure-zero set (NaN from 1/0 etc.), the FP64 outcome
+// is taken and the guard is commented.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ppm_layout.hpp"
+
+namespace ppm {
+
+// ---------------------------------------------------------------------------
+// 3-vectors (algebra.rs:50-181)
+
+struct V3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ V3 mk(float x, float y, float z) { return V3{x, y, z}; }
+__device__ __forceinline__ V3 mk(float4 a) { return V3{a.x, a.y, a.z}; }
+__device__ __forceinline__ V3 operator+(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// algebra.rs:174-180 (component order kept)
+__device__ __forceinline__ V3 cross(V3 a, V3 b) {
+  return V3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+// algebra.rs:151-158: None for a zero vector.
+__device__ __forceinline__ bool normalize(V3 a, V3& out) {
+  float n = sqrtf(dot(a, a));
+  if (n == 0.0f) return false;
+  float inv = 1.0f / n;
+  out = a * inv;
+  return true;
+}
+__device__ __forceinline__ V3 normalize_or(V3 a, V3 fallback) {
+  V3 r;
+  return normalize(a, r) ? r : fallback;
+}
+
+// RGB triple (physics.rs Color / optics.rs Radiance share this shape).
+struct C3 {
+  float r, g, b;
+};
+__device__ __forceinline__ C3 c3(float r, float g, float b) { return C3{r, g, b}; }
+__device__ __forceinline__ C3 c3(float4 a) { return C3{a.x, a.y, a.z}; }
+__device__ __forceinline__ C3 operator+(C3 a, C3 b) { return C3{a.r + b.r, a.g + b.g, a.b + b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, C3 b) { return C3{a.r * b.r, a.g * b.g, a.b * b.b}; }
+__device__ __forceinline__ C3 operator*(C3 a, float s) { return C3{a.r * s, a.g * s, a.b * s}; }
+__device__ __forceinline__ bool nonzero(C3 a) { return a.r != 0.0f || a.g != 0.0f || a.b != 0.0f; }
+__device__ __forceinline__ float chan(float4 c, uint32_t wl) {
+  return wl == 0 ? c.x : (wl == 1 ? c.y : c.z);
+}
+
+// ---------------------------------------------------------------------------
+// Counter-based RNG: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel
+// random numbers: as easy as 1, 2, 3", SC'11).  The reference draws from the
+// unseeded rand::thread_rng (physics.rs:324 and others), which nothing can
+// reproduce; every stream here is keyed by (seed) and indexed by
+// (element, pass, stream tag, block) so that the FP64 oracle draws the same
+// numbers.
+
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    uint32_t lo0 = PHILOX_M0 * c.x;
+    uint32_t hi0 = __umulhi(PHILOX_M0, c.x);
+    uint32_t lo1 = PHILOX_M1 * c.z;
+    uint32_t hi1 = __umulhi(PHILOX_M1, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += PHILOX_W0;
+    k1 += PHILOX_W1;
+  }
+  return c;
+}
+
+// Uniform in [0, 1) with 24 random bits: exact in float and in double.
+__device__ __forceinline__ float u01(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// A sequential stream of uniforms: draw n comes from block n/4, lane n%4.
+struct Rng {
+  uint32_t k0, k1;     // key = seed
+  uint32_t c0, c1, c2; // element, pass, tag
+  uint32_t n;          // draws consumed
+  uint4 blk;
+
+  __device__ __forceinline__ void init(uint64_t seed, uint32_t elem, uint32_t pass, uint32_t tag) {
+    k0 = (uint32_t)seed;
+    k1 = (uint32_t)(seed >> 32);
+    c0 = elem;
+    c1 = pass;
+    c2 = tag;
+    n = 0;
+  }
+  __device__ __forceinline__ float next() {
+    uint32_t j = n & 3u;
+    if (j == 0) blk = philox4x32_10(make_uint4(c0, c1, c2, n >> 2), k0, k1);
+    ++n;
+    uint32_t v = j == 0 ? blk.x : (j == 1 ? blk.y : (j == 2 ? blk.z : blk.w));
+    return u01(v);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Scene view passed by value to every kernel.
+
+struct SceneView {
+  const DevShape* shapes;
+  const DevMaterial* mats;  // n_mats + 1 entries: the last one is M_AIR
+  const DevLight* lights;
+  uint32_t n_shapes;
+  uint32_t n_mats;          // index of M_AIR
+  uint32_t n_lights;
+  uint32_t n_emit;           // photons per pass over all lights
+  DevCamera cam;
+  float4 bmin, bmax;        // Morton quantisation box
+};
+
+// ---------------------------------------------------------------------------
+// Intersection (tracer.rs:306-359, geometry.rs:132-202).
+
+struct Hit {
+  float t;
+  int32_t obj;
+};
+
+// distance_plain (geometry.rs:170-177), distance_sphere (:179-193) and
+// method_moller (:149-165) folded into a nearest-hit update.  Candidates with
+// t < NEARLY0 are dropped (tracer.rs:324); ties keep the earlier object and
+// the earlier candidate, as the stable sort of tracer.rs:335 does.
+__device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, V3 d, Hit& h) {
+  const float eps = NEARLY0_F;
+  switch (s.kind) {
+    case SHAPE_PLAIN: {
+      V3 n = mk(s.n);
+      float c = dot(n, d);
+      if (c != 0.0f) {
+        float t = (s.dist + dot(n, o)) / -c;
+        if (t >= eps && t < h.t) { h.t = t; h.obj = idx; }
+      }
+      break;
+    }
+    case SHAPE_SPHERE: {
+      V3 oc = mk(s.p) - o;
+      float t0 = dot(oc, d);
+      float t1 = s.radius * s.radius - (dot(oc, oc) - t0 * t0);
+      if (t1 > 0.0f) {
+        float t2 = sqrtf(t1);
+        if (t2 == 0.0f) {
+          if (t0 >= eps && t0 < h.t) { h.t = t0; h.obj = idx; }
+        } else {
+          float ta = t0 - t2, tb = t0 + t2;
+          if (ta >= eps && ta < h.t) { h.t = ta; h.obj = idx; }
+          if (tb >= eps && tb < h.t) { h.t = tb; h.obj = idx; }
+        }
+      }
+      break;
+    }
+    case SHAPE_POLYGON:
+    case SHAPE_PARALLELOGRAM: {
+      V3 d1 = mk(s.d1), d2 = mk(s.d2);
+      V3 re2 = cross(d, d2);
+      float det = dot(re2, d1);
+      if (det != 0.0f) {
+        V3 pp = o - mk(s.p);
+        V3 te1 = cross(pp, d1);
+        float inv = 1.0f / det;
+        float u = dot(re2, pp) * inv;
+        float v = dot(te1, d) * inv;
+        float t = dot(te1, d2) * inv;
+        float lim = s.kind == SHAPE_POLYGON ? 1.0f : 2.0f;
+        if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > lim)) {
+          if (t >= eps && t < h.t) { h.t = t; h.obj = idx; }
+        }
+      }
+      break;
+    }
+    default:
+      break;  // Point: never hit (geometry.rs:134)
+  }
+}
+
+// Read-only device data through the constant address space: loads at a
+// wave-uniform index become scalar (s_load) loads shared by the 64 lanes.
+// (The host pass of hipcc also parses device bodies; it has no such space.)
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return ((const __attribute__((address_space(4))) T*)p)[i];
+#else
+  return p[i];
+#endif
+}
+
+__device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d) {
+  Hit h{INF_F, -1};
+  for (uint32_t i = 0; i < sc.n_shapes; ++i) {
+    const DevShape s = ldc(sc.shapes, i);
+    shape_hit(s, (int32_t)i, o, d, h);
+  }
+  return h;
+}
+
+struct Isect {
+  V3 pos;
+  V3 n;       // faces the incoming ray
+  int32_t obj;
+  uint32_t mat;
+  bool inside_to_out;  // io == Out (the normal was flipped)
+};
+
+// calc_intersection tail (tracer.rs:336-348): position, normal, flip.
+__device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit h, Isect& is) {
+  if (h.obj < 0) return false;
+  const DevShape& s = sc.shapes[h.obj];
+  V3 p = o + d * h.t;  // Ray::target (geometry.rs:56-58)
+  V3 n;
+  if (s.kind == SHAPE_SPHERE) {
+    if (!normalize(p - mk(s.p), n)) return false;  // get_normal None
+  } else {
+    n = mk(s.n);
+  }
+  bool flip = dot(n, d) > 0.0f;
+  is.pos = p;
+  is.n = flip ? -n : n;
+  is.obj = h.obj;
+  is.mat = s.mat;
+  is.inside_to_out = flip;
+  return true;
+}
+
+__device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is) {
+  return finish_hit(sc, o, d, nearest(sc, o, d), is);
+}
+
+// ---------------------------------------------------------------------------
+// Physics (physics.rs).
+
+// relative_ior (physics.rs:200-205): ior2/ior1, 1 when ior1 == 0.
+__device__ __forceinline__ float relative_ior(float i1, float i2) { return i1 == 0.0f ? 1.0f : i2 / i1; }
+
+// specular_reflection (physics.rs:214-221).
+__device__ __forceinline__ V3 specular_reflection(V3 n, V3 v, float& cos1) {
+  float c = -dot(v, n);
+  if (c < 0.0f) {
+    cos1 = -c;
+    return n;
+  }
+  cos1 = c;
+  return normalize_or(v + (2.0f * c) * n, n);
+}
+
+// specular_refraction (physics.rs:270-285).  eta == 0 (ior 0 surfaces) is
+// None here: in double g0 = cos1^2 - 1 < 0 except on a measure-zero set, but
+// in float cos1 rounds to exactly 1 near normal incidence and 1/0 would give
+// NaN.
+__device__ __forceinline__ bool specular_refraction(V3 n, V3 v, float eta, V3& t, float& cos2) {
+  cos2 = 0.0f;
+  float c1 = -dot(v, n);
+  if (c1 < 0.0f || eta == 0.0f) return false;
+  float g0 = eta * eta + c1 * c1 - 1.0f;
+  if (g0 < 0.0f) return false;
+  float g = sqrtf(g0);
+  V3 tv = (1.0f / eta) * (v + (c1 - g) * n);
+  if (!normalize(tv, t)) return false;
+  cos2 = g / eta;
+  return true;
+}
+
+// (1 - c)^5 as in schlick (physics.rs:316-318) and reflection_index
+// (surface.rs:512-515).
+__device__ __forceinline__ float pow5(float x) {
+  float x2 = x * x;
+  return x2 * x2 * x;
+}
+__device__ __forceinline__ float schlick(float f0, float c) { return f0 + (1.0f - f0) * pow5(1.0f - c); }
+
+// generate_random_dir_by_angle (algebra.rs:225-235) + diffuse_reflection
+// (surface.rs:408-417): uniform on the sphere, flipped into n's hemisphere.
+__device__ __forceinline__ V3 diffuse_reflection(V3 n, Rng& rng) {
+  float phi = TWO_PI_F * rng.next();
+  float xi = 2.0f * rng.next() - 1.0f;
+  float xi2 = sqrtf(fmaxf(1.0f - xi * xi, 0.0f));
+  float sp, cp;
+  __sincosf(phi, &sp, &cp);
+  V3 d = normalize_or(mk(xi2 * cp, xi, xi2 * sp), mk(0.0f, 1.0f, 0.0f));
+  return dot(n, d) > 0.0f ? d : -d;
+}
+
+// reflection_glossy (physics.rs:232-261).  xi1 = xi0^(pw c0) is formed as
+// exp(t) with 1 - xi1 = -expm1(t), t = pw c0 ln xi0, so that the tiny
+// deviation of near-mirror surfaces (pw ~ 1e-5) keeps float precision.
+__device__ __forceinline__ V3 reflection_glossy(V3 n, V3 r, float pw, Rng& rng) {
+  V3 u;
+  if (!normalize(cross(mk(0.00424f, 1.0f, 0.00764f), r), u))
+    u = normalize_or(cross(mk(1.0f, 0.00424f, 0.00764f), r), mk(1.0f, 0.0f, 0.0f));
+  V3 v = cross(u, r);
+  float c0 = dot(n, r);
+  float xi0 = rng.next();
+  float xi2 = TWO_PI_F * rng.next();
+  float e = pw * c0;
+  float xi1, one_m;
+  if (xi0 == 0.0f) {  // 0^e: 1 when e == 0, else 0
+    xi1 = e == 0.0f ? 1.0f : (e > 0.0f ? 0.0f : INF_F);
+    one_m = 1.0f - xi1;
+  } else {
+    float t = e * logf(xi0);
+    xi1 = expf(t);
+    one_m = -expm1f(t);
+  }
+  float s = sqrtf(fmaxf(one_m * (1.0f + xi1), 0.0f));
+  float sp, cp;
+  __sincosf(xi2, &sp, &cp);
+  float x = cp * s, y = xi1, z = sp * s;
+  V3 wi = x * u + y * r + z * v;
+  if (dot(n, wi) < 0.0f) wi = (-x) * u + y * r - z * v;
+  return normalize_or(wi, mk(1.0f, 0.0f, 0.0f));
+}
+
+// ---------------------------------------------------------------------------
+// Surfaces (surface.rs).
+
+__device__ __forceinline__ bool is_black(float4 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+__device__ __forceinline__ bool is_white(float4 c) { return c.x == 1.0f && c.y == 1.0f && c.z == 1.0f; }
+
+// Surface::reflect (surface.rs:68-100)
+__device__ __forceinline__ bool surf_reflect(const DevMaterial& m, float cos) {
+  if (m.surface == SURF_SIMPLE) return !(m.diff == 1.0f || (cos == 1.0f && is_black(m.c2)));
+  if (m.surface == SURF_TS) return m.metal == 1.0f ? !is_black(m.c2) : true;
+  return false;
+}
+// Surface::refract (surface.rs:102-133)
+__device__ __forceinline__ bool surf_refract(const DevMaterial& m, float cos) {
+  if (m.surface == SURF_SIMPLE) return !(cos == 0.0f && is_white(m.c2));
+  if (m.surface == SURF_TS) return m.metal == 0.0f && m.diff < 1.0f && !is_black(m.c1);
+  return false;
+}
+// Surface::store_photon (surface.rs:289-310)
+__device__ __forceinline__ bool surf_store(const DevMaterial& m) {
+  if (m.surface == SURF_SIMPLE) return m.diff > 0.0f;
+  if (m.surface == SURF_TS) return m.metal != 1.0f && m.diff != 0.0f;
+  return true;
+}
+// Surface::power_glossy (surface.rs:381-402)
+__device__ __forceinline__ float power_glossy(const DevMaterial& m) {
+  return (m.surface == SURF_SIMPLE || m.surface == SURF_TS) ? m.density_pow : 0.0f;
+}
+
+// The BSDF of surface.rs:135-206 is linear in (di, si, ti):
+//   result = wd*di + ws*si + wt*ti
+// Simple (surface.rs:150-153):
+//   f = reflection_index(specular_refl, cos0)
+//   wd = diffuseness * reflectance / pi
+//   ws = (1 - diffuseness) * f
+//   wt = (1 - diffuseness) * (1 - metalness) * (1 - f)
+// TS (surface.rs:164-197):
+//   f = reflection_index(albedo_spec, cos0); f2 = 1 - f
+//   metalness == 0: wd = f2*albedo_diff*scatterness/pi,
+//                   wt = f2*albedo_diff*(1 - scatterness); else wd = wt = 0
+//   ws = f
+struct BsdfW {
+  C3 wd, ws, wt;
+};
+__device__ __forceinline__ C3 reflection_index(float4 col, float c) {
+  float c2 = pow5(1.0f - c);
+  return C3{col.x + (1.0f - col.x) * c2, col.y + (1.0f - col.y) * c2, col.z + (1.0f - col.z) * c2};
+}
+__device__ __forceinline__ BsdfW bsdf_weights(const DevMaterial& m, float cos0) {
+  BsdfW w;
+  C3 z{0.0f, 0.0f, 0.0f};
+  w.wd = z; w.ws = z; w.wt = z;
+  if (m.surface == SURF_SIMPLE) {
+    C3 f = reflection_index(m.c2, cos0);
+    float od = 1.0f - m.diff;
+    w.wd = c3(m.c1) * (m.diff * INV_PI_F);
+    w.ws = f * od;
+    float k = od * (1.0f - m.metal);
+    w.wt = C3{(1.0f - f.r) * k, (1.0f - f.g) * k, (1.0f - f.b) * k};
+  } else if (m.surface == SURF_TS) {
+    C3 f = reflection_index(m.c2, cos0);
+    C3 f2{1.0f - f.r, 1.0f - f.g, 1.0f - f.b};
+    if (m.metal == 0.0f) {
+      C3 a = f2 * c3(m.c1);
+      w.wd = a * (m.diff * INV_PI_F);
+      w.wt = a * (1.0f - m.diff);
+    }
+    w.ws = f;
+  }
+  return w;
+}
+
+// ---------------------------------------------------------------------------
+// Lights (light.rs).
+
+// Σ over one light of light.get_radiance(dists) zipped with cos^2
+// (tracer.rs:263-290, light.rs:93-150).  The reference prepends RADIANCE0 to
+// the radiance list before the zip (light.rs:132), so sample i's cos pairs
+// with sample i-1's radiance and the last radiance is dropped: the running
+// `prev` carries that shift.
+__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, float& cos0sq,
+                                           float& sqd) {
+  V3 ld3;
+  if (!normalize(ld, ld3)) return false;
+  float c0 = dot(n, ld3);
+  if (c0 < 0.0f) return false;
+  Isect is;
+  if (!intersect(sc, p, ld3, is)) return false;
+  float sq_l = dot(ld, ld);
+  V3 od = is.pos - p;
+  float sq_o = dot(od, od);
+  if (sq_l - sq_o > LIGHT_SLACK_F) return false;
+  cos0sq = c0 * c0;
+  sqd = sq_l;
+  return true;
+}
+
+__device__ __forceinline__ C3 light_radiance(const DevLight& L, float sqd) {
+  if (L.kind == LIGHT_POINT) {
+    float l0 = L.flux / (FOUR_PI_F * sqd);
+    return c3(L.color) * l0;
+  }
+  if (L.kind == LIGHT_PARALLELOGRAM) {
+    float l0 = (2.0f * L.flux * PARA_DIV_F * PARA_DIV_F) / (FOUR_PI_F * sqd);
+    return c3(L.color) * l0;
+  }
+  return c3(L.color) * L.flux;  // SunLight
+}
+
+__device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
+  C3 sum{0.0f, 0.0f, 0.0f};
+  for (uint32_t li = 0; li < sc.n_lights; ++li) {
+    const DevLight L = ldc(sc.lights, li);
+    C3 prev{0.0f, 0.0f, 0.0f};
+    if (L.kind == LIGHT_PARALLELOGRAM) {
+      V3 lp0 = mk(L.pos), d1 = mk(L.d1), d2 = mk(L.d2), ln = mk(L.nvec);
+      // TSS grid (light.rs:164-170), row-major in the first coordinate.
+      const float ts[5] = {0.1f, 0.3f, 0.5f, 0.7f, 0.9f};
+      for (int a = 0; a < 5; ++a) {
+        for (int b = 0; b < 5; ++b) {
+          float tx = ts[a], ty = ts[b];
+          V3 ld = (lp0 + tx * d1 + ty * d2) - p;
+          if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
+          float csq, sqd;
+          if (!sample_lit(sc, p, n, ld, csq, sqd)) continue;
+          sum = sum + prev * csq;
+          prev = light_radiance(L, sqd);
+        }
+      }
+    } else if (L.kind == LIGHT_POINT) {
+      V3 ld = mk(L.pos) - p;  // light.rs:116-117
+      float csq, sqd;
+      if (sample_lit(sc, p, n, ld, csq, sqd)) {
+        sum = sum + prev * csq;
+        prev = light_radiance(L, sqd);
+      }
+    } else {  // SunLight (light.rs:132-147)
+      V3 d = mk(L.pos) - p;
+      if (!(dot(mk(L.nvec), d) > 0.0f)) {
+        V3 dt2 = -mk(L.dir);
+        V3 d1 = mk(L.d1), d2 = mk(L.d2);
+        V3 re2 = cross(dt2, d2);
+        float det = dot(re2, d1);
+        if (det != 0.0f) {
+          V3 pp = p - mk(L.pos);
+          V3 te1 = cross(pp, d1);
+          float inv = 1.0f / det;
+          float u = dot(re2, pp) * inv, v = dot(te1, dt2) * inv, t = dot(te1, d2) * inv;
+          if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > 2.0f)) {
+            V3 ld = t * dt2;
+            float csq, sqd;
+            if (sample_lit(sc, p, n, ld, csq, sqd)) {
+              sum = sum + prev * csq;
+              prev = light_radiance(L, sqd);
+            }
+          }
+        }
+      }
+    }
+  }
+  return sum;
+}
+
+// ---------------------------------------------------------------------------
+// Camera::generate_ray (camera.rs:151-168) for pixel (y, x); draws come from
+// the pixel's root stream: blur offsets first, then the AA jitter.
+__device__ __forceinline__ void camera_ray(const DevCamera& cam, int x, int y, Rng& rng, V3& o, V3& d) {
+  V3 blur = mk(0.0f, 0.0f, 0.0f);
+  if (cam.blur) {
+    float r1 = rng.next() - 0.5f;
+    float r2 = rng.next() - 0.5f;
+    blur = r1 * mk(cam.eex) + r2 * mk(cam.eey);
+  }
+  float r3 = 0.0f, r4 = 0.0f;
+  if (cam.progressive && cam.antialias) {
+    r3 = rng.next() - 0.5f;
+    r4 = rng.next() - 0.5f;
+  }
+  o = mk(cam.eye) + blur;
+  V3 dir = mk(cam.origin) + ((float)x + r3) * mk(cam.esx) + ((float)y + r4) * mk(cam.esy) - blur;
+  d = normalize_or(dir, mk(0.0f, 0.0f, 1.0f));
+}
+
+// ---------------------------------------------------------------------------
+// Radius predicate of KdTree::within with squared_euclidean: the left fold
+// ((0 + dx^2) + dy^2) + dz^2 over the axes, compared inclusively.  Evaluated
+// with no contraction so that the host float brute force reproduces it bit
+// for bit.
+
+__device__ __forceinline__ float dist2_nc(float qx, float qy, float qz, float px, float py, float pz) {
+#pragma clang fp contract(off)
+  float dx = qx - px;
+  float dy = qy - py;
+  float dz = qz - pz;
+  float s = dx * dx + dy * dy;
+  return s + dz * dz;
+}
+
+// Squared distance from q to an axis-aligned box, with the same operation
+// order: never larger than dist2_nc(q, p) for any p inside the box, because
+// float subtraction, squaring and addition are monotonic.
+__device__ __forceinline__ float box_dist2_nc(float qx, float qy, float qz, float4 lo, float4 hi) {
+#pragma clang fp contract(off)
+  float dx = fmaxf(fmaxf(lo.x - qx, qx - hi.x), 0.0f);
+  float dy = fmaxf(fmaxf(lo.y - qy, qy - hi.y), 0.0f);
+  float dz = fmaxf(fmaxf(lo.z - qz, qz - hi.z), 0.0f);
+  float s = dx * dx + dy * dy;
+  return s + dz * dz;
+}
+
+// filter_cone (tracer.rs:197-204) and filter_gauss (tracer.rs:206-216) as
+// functions of t = d2 / r2 (both reference filters depend on d2 only through
+// that ratio).
+__device__ __forceinline__ float filter_weight(int pf, float t) {
+  if (pf == FILTER_CONE) {
+    const float d = sqrtf(t) / K_CONE_F;
+    return d > 1.0f ? 0.0f : (1.0f - d) / FAC_K_F;
+  }
+  if (pf == FILTER_GAUSS) {
+    const float e_r = 1.0f - expf(-GAUSS_BETA_F * t * 0.5f);
+    return e_r > GAUSS_E_BETA_F ? 0.0f : GAUSS_ALPHA_F * (1.0f - e_r / GAUSS_E_BETA_F) + GAUSS_CORR_F;
+  }
+  return 1.0f;
+}
+
+}  // namespace ppm

@@ -1,0 +1,1191 @@
+// ppm_kernels.hip — gfx950 kernels of one PPM-PA pass:
+//   photon emission + bouncing   (light.rs:67-91, tracer.rs:31-125)
+//   photon map build             (photonmap.rs:23-29 + crate kdtree)
+//   eye-ray tree + direct light  (tracer.rs:129-177, 263-290, camera.rs:151-168)
+//   radius query + gather        (tracer.rs:179-216, optics.rs:224-233)
+//   FP64 per-pixel accumulation  (util/averager2.rb:49-62)
+//
+// Wavefront-64 design: one photon / pixel / query per lane; wave-wide ballots
+// compact live photons and stored records; the map is a balanced binary tree
+// over Morton-sorted 64-photon leaves, traversed wave-coherently so that one
+// leaf is staged once into LDS (coalesced loads) and tested by all 64 lanes.
+#include <hip/hip_runtime.h>
+
+#include "ppm_dev.hpp"
+#include "ppm_launch.hpp"
+
+namespace ppm {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
+
+static SceneView make_view(const SceneBuf& b) {
+  SceneView v;
+  v.shapes = b.shapes;
+  v.mats = b.mats;
+  v.lights = b.lights;
+  v.n_shapes = b.n_shapes;
+  v.n_mats = b.n_mats;
+  v.n_lights = b.n_lights;
+  v.n_emit = b.n_emit_total;
+  v.cam = b.cam;
+  v.bmin = b.bmin;
+  v.bmax = b.bmax;
+  return v;
+}
+
+// ===========================================================================
+// Pass state (util/iterator.rb:33-38): r_{p}^2 = r_{p-1}^2 (p - 1 + alpha) / p,
+// in double on the device so that a captured pass loop needs no host input.
+
+__global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  PassState s = *st;
+  if (s.pass == 0) {
+    s.pass = first_pass;
+    s.alpha = alpha;
+    s.r2_first = r2_first;
+    s.power = power;
+    double r2 = r2_first;
+    for (uint32_t k = 1; k < first_pass; ++k) r2 *= ((double)k + alpha) / ((double)k + 1.0);
+    s.r2 = r2;
+  } else {
+    s.pass += 1;
+    double k = (double)(s.pass - 1);
+    s.r2 *= (k + s.alpha) / (k + 1.0);
+  }
+  s.r2f = (float)s.r2;
+  s.scale = (float)(s.power / (3.14159265358979323846 * s.r2));
+  s.n_stored = 0;
+  s.next_photon = 0;
+  s.n_queries = 0;
+  *st = s;
+}
+
+__global__ void k_pass_end(PassState* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->total_stored += st->n_stored;
+  st->total_queries += st->n_qsort + min(st->n_queries, 0xFFFFFFFFu);
+}
+
+hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
+                             double power) {
+  hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, s, st, first_pass, r2_first, alpha, power);
+  return hipGetLastError();
+}
+hipError_t launch_pass_end(hipStream_t s, PassState* st) {
+  hipLaunchKernelGGL(k_pass_end, dim3(1), dim3(64), 0, s, st);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// Photon emission (light.rs:67-91) and one bounce (tracer.rs:31-125).
+
+struct PhotonState {
+  V3 o, d;
+  uint32_t wl, med, depth, pid;
+};
+
+// Light::generate_photon for photon `pid` of the pass.
+__device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, Rng& rng, PhotonState& p) {
+  uint32_t li = 0;
+  for (uint32_t i = 0; i < sc.n_lights; ++i)
+    if (pid >= ldc(sc.lights, i).first_photon) li = i;
+  const DevLight& L = sc.lights[li];
+  // select_wavelength / decide_wavelength (light.rs:157-160, physics.rs:74-84)
+  float u = rng.next();
+  p.wl = u < L.color.x ? 0u : (u < L.color.x + L.color.y ? 1u : 2u);
+  if (L.kind == LIGHT_PARALLELOGRAM) {
+    float t1 = rng.next();
+    float t2 = rng.next();
+    p.d = diffuse_reflection(mk(L.nvec), rng);
+    p.o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
+  } else if (L.kind == LIGHT_POINT) {
+    // generate_random_dir (algebra.rs:211-223): rejection in the unit ball
+    V3 v = mk(0.0f, 1.0f, 0.0f);
+    for (int k = 0; k < 64; ++k) {
+      float x = 2.0f * rng.next() - 1.0f;
+      float y = 2.0f * rng.next() - 1.0f;
+      float z = 2.0f * rng.next() - 1.0f;
+      V3 c = mk(x, y, z);
+      float len = sqrtf(dot(c, c));
+      if (0.0f < len && len <= 1.0f) {
+        v = c * (1.0f / len);
+        break;
+      }
+    }
+    p.d = v;
+    p.o = mk(L.pos);
+  } else {  // SunLight
+    float t1 = rng.next();
+    float t2 = rng.next();
+    p.o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
+    p.d = mk(L.dir);
+  }
+  p.med = sc.n_mats;  // M_AIR
+  p.depth = 0;
+  p.pid = pid;
+}
+
+// One step of trace_photon.  Returns whether the photon lives on; `store`
+// reports a photon-map record (position, INCOMING direction, channel).
+__device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int maxd, PhotonState& p, Rng& rng,
+                                              bool& store, V3& spos) {
+  store = false;
+  if ((int)p.depth >= maxd) return false;  // tracer.rs:32
+  Isect is;
+  if (!intersect(sc, p.o, p.d, is)) return false;  // tracer.rs:36-39
+  const DevMaterial& M = sc.mats[is.mat];
+  // tracer.rs:77-79
+  store = (uc == 0 || p.depth > 0) && surf_store(M);
+  spos = is.pos;
+  if (M.surface == SURF_SIMPLE) {
+    // russian_roulette(&[roughness()]) where roughness() is diffuseness
+    // (tracer.rs:51-53, surface.rs:358-367); check_under: 0 iff u <= p.
+    float u = rng.next();
+    if (u <= M.diff) {
+      // reflect_diff (tracer.rs:83-92)
+      float u2 = rng.next();
+      if (!(u2 <= chan(M.c1, p.wl))) return false;
+      p.d = diffuse_reflection(is.n, rng);
+      p.o = is.pos;
+      p.depth += 1;
+      return true;
+    }
+    // reflect_spec (tracer.rs:94-109)
+    float cos1;
+    V3 rdir = specular_reflection(is.n, p.d, cos1);
+    float f = schlick(chan(M.c2, p.wl), cos1);
+    float u2 = rng.next();
+    if (u2 <= f) {
+      p.d = rdir;
+      p.o = is.pos;
+      p.depth += 1;
+      return true;
+    }
+    float ior2 = chan(M.ior, p.wl);
+    if (ior2 == 0.0f) return false;
+    // reflect_trans (tracer.rs:111-125)
+    float eta = relative_ior(chan(sc.mats[p.med].ior, p.wl), ior2);
+    V3 t;
+    float c2;
+    if (!specular_refraction(is.n, p.d, eta, t, c2)) return false;
+    p.med = dot(t, is.n) < 0.0f ? is.mat : sc.n_mats;
+    p.d = t;
+    p.o = is.pos;
+    p.depth += 1;
+    return true;
+  }
+  if (M.surface == SURF_TS) {
+    // Surface::next_direction (surface.rs:211-258), medium rule tracer.rs:65-69
+    float eta = relative_ior(chan(sc.mats[p.med].ior, p.wl), chan(M.ior, p.wl));
+    float cos1;
+    V3 rdir0 = specular_reflection(is.n, p.d, cos1);
+    V3 rdir = reflection_glossy(is.n, rdir0, power_glossy(M), rng);
+    V3 hvec = normalize_or(rdir - p.d, is.n);
+    V3 tdir;
+    float cos2;
+    bool has_t = specular_refraction(hvec, p.d, eta, tdir, cos2);
+    float cs = cos1 < cos2 ? cos1 : cos2;
+    float f = schlick(chan(M.c2, p.wl), cs);
+    V3 nd;
+    bool keep_medium = true;
+    if (rng.next() <= f) {
+      nd = rdir;
+    } else if (!(rng.next() <= chan(M.c1, p.wl))) {
+      return false;
+    } else if (rng.next() <= M.diff) {
+      nd = diffuse_reflection(is.n, rng);
+    } else {
+      if (!has_t) return false;
+      nd = tdir;
+      keep_medium = false;
+    }
+    if (!keep_medium) p.med = is.mat;
+    p.d = nd;
+    p.o = is.pos;
+    p.depth += 1;
+    return true;
+  }
+  return false;  // Nothing / DisneyBRDF / Brady: tracer.rs:75
+}
+
+// Photon tracer.  Wave w owns photons [w*per_wave, (w+1)*per_wave) and the
+// output region rec + w*region_cap.  Each wave keeps 64 photons in flight:
+// lanes whose photon died take the next indices of the wave's range (ballot
+// + mbcnt, a wave-uniform counter in registers), and stored records go to
+// consecutive slots of the region (coalesced 32-byte writes).  No global
+// atomics: the record order depends only on the data, so it is reproducible.
+__global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
+                                                       uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
+                                                       uint64_t seed, int uc, int maxd) {
+  const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (w >= n_waves) return;
+  const uint32_t pass = st->pass;
+  const uint32_t lane = lane_id();
+  const uint64_t lo = (uint64_t)w * per_wave;
+  const uint32_t p_begin = (uint32_t)min<uint64_t>(lo, sc.n_emit);
+  const uint32_t p_end = (uint32_t)min<uint64_t>(lo + per_wave, sc.n_emit);
+  PhotonRec* region = rec + (uint64_t)w * region_cap;
+  uint32_t next = p_begin;  // wave-uniform
+  uint32_t cnt = 0;         // wave-uniform
+  bool alive = false, done = false;
+  PhotonState p;
+  Rng rng;
+  p.pid = 0;
+  while (true) {
+    const bool need = !alive && !done;
+    const uint64_t m = __ballot(need);
+    if (m) {
+      if (need) {
+        const uint32_t pid = next + (uint32_t)__popcll(m & lanemask_lt());
+        if (pid < p_end) {
+          rng.init(seed, pid, pass, TAG_PHOTON);
+          emit_photon(sc, pid, rng, p);
+          alive = true;
+        } else {
+          done = true;
+        }
+      }
+      next += (uint32_t)__popcll(m);
+    }
+    if (!__any(alive)) break;
+    bool store = false;
+    V3 spos = mk(0.0f, 0.0f, 0.0f);
+    const V3 sdir = p.d;
+    const uint32_t smeta = pack_meta(p.wl, p.depth, p.pid);
+    if (alive) alive = photon_bounce(sc, uc, maxd, p, rng, store, spos);
+    const uint64_t sm = __ballot(store);
+    if (sm) {
+      if (store) {
+        const uint32_t slot = cnt + (uint32_t)__popcll(sm & lanemask_lt());
+        if (slot < region_cap) {
+          PhotonRec r;
+          r.pm = make_float4(spos.x, spos.y, spos.z, __uint_as_float(smeta));
+          r.dm = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
+          region[slot] = r;
+        }
+      }
+      cnt += (uint32_t)__popcll(sm);
+    }
+  }
+  if (lane == 0) {
+    if (cnt > region_cap) atomicOr(&st->overflow, 1u);
+    count[w] = min(cnt, region_cap);
+  }
+}
+
+// Exclusive scan of up to 65536 counts in one block; the total goes to
+// *total (the pass's stored-photon or sorted-query count).
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uint32_t* offset, uint32_t n,
+                                                      uint32_t* total) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const uint32_t per = (n + 1023) / 1024;
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t i = t * per + k;
+    if (i < n) sum += count[i];
+  }
+  uint32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < 16; ++i) {
+      const uint32_t v = wsum[i];
+      wsum[i] = acc;
+      acc += v;
+    }
+    *total = acc;
+  }
+  __syncthreads();
+  uint32_t run = wsum[w] + incl - sum;
+  for (uint32_t k = 0; k < per; ++k) {
+    const uint32_t i = t * per + k;
+    if (i < n) {
+      const uint32_t c = count[i];
+      offset[i] = run;
+      run += c;
+    }
+  }
+}
+
+uint32_t trace_waves(uint64_t n_photon, int n_cu) {
+  uint64_t w = (n_photon + 255) / 256;
+  const uint64_t maxw = (uint64_t)n_cu * 16u;
+  if (w > maxw) w = maxw;
+  if (w > 8192) w = 8192;
+  if (w < 1) w = 1;
+  return (uint32_t)w;
+}
+
+hipError_t launch_trace_photons(hipStream_t s, const SceneBuf& sc, PassState* st, PhotonBuf& pb, uint64_t seed,
+                                int use_classic, int max_depth) {
+  const uint32_t n_waves = pb.n_regions;
+  const uint32_t per_wave = (uint32_t)((sc.n_emit_total + (uint64_t)n_waves - 1) / n_waves);
+  hipLaunchKernelGGL(k_trace_photons, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+                     pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, pb.count, pb.offset, n_waves, &st->n_stored);
+  return hipGetLastError();
+}
+
+// Region lookup: the last region whose offset is <= i.
+__device__ __forceinline__ uint64_t region_index(const uint32_t* offset, uint32_t n_regions, uint32_t region_cap,
+                                                 uint32_t i) {
+  uint32_t lo = 0, hi = n_regions;  // offset[lo] <= i < offset[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (offset[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return (uint64_t)lo * region_cap + (i - offset[lo]);
+}
+
+__global__ __launch_bounds__(256) void k_compact(const PhotonRec* rec, const uint32_t* offset, uint32_t n_regions,
+                                                 uint32_t region_cap, const PassState* st, PhotonRec* dense) {
+  const uint32_t n = st->n_stored;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dense[i] = rec[region_index(offset, n_regions, region_cap, i)];
+}
+
+hipError_t launch_compact(hipStream_t s, const PhotonBuf& pb, const PassState* st, PhotonRec* dense) {
+  uint64_t blocks = (pb.cap + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_compact, dim3((uint32_t)blocks), dim3(256), 0, s, pb.rec, pb.offset, pb.n_regions,
+                     pb.region_cap, st, dense);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// Photon map build (photonmap.rs:23-29).  The reference inserts photons one
+// by one into the kdtree crate; here: Morton keys in the quantisation box ->
+// LSD radix sort (8-bit digits, device-side count) -> leaves of 64
+// consecutive photons -> balanced binary tree (heap order) of AABBs built
+// bottom-up with arrival counters.
+
+__device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// Morton key (30 bits) of a position in the quantisation box; positions
+// outside get bit 30 and a code in a box 64x as large.
+__device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, float4 bmin, float4 bscale, float fq) {
+  float fx = (px - bmin.x) * bscale.x;
+  float fy = (py - bmin.y) * bscale.y;
+  float fz = (pz - bmin.z) * bscale.z;
+  uint32_t outside = 0;
+  if (!(fx >= 0.0f && fx < fq + 1.0f && fy >= 0.0f && fy < fq + 1.0f && fz >= 0.0f && fz < fq + 1.0f)) {
+    const float h = 0.5f * (fq + 1.0f);
+    fx = (fx - h) * (1.0f / 64.0f) + h;
+    fy = (fy - h) * (1.0f / 64.0f) + h;
+    fz = (fz - h) * (1.0f / 64.0f) + h;
+    outside = 1u << 30;
+  }
+  fx = fminf(fmaxf(fx, 0.0f), fq);
+  fy = fminf(fmaxf(fy, 0.0f), fq);
+  fz = fminf(fmaxf(fz, 0.0f), fq);
+  return outside | (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) |
+         expand_bits10((uint32_t)fz);
+}
+
+static float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax) {
+  float4 scale;
+  scale.x = (float)((double)(qmax + 1) / fmax((double)bmax.x - (double)bmin.x, 1e-30));
+  scale.y = (float)((double)(qmax + 1) / fmax((double)bmax.y - (double)bmin.y, 1e-30));
+  scale.z = (float)((double)(qmax + 1) / fmax((double)bmax.z - (double)bmin.z, 1e-30));
+  scale.w = 0.0f;
+  return scale;
+}
+
+
+// Key = 30-bit Morton code of the position quantised in the box; photons
+// outside the box (e.g. walks that leaked through a corner of infinite
+// planes) get bit 30 and a Morton code in a box 64x as large, so they sort
+// into leaves of their own instead of widening the leaves at the faces.
+__global__ __launch_bounds__(256) void k_morton(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
+                                                uint32_t region_cap, const PassState* st, uint32_t* keys,
+                                                uint32_t* vals, float4 bmin, float4 bscale, uint32_t qmax,
+                                                uint32_t* nonfinite) {
+  const uint32_t n = st->n_stored;
+  const float fq = (float)qmax;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t src = region_index(offset, n_regions, region_cap, i);
+    const float4 p = ph[src].pm;
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(nonfinite, 1u);
+    keys[i] = morton_key(p.x, p.y, p.z, bmin, bscale, fq);
+    vals[i] = (uint32_t)src;
+  }
+}
+
+// --- LSD radix sort, 8-bit digits, count on the device ---------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_TILE = RS_THREADS * 8;
+
+__device__ __forceinline__ void rs_range(uint32_t n, uint32_t G, uint32_t b, uint32_t& beg, uint32_t& end) {
+  uint32_t per = (n + G - 1) / G;
+  per = (per + RS_TILE - 1) / RS_TILE * RS_TILE;
+  uint64_t bb = (uint64_t)b * per;
+  uint64_t ee = bb + per;
+  beg = bb < n ? (uint32_t)bb : n;
+  end = ee < n ? (uint32_t)ee : n;
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, const uint32_t* np,
+                                                         uint32_t* ghist, uint32_t shift) {
+  __shared__ uint32_t h[256];
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t beg, end;
+  rs_range(*np, G, b, beg, end);
+  for (uint32_t i = beg + threadIdx.x; i < end; i += RS_THREADS) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  ghist[threadIdx.x * G + b] = h[threadIdx.x];
+}
+
+// One block per digit: exclusive scan of that digit's per-block counts.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* ghist, uint32_t* totals, uint32_t G) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t d = blockIdx.x;
+  uint32_t* row = ghist + (uint64_t)d * G;
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (G + 1023) / 1024;  // entries per thread
+  uint32_t local[4] = {0, 0, 0, 0};
+  uint32_t sum = 0;
+  for (uint32_t k = 0; k < per && k < 4; ++k) {
+    uint32_t i = t * per + k;
+    local[k] = i < G ? row[i] : 0u;
+    sum += local[k];
+  }
+  // block exclusive scan of `sum`
+  uint32_t lane = t & 63u, w = t >> 6;
+  uint32_t incl = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t v = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (t == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < 16; ++i) {
+      uint32_t v = wsum[i];
+      wsum[i] = acc;
+      acc += v;
+    }
+    totals[d] = acc;
+  }
+  __syncthreads();
+  uint32_t run = wsum[w] + incl - sum;
+  for (uint32_t k = 0; k < per && k < 4; ++k) {
+    uint32_t i = t * per + k;
+    if (i < G) row[i] = run;
+    run += local[k];
+  }
+}
+
+// Stable scatter.  Each round moves 256 keys: lanes with equal digits are
+// matched with 8 ballots, ranked by mbcnt, and placed after the same digit's
+// keys of earlier waves and earlier rounds.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                            uint32_t* vout, const uint32_t* np,
+                                                            const uint32_t* ghist, const uint32_t* totals,
+                                                            uint32_t shift) {
+  __shared__ uint32_t run_off[256];
+  __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+  __shared__ uint32_t dig[256];
+  const uint32_t G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+  const uint32_t lane = t & 63u, w = t >> 6;
+  // exclusive scan of the digit totals
+  dig[t] = totals[t];
+  __syncthreads();
+  if (t < 64) {
+    uint32_t a0 = dig[t * 4], a1 = dig[t * 4 + 1], a2 = dig[t * 4 + 2], a3 = dig[t * 4 + 3];
+    uint32_t s = a0 + a1 + a2 + a3;
+    uint32_t incl = s;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t v = __shfl_up(incl, o);
+      if (lane >= (uint32_t)o) incl += v;
+    }
+    uint32_t ex = incl - s;
+    dig[t * 4] = ex;
+    dig[t * 4 + 1] = ex + a0;
+    dig[t * 4 + 2] = ex + a0 + a1;
+    dig[t * 4 + 3] = ex + a0 + a1 + a2;
+  }
+  __syncthreads();
+  run_off[t] = dig[t] + ghist[t * G + b];
+  for (uint32_t k = 0; k < RS_THREADS / 64; ++k) wcnt[k][t] = 0;
+  __syncthreads();
+  uint32_t beg, end;
+  rs_range(*np, G, b, beg, end);
+  for (uint32_t base = beg; base < end; base += RS_THREADS) {
+    uint32_t i = base + t;
+    bool valid = i < end;
+    uint32_t key = 0, val = 0, d = 0;
+    if (valid) {
+      key = kin[i];
+      val = vin[i];
+      d = (key >> shift) & 255u;
+    }
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+      peers &= ((d >> bit) & 1u) ? bb : ~bb;
+    }
+    uint32_t rank = (uint32_t)__popcll(peers & lanemask_lt());
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pre = 0;
+      for (uint32_t k = 0; k < w; ++k) pre += wcnt[k][d];
+      uint32_t pos = run_off[d] + pre + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    uint32_t s = 0;
+    for (uint32_t k = 0; k < RS_THREADS / 64; ++k) {
+      s += wcnt[k][t];
+      wcnt[k][t] = 0;
+    }
+    run_off[t] += s;
+    __syncthreads();
+  }
+}
+
+// Leaves + tree.  The tree is a complete binary tree in heap order over
+// lpad = 2^depth leaves of 64 consecutive photons (empty leaves have empty
+// boxes).  k_build_leaves: one block per 64 leaves; each wave gathers 16
+// leaves' photons into tree order (coalesced 32-byte writes), reduces their
+// boxes with cross-lane shuffles, and the block folds its 64 boxes up six
+// levels in LDS.  k_build_top: one block folds the remaining top levels.  No
+// cross-workgroup synchronisation (an agent-scope acquire/release would
+// write back and invalidate the L2 of every XCD).
+__device__ __forceinline__ float wmin(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ float wmax(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  return v;
+}
+
+__device__ __forceinline__ void tree_shape(uint32_t n, uint32_t& L, uint32_t& depth) {
+  L = (n + LEAF - 1) / LEAF;
+  depth = 0;
+  while ((1u << depth) < L) ++depth;
+}
+
+__device__ __forceinline__ Node node_union(const Node& a, const Node& c) {
+  Node u;
+  u.lo = make_float4(fminf(a.lo.x, c.lo.x), fminf(a.lo.y, c.lo.y), fminf(a.lo.z, c.lo.z), 0.0f);
+  u.hi = make_float4(fmaxf(a.hi.x, c.hi.x), fmaxf(a.hi.y, c.hi.y), fmaxf(a.hi.z, c.hi.z), 0.0f);
+  return u;
+}
+
+constexpr uint32_t SUB_LEAVES = 64;  // leaves per block of k_build_leaves
+constexpr uint32_t SUB_LEVELS = 6;   // log2(SUB_LEAVES)
+
+__global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const double* in64, const uint32_t* order,
+                                                      PhotonRec* out, double* out64, uint32_t* src_idx, Node* nodes,
+                                                      PassState* st) {
+  __shared__ Node box[SUB_LEAVES];
+  const uint32_t n = st->n_stored;
+  uint32_t L, depth;
+  tree_shape(n, L, depth);
+  const uint32_t lpad = 1u << depth;
+  const uint32_t groups = (lpad + SUB_LEAVES - 1) / SUB_LEAVES;
+  const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->n_leaves = L;
+    st->depth = depth;
+  }
+  for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+    for (uint32_t k = 0; k < SUB_LEAVES / 4; ++k) {
+      const uint32_t jl = wib * (SUB_LEAVES / 4) + k;  // leaf within the group
+      const uint32_t j = g * SUB_LEAVES + jl;
+      float lx = INF_F, ly = INF_F, lz = INF_F, hx = -INF_F, hy = -INF_F, hz = -INF_F;
+      const uint32_t e = j * LEAF + lane;
+      if (j < L && e < n) {
+        const uint32_t src = order[e];
+        const PhotonRec r = in[src];
+        out[e] = r;
+        src_idx[e] = src;
+        if (in64) {
+          out64[3 * (uint64_t)e + 0] = in64[3 * (uint64_t)src + 0];
+          out64[3 * (uint64_t)e + 1] = in64[3 * (uint64_t)src + 1];
+          out64[3 * (uint64_t)e + 2] = in64[3 * (uint64_t)src + 2];
+        }
+        lx = hx = r.pm.x;
+        ly = hy = r.pm.y;
+        lz = hz = r.pm.z;
+      }
+      lx = wmin(lx); ly = wmin(ly); lz = wmin(lz);
+      hx = wmax(hx); hy = wmax(hy); hz = wmax(hz);
+      if (lane == 0) {
+        Node nd;
+        nd.lo = make_float4(lx, ly, lz, 0.0f);
+        nd.hi = make_float4(hx, hy, hz, 0.0f);
+        box[jl] = nd;
+        if (j < lpad) nodes[lpad - 1 + j] = nd;
+      }
+    }
+    __syncthreads();
+    // fold up to six levels inside the group
+    uint32_t cnt = min(SUB_LEAVES, lpad);
+    for (uint32_t lev = 1; lev <= SUB_LEVELS && cnt > 1; ++lev) {
+      cnt >>= 1;
+      Node u;
+      const uint32_t t = threadIdx.x;
+      if (t < cnt) u = node_union(box[2 * t], box[2 * t + 1]);
+      __syncthreads();
+      if (t < cnt) {
+        box[t] = u;
+        const uint32_t level = depth - lev;
+        nodes[((1u << level) - 1u) + ((g * SUB_LEAVES) >> lev) + t] = u;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_build_top(Node* nodes, const PassState* st) {
+  uint32_t L, depth;
+  tree_shape(st->n_stored, L, depth);
+  if (depth <= SUB_LEVELS) return;
+  for (int level = (int)depth - (int)SUB_LEVELS - 1; level >= 0; --level) {
+    const uint32_t first = (1u << level) - 1u;
+    for (uint32_t i = threadIdx.x; i < (1u << level); i += blockDim.x) {
+      const uint32_t nd = first + i;
+      nodes[nd] = node_union(nodes[2 * nd + 1], nodes[2 * nd + 2]);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// LSD radix sort of (key, value) pairs with the count on the device; the
+// result lands in keys[*cur] / vals[*cur].
+static hipError_t radix_sort(hipStream_t s, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np, uint32_t* ghist,
+                             uint32_t* totals, uint32_t blocks, uint32_t key_bits, int* cur) {
+  *cur = 0;
+  for (uint32_t shift = 0; shift < key_bits; shift += 8) {
+    hipLaunchKernelGGL(k_rs_hist, dim3(blocks), dim3(RS_THREADS), 0, s, keys[*cur], np, ghist, shift);
+    hipLaunchKernelGGL(k_rs_scan, dim3(256), dim3(1024), 0, s, ghist, totals, blocks);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(blocks), dim3(RS_THREADS), 0, s, keys[*cur], vals[*cur], keys[*cur ^ 1],
+                       vals[*cur ^ 1], np, ghist, totals, shift);
+    *cur ^= 1;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax) {
+  const uint32_t qmax = (1u << m.morton_bits) - 1u;
+  const float4 scale = morton_scale(bmin, bmax, qmax);
+  uint32_t mblocks = (uint32_t)((m.cap + 255) / 256);
+  if (mblocks > 8192) mblocks = 8192;
+  if (mblocks == 0) mblocks = 1;
+  hipLaunchKernelGGL(k_morton, dim3(mblocks), dim3(256), 0, s, in.rec, in.offset, in.n_regions, in.region_cap, st,
+                     m.keys[0], m.vals[0], bmin, scale, qmax, &st->nonfinite);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int cur = 0;
+  e = radix_sort(s, m.keys, m.vals, &st->n_stored, m.ghist, m.totals, m.sort_blocks, 31u, &cur);
+  if (e != hipSuccess) return e;
+  uint32_t groups = (m.lpad_max + SUB_LEAVES - 1) / SUB_LEAVES;
+  if (groups > 16384) groups = 16384;
+  hipLaunchKernelGGL(k_build_leaves, dim3(groups), dim3(256), 0, s, in.rec, in.rec64, m.vals[cur], m.sorted,
+                     m.sorted64, m.src_idx, m.nodes, st);
+  hipLaunchKernelGGL(k_build_top, dim3(1), dim3(1024), 0, s, m.nodes, st);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// Eye rays (tracer.rs:129-177).  The reference recurses over a binary tree
+// (reflection child + refraction child).  The BSDF is linear in the child
+// radiances, so the tree is walked depth-first with a per-lane throughput:
+// every node adds  throughput * (emittance/(2 pi) + wd * direct)  to its
+// pixel and emits one gather query with weight throughput * wd; children
+// whose weight is exactly zero in all channels contribute nothing and are
+// not walked.  The gather itself runs in k_gather over all queries.
+
+struct EyeNode {
+  V3 o, d;
+  C3 w;
+  uint32_t meta;  // depth (5 bits) | medium (11 bits) << 5 | path (16 bits) << 16
+};
+
+__global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
+                                             int maxd) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t wave = t >> 6;  // = tile index
+  const uint32_t pass = st->pass;
+  const uint32_t wi = t & 63u;
+  const int x = (int)((wave % (uint32_t)sc.cam.tiles_x) * 8u + (wi & 7u));
+  const int y = (int)((wave / (uint32_t)sc.cam.tiles_x) * 8u + (wi >> 3));
+  const bool valid = t < eb.q_primary && x < sc.cam.xreso && y < sc.cam.yreso;
+  if (t < eb.q_primary) eb.q[t].a.w = __uint_as_float(NO_PIXEL);
+  const uint32_t pixel = valid ? (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x : 0u;
+  C3 acc{0.0f, 0.0f, 0.0f};
+  uint32_t n_sec = 0;
+  bool primary_q = false;
+  if (valid) {
+    Rng rng;
+    rng.init(seed, pixel, pass, TAG_EYE);
+    EyeNode cur;
+    camera_ray(sc.cam, x, y, rng, cur.o, cur.d);
+    const uint32_t cam_draws = rng.n;
+    cur.w = C3{1.0f, 1.0f, 1.0f};
+    cur.meta = (sc.n_mats << 5);
+    EyeNode stack[MAX_DEPTH_LIMIT];
+    int sp = 0;
+    bool have = true;
+    while (have) {
+      const uint32_t depth = cur.meta & 31u;
+      const uint32_t med = (cur.meta >> 5) & 2047u;
+      const uint32_t path = cur.meta >> 16;
+      bool next_set = false;
+      EyeNode next;
+      Isect is;
+      if ((int)depth < maxd && intersect(sc, cur.o, cur.d, is)) {
+        const DevMaterial& M = sc.mats[is.mat];
+        const DevMaterial& Mm = sc.mats[med];
+        Rng nr;
+        nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
+        if (depth == 0) nr.n = cam_draws, nr.blk = rng.blk;
+        acc = acc + cur.w * (c3(M.em) * SR_HALF_F);
+        float cos1;
+        V3 rdir = specular_reflection(is.n, cur.d, cos1);
+        const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
+        V3 tdir = mk(0.0f, 0.0f, 0.0f);
+        bool has_t = false;
+        float cos2 = 0.0f;
+        if (spec_part) {
+          // tracer.rs:150-161
+          rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
+          const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
+          const V3 hvec = normalize_or(rdir - cur.d, is.n);
+          has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+        }
+        const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
+        const BsdfW W = bsdf_weights(M, cos0);
+        const C3 wd = cur.w * W.wd;
+        if (nonzero(wd)) {
+          if (uc) acc = acc + wd * direct_light(sc, is.pos, is.n);  // tracer.rs:136-144
+          uint32_t slot = t;
+          if (depth == 0) primary_q = true;
+          if (depth > 0) {
+            // deeper query: the pixel's own slots, else the overflow region
+            if (n_sec < eb.sec_max) {
+              slot = eb.q_primary * (1u + n_sec) + t;
+              ++n_sec;
+            } else {
+              const uint32_t j = atomicAdd(&st->n_queries, 1u);
+              slot = j < eb.ovf_cap ? eb.q_primary * (1u + eb.sec_max) + j : 0xFFFFFFFFu;
+              if (j >= eb.ovf_cap) atomicOr(&st->overflow, 2u);
+            }
+          }
+          if (slot != 0xFFFFFFFFu) {
+            Query qq;
+            qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
+            qq.b = make_float4(is.n.x, is.n.y, is.n.z, 0.0f);
+            qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
+            eb.q[slot] = qq;
+          }
+        }
+        if ((int)depth + 1 < maxd) {
+          const C3 ws = cur.w * W.ws;
+          const C3 wt = cur.w * W.wt;
+          const bool rc = surf_reflect(M, cos1) && nonzero(ws);
+          const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
+          if (tc) {
+            const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
+            const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
+            if (rc) {
+              if (sp < MAX_DEPTH_LIMIT) stack[sp++] = c;
+            } else {
+              next = c;
+              next_set = true;
+            }
+          }
+          if (rc) {
+            next = EyeNode{is.pos, rdir, ws, (depth + 1) | (med << 5) | (path << 16)};
+            next_set = true;
+          }
+        }
+      }
+      if (next_set) {
+        cur = next;
+      } else if (sp > 0) {
+        cur = stack[--sp];
+      } else {
+        have = false;
+      }
+    }
+    eb.pix[pixel] = make_float4(acc.r, acc.g, acc.b, 0.0f);
+  }
+  if (t < eb.q_primary) eb.sec_count[t] = n_sec;
+  // queries of this tile kept in place (primary + deeper), for the sort
+  uint32_t c = (primary_q ? 1u : 0u) + n_sec;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((threadIdx.x & 63u) == 0 && wave < eb.n_tiles) eb.tile_count[wave] = c;
+}
+
+hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
+                      int use_classic, int max_depth) {
+  const uint32_t blocks = (eb.q_primary + 255) / 256;
+  hipLaunchKernelGGL(k_eye, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic, max_depth);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// Wave-coherent radius query over the heap-ordered tree.  Stackless DFS:
+// descend to the left child while any lane's ball overlaps the node box;
+// otherwise step to the next sibling, climbing while at a right child.  At a
+// leaf, the 64 photons are staged once into this wave's LDS slice with
+// coalesced loads and every lane tests them from LDS (broadcast reads).
+
+struct LeafLds {
+  float4 pm[LEAF];
+  float4 dm[LEAF];
+};
+
+template <typename OnPhoton>
+__device__ __forceinline__ void wave_query(const Node* nodes, const PhotonRec* ph, uint32_t n, uint32_t depth,
+                                           bool active, float qx, float qy, float qz, float r2box, LeafLds& lds,
+                                           OnPhoton&& on_photon) {
+  const uint32_t lane = lane_id();
+  const uint32_t first_leaf = (1u << depth) - 1u;
+  uint32_t node = 0;
+  if (n == 0) return;
+  while (true) {
+    node = __builtin_amdgcn_readfirstlane(node);
+    const Node nd = ldc(nodes, node);
+    const bool ov = active && box_dist2_nc(qx, qy, qz, nd.lo, nd.hi) <= r2box;
+    if (__ballot(ov)) {
+      if (node < first_leaf) {
+        node = 2 * node + 1;
+        continue;
+      }
+      const uint32_t base = (node - first_leaf) * LEAF;
+      const uint32_t cnt = base < n ? min((uint32_t)LEAF, n - base) : 0u;
+      if (lane < cnt) {
+        const PhotonRec r = ph[base + lane];
+        lds.pm[lane] = r.pm;
+        lds.dm[lane] = r.dm;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (ov) {
+        for (uint32_t j = 0; j < cnt; ++j) on_photon(base + j, lds.pm[j], lds.dm[j]);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    while (node != 0 && (node & 1u) == 0) node = (node - 1) >> 1;
+    if (node == 0) break;
+    node += 1;
+  }
+}
+
+// Gather of one query: Σ filter * max(0, -n.dir) per channel over the ball
+// (estimate_radiance, tracer.rs:179-195; photon_to_radiance, optics.rs:224-233).
+struct GatherAcc {
+  float r, g, b;
+};
+
+__device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const PhotonRec* ph, uint32_t n, uint32_t depth,
+                                                  bool active, const Query& qq, float r2, int pf, LeafLds& lds) {
+  GatherAcc a{0.0f, 0.0f, 0.0f};
+  const float qx = qq.a.x, qy = qq.a.y, qz = qq.a.z;
+  const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+  const float inv_r2 = 1.0f / r2;
+  wave_query(nodes, ph, n, depth, active, qx, qy, qz, r2, lds, [&](uint32_t, float4 pm, float4 dm) {
+    const float d2 = dist2_nc(qx, qy, qz, pm.x, pm.y, pm.z);
+    if (d2 <= r2) {
+      const float c = nx * dm.x + ny * dm.y + nz * dm.z;
+      const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) : 0.0f;
+      const uint32_t wl = __float_as_uint(pm.w) & 3u;
+      a.r += wl == 0 ? v : 0.0f;
+      a.g += wl == 1 ? v : 0.0f;
+      a.b += wl == 2 ? v : 0.0f;
+    }
+  });
+  return a;
+}
+
+// Overflow queries (deeper eye-tree nodes beyond a pixel's own slots):
+// added into their pixel with float atomics before the accumulation.
+__global__ __launch_bounds__(256) void k_gather_overflow(const Node* nodes, const PhotonRec* ph, PassState* st,
+                                                         EyeBuf eb, int pf) {
+  __shared__ LeafLds lds[4];
+  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t n = st->n_stored, depth = st->depth;
+  const float r2 = st->r2f, scale = st->scale;
+  const uint32_t n_ovf = min(st->n_queries, eb.ovf_cap);
+  const uint32_t groups = (n_ovf + 63) / 64;
+  const uint32_t waves = gridDim.x * 4;
+  const uint64_t base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
+  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
+    const uint32_t i = g * 64 + lane;
+    Query qq{};
+    const bool active = i < n_ovf;
+    if (active) qq = eb.q[base + i];
+    const GatherAcc a = gather_query(nodes, ph, n, depth, active, qq, r2, pf, lds[wib]);
+    if (active && (a.r != 0.0f || a.g != 0.0f || a.b != 0.0f)) {
+      const uint32_t p = __float_as_uint(qq.a.w);
+      atomicAdd(&eb.pix[p].x, a.r * scale * qq.w.x);
+      atomicAdd(&eb.pix[p].y, a.g * scale * qq.w.y);
+      atomicAdd(&eb.pix[p].z, a.b * scale * qq.w.z);
+    }
+  }
+}
+
+// Sort keys of the in-place queries: one wave per tile; lane i writes its
+// pixel's queries (primary, then deeper ones) at the tile's offset plus the
+// prefix of the earlier lanes' counts.  Key = Morton code of the position.
+__global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
+                                                    float fq, uint32_t* keys, uint32_t* vals) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t g = t >> 6, lane = lane_id();
+  if (g >= eb.n_tiles) return;
+  const bool primary = __float_as_uint(eb.q[t].a.w) != NO_PIXEL;
+  const uint32_t nsec = eb.sec_count[t];
+  const uint32_t c = (primary ? 1u : 0u) + nsec;
+  uint32_t incl = c;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= (uint32_t)o) incl += v;
+  }
+  uint32_t pos = eb.tile_off[g] + incl - c;
+  if (primary) {
+    const float4 a = eb.q[t].a;
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+    vals[pos] = t;
+    ++pos;
+  }
+  for (uint32_t k = 0; k < nsec; ++k) {
+    const uint32_t slot = eb.q_primary * (1u + k) + t;
+    const float4 a = eb.q[slot].a;
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+    vals[pos] = slot;
+    ++pos;
+  }
+}
+
+// Gather in Morton order: wave w takes sorted queries [64 w, 64 w + 64),
+// which lie close together in space, so the union of their balls touches
+// few leaves.  res[slot] = w * scale * Σ (per channel).
+__global__ __launch_bounds__(256) void k_gather_sorted(const Node* nodes, const PhotonRec* ph, const PassState* st,
+                                                       EyeBuf eb, const uint32_t* order, int pf) {
+  __shared__ LeafLds lds[4];
+  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t n = st->n_stored, depth = st->depth;
+  const float r2 = st->r2f, scale = st->scale;
+  const uint32_t nq = st->n_qsort;
+  const uint32_t groups = (nq + 63) / 64;
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
+    const uint32_t i = g * 64 + lane;
+    const bool active = i < nq;
+    uint32_t slot = 0;
+    Query qq{};
+    if (active) {
+      slot = order[i];
+      qq = eb.q[slot];
+    }
+    const GatherAcc a = gather_query(nodes, ph, n, depth, active, qq, r2, pf, lds[wib]);
+    if (active) eb.res[slot] = make_float4(a.r * scale * qq.w.x, a.g * scale * qq.w.y, a.b * scale * qq.w.z, 0.0f);
+  }
+}
+
+// Per-pixel pass radiance = pix (emission + direct light + overflow
+// gathers) + res of the pixel's primary and deeper queries, in a fixed
+// order; accumulated into the FP64 sums (util/averager2.rb:58-60) when sum
+// is given, else stored back into pix.
+__global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, double* sum, double* sumsq) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= eb.q_primary) return;
+  const uint32_t g = t >> 6, wi = t & 63u;
+  const int x = (int)((g % (uint32_t)cam.tiles_x) * 8u + (wi & 7u));
+  const int y = (int)((g / (uint32_t)cam.tiles_x) * 8u + (wi >> 3));
+  if (x >= cam.xreso || y >= cam.yreso) return;
+  const uint32_t p = (uint32_t)y * (uint32_t)cam.xreso + (uint32_t)x;
+  float4 c = eb.pix[p];
+  if (__float_as_uint(eb.q[t].a.w) != NO_PIXEL) {
+    const float4 r = eb.res[t];
+    c.x += r.x;
+    c.y += r.y;
+    c.z += r.z;
+  }
+  const uint32_t nsec = eb.sec_count[t];
+  for (uint32_t k = 0; k < nsec; ++k) {
+    const float4 r = eb.res[eb.q_primary * (1u + k) + t];
+    c.x += r.x;
+    c.y += r.y;
+    c.z += r.z;
+  }
+  if (sum) {
+    const double r = c.x, gg = c.y, b = c.z;
+    double* s3 = sum + 3 * (uint64_t)p;
+    double* q3 = sumsq + 3 * (uint64_t)p;
+    s3[0] += r;
+    s3[1] += gg;
+    s3[2] += b;
+    q3[0] += r * r;
+    q3[1] += gg * gg;
+    q3[2] += b * b;
+  } else {
+    eb.pix[p] = c;
+  }
+}
+
+hipError_t launch_gather(hipStream_t s, const MapBuf& m, PassState* st, const EyeBuf& eb, const DevCamera& cam,
+                         float4 bmin, float4 bmax, int pfilter, int n_cu, double* sum, double* sumsq,
+                         hipEvent_t* ev) {
+  const uint32_t blocks = (uint32_t)n_cu * 8u;
+  hipLaunchKernelGGL(k_gather_overflow, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, st, eb, pfilter);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, eb.tile_count, eb.tile_off, eb.n_tiles,
+                     &st->n_qsort);
+  const uint32_t qmax = (1u << m.morton_bits) - 1u;
+  hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, st, bmin,
+                     morton_scale(bmin, bmax, qmax), (float)qmax, eb.qkeys[0], eb.qvals[0]);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  uint32_t* qk[2] = {eb.qkeys[0], eb.qkeys[1]};
+  uint32_t* qv[2] = {eb.qvals[0], eb.qvals[1]};
+  int cur = 0;
+  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 31u, &cur);
+  if (e != hipSuccess) return e;
+  if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_gather_sorted, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, st, eb, qv[cur], pfilter);
+  if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, cam, sum, sumsq);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// Test hooks: radius query (KdTree::within) and gather (estimate_radiance) on
+// host-given query points.  refine64 decides photons whose float squared
+// distance lies within the float error band of r^2 again in double, from the
+// double positions kept by ppm_photons_upload, which reproduces the FP64
+// predicate of the reference exactly.
+
+// Membership of photon j; *t_out receives d2 / r2 for the filter weight,
+// formed from the double distance when refining.
+__device__ __forceinline__ bool in_ball(float d2f, float r2f, float band, bool refine, const double* p64,
+                                        uint32_t j, double qx, double qy, double qz, double r2, float* t_out) {
+  if (!refine) {
+    *t_out = d2f / r2f;
+    return d2f <= r2f;
+  }
+  const double dx = qx - p64[3 * (uint64_t)j + 0];
+  const double dy = qy - p64[3 * (uint64_t)j + 1];
+  const double dz = qz - p64[3 * (uint64_t)j + 2];
+  double s;
+  {
+#pragma clang fp contract(off)
+    s = dx * dx + dy * dy;
+    s = s + dz * dz;
+  }
+  *t_out = (float)(s / r2);
+  if (d2f < r2f - band) return true;
+  if (d2f > r2f + band) return false;
+  return s <= r2;
+}
+
+__global__ __launch_bounds__(256) void k_query(const Node* nodes, const PhotonRec* ph, const double* ph64,
+                                               const uint32_t* src_idx, const PassState* st, const float4* qf,
+                                               const double* q64, const float4* qn, uint32_t nq, double r2,
+                                               int refine, int mode, int pf, uint32_t* counts,
+                                               const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out) {
+  __shared__ LeafLds lds[4];
+  const uint32_t w_in_b = threadIdx.x >> 6;
+  const uint32_t lane = lane_id();
+  const uint32_t n = st->n_stored;
+  const uint32_t depth = st->depth;
+  const float r2f = (float)r2;
+  const uint32_t groups = (nq + 63) / 64;
+  const uint32_t waves = gridDim.x * 4;
+  for (uint32_t g = blockIdx.x * 4 + w_in_b; g < groups; g += waves) {
+    const uint32_t i = g * 64 + lane;
+    const bool active = i < nq;
+    const float4 qp = active ? qf[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 nn = (active && mode == 2) ? qn[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    double qx = 0, qy = 0, qz = 0;
+    float band = 0.0f;
+    if (active && refine) {
+      qx = q64[3 * (uint64_t)i];
+      qy = q64[3 * (uint64_t)i + 1];
+      qz = q64[3 * (uint64_t)i + 2];
+      // generous bound on |fl32(d2) - d2| for photons near the sphere
+      const double sc = fmax(fmax(fabs(qx), fabs(qy)), fabs(qz)) + sqrt(r2);
+      band = (float)(64.0 * 5.9604644775390625e-8 * (sc * sqrt(r2) + r2) + 1e-30);
+    }
+    const float r2box = refine ? r2f + band : r2f;
+    uint32_t cnt = 0;
+    const uint64_t off = (mode == 1 && active) ? offsets[i] : 0;
+    float ar = 0.f, ag = 0.f, ab = 0.f;
+    wave_query(nodes, ph, n, depth, active, qp.x, qp.y, qp.z, r2box, lds[w_in_b],
+               [&](uint32_t j, float4 pm, float4 dm) {
+                 const float d2 = dist2_nc(qp.x, qp.y, qp.z, pm.x, pm.y, pm.z);
+                 float t;
+                 if (in_ball(d2, r2f, band, refine != 0, ph64, j, qx, qy, qz, r2, &t)) {
+                   if (mode == 1) idx_out[off + cnt] = src_idx[j];
+                   ++cnt;
+                   if (mode == 2) {
+                     const float c = nn.x * dm.x + nn.y * dm.y + nn.z * dm.z;
+                     const float v = c < 0.0f ? -c * filter_weight(pf, t) : 0.0f;
+                     const uint32_t wl = __float_as_uint(pm.w) & 3u;
+                     ar += wl == 0 ? v : 0.0f;
+                     ag += wl == 1 ? v : 0.0f;
+                     ab += wl == 2 ? v : 0.0f;
+                   }
+                 }
+               });
+    if (active) {
+      if (mode == 0) counts[i] = cnt;
+      if (mode == 2) rgb_out[i] = make_float4(ar, ag, ab, 0.0f);
+    }
+  }
+}
+
+hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const float4* qf, const double* q64,
+                        const float4* qn, uint32_t nq, double r2, int refine64, int mode, int pfilter,
+                        uint32_t* counts, const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out, int n_cu) {
+  uint32_t groups = (nq + 63) / 64;
+  uint32_t blocks = (groups + 3) / 4;
+  uint32_t maxb = (uint32_t)n_cu * 8u;
+  if (blocks > maxb) blocks = maxb;
+  if (blocks == 0) blocks = 1;
+  hipLaunchKernelGGL(k_query, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, m.sorted64, m.src_idx, st, qf, q64,
+                     qn, nq, r2, refine64, mode, pfilter, counts, offsets, idx_out, rgb_out);
+  return hipGetLastError();
+}
+
+}  // namespace ppm

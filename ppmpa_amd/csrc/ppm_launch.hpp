@@ -1,0 +1,110 @@
+// ppm_launch.hpp — host-side launchers of the gfx950 kernels (ppm_kernels.hip),
+// called by the C ABI in ppm_api.cpp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "ppm_layout.hpp"
+
+namespace ppm {
+
+// Device-side scene (pointers into device memory + camera + bounds).
+struct SceneBuf {
+  DevShape* shapes = nullptr;
+  DevMaterial* mats = nullptr;  // n_mats + 1 (M_AIR last)
+  DevLight* lights = nullptr;
+  uint32_t n_shapes = 0, n_mats = 0, n_lights = 0;
+  uint32_t n_emit_total = 0;  // photons per pass over all lights
+  DevCamera cam{};
+  float4 bmin{}, bmax{};
+};
+
+// Photons in regions: region r holds count[r] records starting at
+// rec + r * region_cap; offset[r] is the exclusive prefix of the counts, so
+// the set reads densely as i -> region r = last with offset[r] <= i.
+// A traced set has one region per tracing wave; an uploaded set has one.
+struct PhotonBuf {
+  PhotonRec* rec = nullptr;
+  double* rec64 = nullptr;   // double positions (uploaded sets only)
+  uint32_t* count = nullptr;
+  uint32_t* offset = nullptr;
+  uint32_t n_regions = 0;    // regions in use
+  uint32_t max_regions = 0;  // capacity of count/offset
+  uint32_t region_cap = 0;
+  uint64_t cap = 0;          // records allocated
+};
+
+struct MapBuf {
+  uint64_t cap = 0;
+  PhotonRec* sorted = nullptr;  // photons in tree order
+  double* sorted64 = nullptr;   // optional double positions in tree order (3 per photon)
+  uint32_t* src_idx = nullptr;  // tree order -> index in the input set
+  Node* nodes = nullptr;        // 2 * lpad_max - 1
+  uint32_t lpad_max = 0;
+  uint32_t* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t* ghist = nullptr;    // 256 * sort_blocks
+  uint32_t* totals = nullptr;   // 256
+  uint32_t sort_blocks = 0;
+  int morton_bits = 10;         // per axis
+};
+
+// Eye-pass buffers.  Queries: slot t in [0, q_primary) is pixel t's primary
+// query (8x8-tile order); the k-th deeper query of pixel t (k < sec_max)
+// sits at q_primary * (1 + k) + t, so round k of a tile reads 64 consecutive
+// slots; further ones go to an overflow region of ovf_cap slots taken with a
+// global counter.  sec_count[t] = deeper queries of pixel t kept in place.
+// The gather sorts the in-place queries along a Morton curve of their
+// positions (qkeys/qvals, per-tile counts tile_count -> tile_off) so that
+// each wave gathers 64 spatially adjacent queries; res[slot] receives each
+// query's weighted radiance.
+struct EyeBuf {
+  Query* q = nullptr;
+  uint32_t q_primary = 0;
+  uint32_t sec_max = 0;
+  uint32_t* sec_count = nullptr;
+  uint32_t ovf_cap = 0;
+  float4* pix = nullptr;  // per-pixel float radiance of the pass (row-major)
+  uint32_t n_pix = 0;
+  uint32_t n_tiles = 0;
+  uint32_t* tile_count = nullptr;
+  uint32_t* tile_off = nullptr;
+  uint32_t* qkeys[2] = {nullptr, nullptr};
+  uint32_t* qvals[2] = {nullptr, nullptr};
+  uint32_t* qghist = nullptr;
+  uint32_t* qtotals = nullptr;
+  uint32_t q_sort_blocks = 0;
+  float4* res = nullptr;
+};
+
+hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
+                             double power);
+hipError_t launch_pass_end(hipStream_t s, PassState* st);
+
+// Number of tracing waves (= photon regions) used for n photons.
+uint32_t trace_waves(uint64_t n_photon, int n_cu);
+
+hipError_t launch_trace_photons(hipStream_t s, const SceneBuf& sc, PassState* st, PhotonBuf& pb, uint64_t seed,
+                                int use_classic, int max_depth);
+hipError_t launch_compact(hipStream_t s, const PhotonBuf& pb, const PassState* st, PhotonRec* dense);
+
+hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax);
+
+hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
+                      int use_classic, int max_depth);
+
+// Gathers of one pass.  sum/sumsq != null: the pixel's pass radiance is
+// added into the FP64 sums; otherwise it is left in pix.
+// ev (optional, 2 events) brackets the k_gather_sorted launch alone.
+hipError_t launch_gather(hipStream_t s, const MapBuf& m, PassState* st, const EyeBuf& eb, const DevCamera& cam,
+                         float4 bmin, float4 bmax, int pfilter, int n_cu, double* sum, double* sumsq,
+                         hipEvent_t* ev = nullptr);
+
+// Test hooks (radius query / gather on host-provided query points).
+// mode 0: count, 1: list, 2: gather rgb
+hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const float4* qf, const double* q64,
+                        const float4* qn, uint32_t nq, double r2, int refine64, int mode, int pfilter,
+                        uint32_t* counts, const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out, int n_cu);
+
+}  // namespace ppm

@@ -1,0 +1,359 @@
+"""Scene description: materials, shapes, lights and camera in the reference's
+terms, and the synthetic scenes of the benchmark configurations.
+
+Reference types: Material (material.rs:11-16), Surface (surface.rs:16-66),
+Shape (geometry.rs:66-115), Object (object.rs:10-13), Light (light.rs:16-39),
+Camera (camera.rs:23-50, read_camera camera.rs:201-299), M_AIR
+(scene.rs:13-18).  A Scene converts to the ppm_scene_desc of include/ppm.h.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import List, Sequence, Tuple
+
+import numpy as np
+
+from . import _abi as A
+
+Vec = Tuple[float, float, float]
+
+
+def _v(x) -> Vec:
+    return (float(x[0]), float(x[1]), float(x[2]))
+
+
+def color_normalize(c: Sequence[float]) -> Vec:
+    """Color::normalize (physics.rs:61-71): clip negatives, scale to sum 1."""
+    r, g, b = (max(0.0, float(x)) for x in c)
+    mag = r + g + b
+    if mag == 0.0:
+        return (1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0)
+    return (r / mag, g / mag, b / mag)
+
+
+@dataclass
+class Material:
+    """Material + surface parameters (material.rs:11-16, surface.rs:16-41).
+
+    color1/color2 are reflectance/specular_refl for Simple and
+    albedo_diff/albedo_spec for TS; diffuseness is scatterness for TS.
+    """
+    surface: int = A.SURF_SIMPLE
+    color1: Vec = (0.0, 0.0, 0.0)
+    color2: Vec = (0.0, 0.0, 0.0)
+    diffuseness: float = 0.0
+    metalness: float = 0.0
+    roughness: float = 0.0
+    emittance: Vec = (0.0, 0.0, 0.0)
+    transmittance: Vec = (0.0, 0.0, 0.0)
+    ior: Vec = (0.0, 0.0, 0.0)
+
+
+def simple(reflectance, specular_refl, diffuseness, metalness, roughness, *, emittance=(0, 0, 0),
+           ior=(0, 0, 0), transmittance=(0, 0, 0)) -> Material:
+    """Surface::new_simple (surface.rs:45-54) inside a Material."""
+    return Material(A.SURF_SIMPLE, _v(reflectance), _v(specular_refl), float(diffuseness), float(metalness),
+                    float(roughness), _v(emittance), _v(transmittance), _v(ior))
+
+
+def ts(albedo_diff, albedo_spec, scatterness, metalness, roughness, *, emittance=(0, 0, 0), ior=(0, 0, 0),
+       transmittance=(0, 0, 0)) -> Material:
+    """Surface::new_ts (surface.rs:56-66) inside a Material."""
+    return Material(A.SURF_TS, _v(albedo_diff), _v(albedo_spec), float(scatterness), float(metalness),
+                    float(roughness), _v(emittance), _v(transmittance), _v(ior))
+
+
+# Encodings of the benchmark materials in the reference's Simple model.
+def lambertian(rho) -> Material:
+    if not isinstance(rho, (tuple, list, np.ndarray)):
+        rho = (rho, rho, rho)
+    return simple(rho, (0, 0, 0), 1.0, 0.0, 0.0)
+
+
+def mirror() -> Material:
+    # Schlick with F0 = 1 always reflects (physics.rs:316-318)
+    return simple((0, 0, 0), (1, 1, 1), 0.0, 1.0, 0.0)
+
+
+def glass(ior=1.5) -> Material:
+    f0 = ((ior - 1.0) / (ior + 1.0)) ** 2  # 0.04 at n = 1.5
+    return simple((0, 0, 0), (f0, f0, f0), 0.0, 0.0, 0.0, ior=(ior, ior, ior))
+
+
+@dataclass
+class Shape:
+    kind: int
+    material: int
+    pos: Vec = (0.0, 0.0, 0.0)
+    nvec: Vec = (0.0, 0.0, 0.0)
+    dir1: Vec = (0.0, 0.0, 0.0)
+    dir2: Vec = (0.0, 0.0, 0.0)
+    dist: float = 0.0
+    radius: float = 0.0
+
+
+def plain(nvec, dist, material) -> Shape:
+    """Shape::Plain: points x with nvec.x + dist = 0 (geometry.rs:70-73,170-177)."""
+    return Shape(A.SHAPE_PLAIN, material, nvec=_v(nvec), dist=float(dist))
+
+
+def sphere(center, radius, material) -> Shape:
+    return Shape(A.SHAPE_SPHERE, material, pos=_v(center), radius=float(radius))
+
+
+def _poly(kind, p0, p1, p2, material, nvec=None) -> Shape:
+    p0, p1, p2 = np.asarray(p0, float), np.asarray(p1, float), np.asarray(p2, float)
+    d1, d2 = p1 - p0, p2 - p0
+    if nvec is None:
+        n = np.cross(d1, d2)
+        nn = np.linalg.norm(n)
+        if nn == 0.0:
+            raise ValueError("degenerate polygon")
+        nvec = n / nn
+    return Shape(kind, material, pos=_v(p0), nvec=_v(nvec), dir1=_v(d1), dir2=_v(d2))
+
+
+def polygon(p0, p1, p2, material) -> Shape:
+    """Shape::new_polygon (geometry.rs:93-103)."""
+    return _poly(A.SHAPE_POLYGON, p0, p1, p2, material)
+
+
+def parallelogram(p0, p1, p2, material, nvec=None) -> Shape:
+    """Shape::new_parallelogram (geometry.rs:105-115)."""
+    return _poly(A.SHAPE_PARALLELOGRAM, p0, p1, p2, material, nvec)
+
+
+@dataclass
+class Light:
+    kind: int
+    color: Vec
+    flux: float
+    pos: Vec = (0.0, 0.0, 0.0)
+    nvec: Vec = (0.0, 0.0, 0.0)
+    dir1: Vec = (0.0, 0.0, 0.0)
+    dir2: Vec = (0.0, 0.0, 0.0)
+    dir: Vec = (0.0, 0.0, 0.0)
+
+
+def parallelogram_light(color, flux, pos, nvec, dir1, dir2) -> Light:
+    return Light(A.LIGHT_PARALLELOGRAM, color_normalize(color), float(flux), _v(pos), _v(nvec), _v(dir1), _v(dir2))
+
+
+def point_light(color, flux, pos) -> Light:
+    return Light(A.LIGHT_POINT, color_normalize(color), float(flux), _v(pos))
+
+
+def sun_light(color, flux, pos, nvec, dir1, dir2, direction) -> Light:
+    return Light(A.LIGHT_SUN, color_normalize(color), float(flux), _v(pos), _v(nvec), _v(dir1), _v(dir2),
+                 _v(direction))
+
+
+@dataclass
+class Camera:
+    xreso: int
+    yreso: int
+    eye: Vec
+    target: Vec
+    upper: Vec = (0.0, 1.0, 0.0)
+    focus: float = 7.0
+    focal_len: float = 0.05    # metres (camera.rs:235 divides mm by 1000)
+    f_number: float = 4.0
+    progressive: bool = True
+    antialias: bool = True
+    blur: bool = True
+    pfilter: int = A.FILTER_NONE
+
+    def screen(self):
+        """Derived vectors of read_camera (camera.rs:244-256)."""
+        eye = np.asarray(self.eye, float)
+        ez = np.asarray(self.target, float) - eye
+        ez = ez / np.linalg.norm(ez)
+        ex = np.cross(np.asarray(self.upper, float), ez)
+        ex = ex / np.linalg.norm(ex)
+        ey = np.cross(ex, ez)
+        ey = ey / np.linalg.norm(ey)
+        step = (self.focus * 0.035 / self.focal_len) / self.xreso
+        esx, esy = step * ex, step * ey
+        ea = self.focal_len / self.f_number
+        lx, ly = float(self.xreso // 2), float(self.yreso // 2)
+        origin = self.focus * ez - (lx - 0.5) * esx - (ly - 0.5) * esy
+        return dict(eye_pos=eye, origin=origin, esx=esx, esy=esy, eex=ea * ex, eey=ea * ey)
+
+    def desc(self) -> A.CameraDesc:
+        s = self.screen()
+        d = A.CameraDesc()
+        d.xreso, d.yreso = int(self.xreso), int(self.yreso)
+        d.progressive, d.antialias, d.blur = int(self.progressive), int(self.antialias), int(self.blur)
+        d.pfilter = int(self.pfilter)
+        for k, v in s.items():
+            setattr(d, k, A.D3(*[float(x) for x in v]))
+        return d
+
+
+def pinhole(xreso, yreso, eye, target, fov_deg=60.0, upper=(0.0, 1.0, 0.0), antialias=True) -> Camera:
+    """A pinhole camera with horizontal field of view fov_deg: the reference's
+    camera with blur off and focus*0.035/focal_len = 2 tan(fov/2)."""
+    focal_len = 0.035 / (2.0 * math.tan(math.radians(fov_deg) / 2.0))
+    return Camera(xreso, yreso, _v(eye), _v(target), _v(upper), focus=1.0, focal_len=focal_len, f_number=4.0,
+                  progressive=True, antialias=antialias, blur=False)
+
+
+def read_camera() -> Camera:
+    """read_camera's built-in defaults (camera.rs:201-221); the file is ignored
+    by the reference as well."""
+    return Camera(256, 256, (1.0, 2.0, -4.5), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), focus=7.0, focal_len=0.05,
+                  f_number=4.0, progressive=True, antialias=True, blur=True)
+
+
+@dataclass
+class Scene:
+    materials: List[Material] = field(default_factory=list)
+    shapes: List[Shape] = field(default_factory=list)
+    lights: List[Light] = field(default_factory=list)
+    camera: Camera = None
+    bounds_min: Vec = (0.0, 0.0, 0.0)
+    bounds_max: Vec = (1.0, 1.0, 1.0)
+
+    def add_material(self, m: Material) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def desc(self):
+        """ppm_scene_desc; the returned tuple keeps the arrays alive."""
+        shapes = (A.ShapeDesc * max(1, len(self.shapes)))()
+        for i, s in enumerate(self.shapes):
+            d = shapes[i]
+            d.kind, d.material = int(s.kind), int(s.material)
+            d.pos, d.nvec, d.dir1, d.dir2 = A.D3(*s.pos), A.D3(*s.nvec), A.D3(*s.dir1), A.D3(*s.dir2)
+            d.dist, d.radius = float(s.dist), float(s.radius)
+        mats = (A.MaterialDesc * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            d = mats[i]
+            d.emittance, d.transmittance, d.ior = A.D3(*m.emittance), A.D3(*m.transmittance), A.D3(*m.ior)
+            d.surface = int(m.surface)
+            d.color1, d.color2 = A.D3(*m.color1), A.D3(*m.color2)
+            d.diffuseness, d.metalness, d.roughness = float(m.diffuseness), float(m.metalness), float(m.roughness)
+        lights = (A.LightDesc * max(1, len(self.lights)))()
+        for i, l in enumerate(self.lights):
+            d = lights[i]
+            d.kind = int(l.kind)
+            d.color, d.flux = A.D3(*l.color), float(l.flux)
+            d.pos, d.nvec, d.dir1, d.dir2, d.dir = (A.D3(*l.pos), A.D3(*l.nvec), A.D3(*l.dir1), A.D3(*l.dir2),
+                                                    A.D3(*l.dir))
+        sd = A.SceneDesc()
+        sd.n_shapes, sd.n_materials, sd.n_lights = len(self.shapes), len(self.materials), len(self.lights)
+        sd.shapes = C.cast(shapes, C.POINTER(A.ShapeDesc))
+        sd.materials = C.cast(mats, C.POINTER(A.MaterialDesc))
+        sd.lights = C.cast(lights, C.POINTER(A.LightDesc))
+        sd.camera = self.camera.desc()
+        sd.bounds_min, sd.bounds_max = A.D3(*self.bounds_min), A.D3(*self.bounds_max)
+        return sd, (shapes, mats, lights)
+
+
+# ---------------------------------------------------------------------------
+# Synthetic scenes of the benchmark configurations.
+
+BOX_BOUNDS = ((-0.01, -0.01, -1.0), (1.01, 1.01, 1.01))
+
+
+def cornell_box(sc: Scene, left=(0.75, 0.25, 0.25), right=(0.25, 0.75, 0.25), grey=0.75, flux=100.0,
+                light_size=0.25):
+    """The 1x1x1 m box of five Lambertian planes (floor y=0, ceiling y=1, back
+    z=1, left x=0, right x=1; open at z=0) with a light_size^2 ceiling area
+    light of `flux` watts of white light at y = 0.999.
+
+    The light is a ParallelogramLight plus an emissive Parallelogram object at
+    the same place: the reference's direct-light test needs the shadow ray to
+    hit the light's own object (tracer.rs:281-286, scene.rs:434-441).  The
+    object's material follows the reference's light material (scene.rs:226-238:
+    Simple, specular 0.8, ior 0) with emittance 2*flux/area, so that its
+    emitted radiance emittance/(2 pi) (tracer.rs:175) is the Lambertian
+    flux/(pi area).
+    """
+    g = sc.add_material(lambertian(grey))
+    l = sc.add_material(lambertian(left))
+    r = sc.add_material(lambertian(right))
+    sc.shapes += [
+        plain((0, 1, 0), 0.0, g),    # floor
+        plain((0, -1, 0), 1.0, g),   # ceiling
+        plain((0, 0, -1), 1.0, g),   # back wall
+        plain((1, 0, 0), 0.0, l),    # left wall
+        plain((-1, 0, 0), 1.0, r),   # right wall
+    ]
+    s = light_size
+    x0 = 0.5 - s / 2
+    p0 = (x0, 0.999, x0)
+    area = s * s
+    em = 2.0 * flux / area
+    lm = sc.add_material(simple((0, 0, 0), (0.8, 0.8, 0.8), 0.0, 0.0, 0.0, emittance=(em, em, em)))
+    sc.shapes.append(parallelogram(p0, (x0 + s, 0.999, x0), (x0, 0.999, x0 + s), lm, nvec=(0, -1, 0)))
+    sc.lights.append(parallelogram_light((1, 1, 1), flux, p0, (0, -1, 0), (s, 0, 0), (0, 0, s)))
+    sc.bounds_min, sc.bounds_max = BOX_BOUNDS
+    return sc
+
+
+def box_camera(xreso, yreso, antialias=True) -> Camera:
+    # 60 degree horizontal FOV: the open face (width 1 at z = 0) fills the view
+    # from z = -0.5/tan(30 deg).
+    return pinhole(xreso, yreso, (0.5, 0.5, -0.87), (0.5, 0.5, 1.0), 60.0, antialias=antialias)
+
+
+def config0_scene(xreso=256, yreso=256) -> Scene:
+    """Cornell box + a diffuse sphere (r 0.2, albedo 0.75) + a perfect-mirror
+    sphere (r 0.2); pinhole camera, 60 deg FOV."""
+    sc = cornell_box(Scene())
+    d = sc.add_material(lambertian(0.75))
+    m = sc.add_material(mirror())
+    sc.shapes += [sphere((0.3, 0.2, 0.65), 0.2, d), sphere((0.72, 0.2, 0.45), 0.2, m)]
+    sc.camera = box_camera(xreso, yreso)
+    return sc
+
+
+def config1_scene(xreso=1024, yreso=768) -> Scene:
+    """config0's box with a diffuse, a mirror and a glass (IOR 1.5) sphere
+    (r 0.2 each); the glass sphere sits under the light to cast a caustic."""
+    sc = cornell_box(Scene())
+    d = sc.add_material(lambertian(0.75))
+    m = sc.add_material(mirror())
+    gl = sc.add_material(glass(1.5))
+    sc.shapes += [sphere((0.25, 0.2, 0.75), 0.2, d), sphere((0.75, 0.2, 0.75), 0.2, m),
+                  sphere((0.5, 0.2, 0.3), 0.2, gl)]
+    sc.camera = box_camera(xreso, yreso)
+    return sc
+
+
+def reference_room() -> Scene:
+    """A room in the style of the reference's built-in scene (scene.rs:20-447):
+    4 x 4 x 11 m walls of Plain shapes, a 1.33 m ceiling parallelogram light of
+    5 W with its emissive object (scene.rs:226-238,434-441), and balls of the
+    reference's Simple/TS materials; camera = read_camera's defaults."""
+    sc = Scene()
+    mwall = sc.add_material(simple((0.5, 0.5, 0.5), (0.8, 0.8, 0.8), 1.0, 0.0, 0.0))
+    mwallb = sc.add_material(simple((0.1, 0.1, 0.4), (0.8, 0.0, 0.8), 1.0, 0.0, 0.0))
+    mwallr = sc.add_material(simple((0.4, 0.1, 0.1), (0.0, 0.0, 0.0), 1.0, 0.0, 0.0))
+    mparal = sc.add_material(simple((0, 0, 0), (0.8, 0.8, 0.8), 0.0, 0.0, 0.0, emittance=(0.15, 0.15, 0.15)))
+    silver = sc.add_material(simple((0, 0, 0), (0.78, 0.78, 0.78), 0.0, 1.0, 0.7))
+    tsball = sc.add_material(ts((0, 0, 0), (0.78, 0.78, 0.78), 0.0, 1.0, 0.5))
+    glassb = sc.add_material(ts((1, 1, 1), (0.05, 0.05, 0.05), 0.0, 0.0, 0.8, ior=(1.5, 1.5, 1.5)))
+    sc.shapes += [
+        plain((0, 1, 0), 0.0, mwall), plain((0, -1, 0), 4.0, mwall),
+        plain((-1, 0, 0), 2.0, mwallb), plain((1, 0, 0), 2.0, mwallr),
+        plain((0, 0, -1), 6.0, mwall), plain((0, 0, 1), 5.0, mwall),
+        sphere((-0.8, 0.5, 2.5), 0.4, silver), sphere((0.0, 0.5, 2.5), 0.4, tsball),
+        sphere((0.8, 0.5, 2.5), 0.4, glassb),
+        parallelogram((-0.67, 3.99, 2.33), (0.67, 3.99, 2.33), (-0.67, 3.99, 3.67), mparal, nvec=(0, -1, 0)),
+    ]
+    sc.lights.append(parallelogram_light((1, 1, 1), 5.0, (-0.67, 3.99, 2.33), (0, -1, 0), (1.33, 0, 0),
+                                         (0, 0, 1.33)))
+    sc.camera = read_camera()
+    sc.bounds_min, sc.bounds_max = (-2.01, -0.01, -5.01), (2.01, 4.01, 6.01)
+    return sc
+
+
+# Pass schedules of the configurations (BASELINE configs 0 and 1).
+CONFIGS = {
+    "config0": dict(scene=config0_scene, n_photon=200_000, passes=16, radius0=0.01, alpha=2.0 / 3.0, max_depth=8,
+                    seed=1),
+    "config1": dict(scene=config1_scene, n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0,
+                    max_depth=16, seed=1),
+}
