@@ -232,12 +232,12 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   if ((e = dalloc(&eb.tile_count, tiles)) != hipSuccess) return e;
   if ((e = dalloc(&eb.tile_off, tiles)) != hipSuccess) return e;
   for (int i = 0; i < 2; ++i) {
-    if ((e = dalloc(&eb.qkeys[i], inplace)) != hipSuccess) return e;
-    if ((e = dalloc(&eb.qvals[i], inplace)) != hipSuccess) return e;
+    if ((e = dalloc(&eb.qkeys[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
+    if ((e = dalloc(&eb.qvals[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
   }
   if ((e = dalloc(&eb.qghist, 256ull * eb.q_sort_blocks)) != hipSuccess) return e;
   if ((e = dalloc(&eb.qtotals, 256)) != hipSuccess) return e;
-  if ((e = dalloc(&eb.res, inplace)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.res, inplace + eb.ovf_cap)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -835,8 +835,8 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
     hs.overflow = 0;
     if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth)) != hipSuccess) break;
-    if ((e = launch_gather(c->stream, m->m, st, eb, sc->buf.cam, f4(sc->bmin), f4(sc->bmax), pfilter, c->n_cu,
-                           nullptr, nullptr)) != hipSuccess)
+    if ((e = launch_gather(c->stream, sc->buf, m->m, st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=
+        hipSuccess)
       break;
     h.resize(eb.n_pix);
     if ((e = hipMemcpyAsync(h.data(), eb.pix, sizeof(float4) * eb.n_pix, hipMemcpyDeviceToHost, c->stream)) !=
@@ -949,7 +949,7 @@ static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
   if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
-  if ((e = launch_gather(st, s->map, s->st, s->eb, sc->buf.cam, bmin, bmax, cfg.pfilter, s->ctx->n_cu, s->sum,
+  if ((e = launch_gather(st, s->sbuf, s->map, s->st, s->eb, cfg.pfilter, cfg.use_classic, s->ctx->n_cu, s->sum,
                          s->sumsq, ev ? ev + 4 : nullptr)) != hipSuccess)
     return e;
   if ((e = launch_pass_end(st, s->st)) != hipSuccess) return e;

@@ -278,7 +278,7 @@ __global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* 
 // Exclusive scan of up to 65536 counts in one block; the total goes to
 // *total (the pass's stored-photon or sorted-query count).
 __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uint32_t* offset, uint32_t n,
-                                                      uint32_t* total) {
+                                                      uint32_t* total, const uint32_t* extra, uint32_t extra_cap) {
   __shared__ uint32_t wsum[16];
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t per = (n + 1023) / 1024;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uin
       wsum[i] = acc;
       acc += v;
     }
-    *total = acc;
+    *total = acc + (extra ? min(*extra, extra_cap) : 0u);
   }
   __syncthreads();
   uint32_t run = wsum[w] + incl - sum;
@@ -333,7 +333,8 @@ hipError_t launch_trace_photons(hipStream_t s, const SceneBuf& sc, PassState* st
                      pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, pb.count, pb.offset, n_waves, &st->n_stored);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, pb.count, pb.offset, n_waves, &st->n_stored,
+                     (const uint32_t*)nullptr, 0u);
   return hipGetLastError();
 }
 
@@ -792,7 +793,8 @@ __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf
         const BsdfW W = bsdf_weights(M, cos0);
         const C3 wd = cur.w * W.wd;
         if (nonzero(wd)) {
-          if (uc) acc = acc + wd * direct_light(sc, is.pos, is.n);  // tracer.rs:136-144
+          // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
+          // both are evaluated per query by k_shade_sorted
           uint32_t slot = t;
           if (depth == 0) primary_q = true;
           if (depth > 0) {
@@ -938,41 +940,27 @@ __device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const Photo
   return a;
 }
 
-// Overflow queries (deeper eye-tree nodes beyond a pixel's own slots):
-// added into their pixel with float atomics before the accumulation.
-__global__ __launch_bounds__(256) void k_gather_overflow(const Node* nodes, const PhotonRec* ph, PassState* st,
-                                                         EyeBuf eb, int pf) {
-  __shared__ LeafLds lds[4];
-  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
-  const uint32_t n = st->n_stored, depth = st->depth;
-  const float r2 = st->r2f, scale = st->scale;
-  const uint32_t n_ovf = min(st->n_queries, eb.ovf_cap);
-  const uint32_t groups = (n_ovf + 63) / 64;
-  const uint32_t waves = gridDim.x * 4;
-  const uint64_t base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
-  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
-    const uint32_t i = g * 64 + lane;
-    Query qq{};
-    const bool active = i < n_ovf;
-    if (active) qq = eb.q[base + i];
-    const GatherAcc a = gather_query(nodes, ph, n, depth, active, qq, r2, pf, lds[wib]);
-    if (active && (a.r != 0.0f || a.g != 0.0f || a.b != 0.0f)) {
-      const uint32_t p = __float_as_uint(qq.a.w);
-      atomicAdd(&eb.pix[p].x, a.r * scale * qq.w.x);
-      atomicAdd(&eb.pix[p].y, a.g * scale * qq.w.y);
-      atomicAdd(&eb.pix[p].z, a.b * scale * qq.w.z);
-    }
-  }
-}
-
-// Sort keys of the in-place queries: one wave per tile; lane i writes its
-// pixel's queries (primary, then deeper ones) at the tile's offset plus the
-// prefix of the earlier lanes' counts.  Key = Morton code of the position.
+// Sort keys of the queries: one wave per tile; lane i writes its pixel's
+// in-place queries (primary, then deeper ones) at the tile's offset plus the
+// prefix of the earlier lanes' counts.  Threads past the tiles key the
+// overflow queries, which follow all in-place ones.  Key = Morton code of
+// the position.
 __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
                                                     float fq, uint32_t* keys, uint32_t* vals) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t g = t >> 6, lane = lane_id();
-  if (g >= eb.n_tiles) return;
+  const uint64_t ovf_base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
+  if (g >= eb.n_tiles) {
+    const uint32_t n_ovf = min(st->n_queries, eb.ovf_cap);
+    const uint32_t i = t - eb.n_tiles * 64u;
+    if (i < n_ovf) {
+      const uint32_t pos = st->n_qsort - n_ovf + i;
+      const float4 a = eb.q[ovf_base + i].a;
+      keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+      vals[pos] = (uint32_t)(ovf_base + i);
+    }
+    return;
+  }
   const bool primary = __float_as_uint(eb.q[t].a.w) != NO_PIXEL;
   const uint32_t nsec = eb.sec_count[t];
   const uint32_t c = (primary ? 1u : 0u) + nsec;
@@ -998,11 +986,15 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
   }
 }
 
-// Gather in Morton order: wave w takes sorted queries [64 w, 64 w + 64),
-// which lie close together in space, so the union of their balls touches
-// few leaves.  res[slot] = w * scale * Σ (per channel).
-__global__ __launch_bounds__(256) void k_gather_sorted(const Node* nodes, const PhotonRec* ph, const PassState* st,
-                                                       EyeBuf eb, const uint32_t* order, int pf) {
+// Shading of the queries in Morton order: wave w takes sorted queries
+// [64 w, 64 w + 64), which lie close together in space, so the union of
+// their balls touches few leaves and their shadow rays are coherent.
+//   res[slot] = w * (direct light (when uc) + scale * Σ filter * max(0,-n.dir))
+// per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
+// BSDF factor and path throughput.
+__global__ __launch_bounds__(256) void k_shade_sorted(SceneView sc, const Node* nodes, const PhotonRec* ph,
+                                                      const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
+                                                      int uc) {
   __shared__ LeafLds lds[4];
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
   const uint32_t n = st->n_stored, depth = st->depth;
@@ -1020,12 +1012,27 @@ __global__ __launch_bounds__(256) void k_gather_sorted(const Node* nodes, const 
       qq = eb.q[slot];
     }
     const GatherAcc a = gather_query(nodes, ph, n, depth, active, qq, r2, pf, lds[wib]);
-    if (active) eb.res[slot] = make_float4(a.r * scale * qq.w.x, a.g * scale * qq.w.y, a.b * scale * qq.w.z, 0.0f);
+    if (active) {
+      C3 di = C3{a.r * scale, a.g * scale, a.b * scale};
+      if (uc) di = di + direct_light(sc, mk(qq.a), mk(qq.b));
+      eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
+    }
   }
 }
 
-// Per-pixel pass radiance = pix (emission + direct light + overflow
-// gathers) + res of the pixel's primary and deeper queries, in a fixed
+// Overflow results into their pixels (float atomics; few per pass).
+__global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState* st) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= min(st->n_queries, eb.ovf_cap)) return;
+  const uint64_t slot = (uint64_t)eb.q_primary * (1u + eb.sec_max) + i;
+  const uint32_t p = __float_as_uint(eb.q[slot].a.w);
+  const float4 r = eb.res[slot];
+  atomicAdd(&eb.pix[p].x, r.x);
+  atomicAdd(&eb.pix[p].y, r.y);
+  atomicAdd(&eb.pix[p].z, r.z);
+}
+
+// Per-pixel pass radiance = pix (emission + overflow results) + res of the pixel's primary and deeper queries, in a fixed
 // order; accumulated into the FP64 sums (util/averager2.rb:58-60) when sum
 // is given, else stored back into pix.
 __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, double* sum, double* sumsq) {
@@ -1065,16 +1072,14 @@ __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, do
   }
 }
 
-hipError_t launch_gather(hipStream_t s, const MapBuf& m, PassState* st, const EyeBuf& eb, const DevCamera& cam,
-                         float4 bmin, float4 bmax, int pfilter, int n_cu, double* sum, double* sumsq,
-                         hipEvent_t* ev) {
+hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
+                         int pfilter, int use_classic, int n_cu, double* sum, double* sumsq, hipEvent_t* ev) {
   const uint32_t blocks = (uint32_t)n_cu * 8u;
-  hipLaunchKernelGGL(k_gather_overflow, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, st, eb, pfilter);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, eb.tile_count, eb.tile_off, eb.n_tiles,
-                     &st->n_qsort);
+                     &st->n_qsort, (const uint32_t*)&st->n_queries, eb.ovf_cap);
   const uint32_t qmax = (1u << m.morton_bits) - 1u;
-  hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, st, bmin,
-                     morton_scale(bmin, bmax, qmax), (float)qmax, eb.qkeys[0], eb.qvals[0]);
+  hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, st, sc.bmin,
+                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, eb.qkeys[0], eb.qvals[0]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint32_t* qk[2] = {eb.qkeys[0], eb.qkeys[1]};
@@ -1083,9 +1088,11 @@ hipError_t launch_gather(hipStream_t s, const MapBuf& m, PassState* st, const Ey
   e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 31u, &cur);
   if (e != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_gather_sorted, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, st, eb, qv[cur], pfilter);
+  hipLaunchKernelGGL(k_shade_sorted, dim3(blocks), dim3(256), 0, s, make_view(sc), m.nodes, m.sorted, st, eb, qv[cur],
+                     pfilter, use_classic);
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, cam, sum, sumsq);
+  hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
+  hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);
   return hipGetLastError();
 }
 

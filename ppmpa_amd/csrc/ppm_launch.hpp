@@ -94,11 +94,12 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth);
 
-// Gathers of one pass.  sum/sumsq != null: the pixel's pass radiance is
-// added into the FP64 sums; otherwise it is left in pix.
-// ev (optional, 2 events) brackets the k_gather_sorted launch alone.
-hipError_t launch_gather(hipStream_t s, const MapBuf& m, PassState* st, const EyeBuf& eb, const DevCamera& cam,
-                         float4 bmin, float4 bmax, int pfilter, int n_cu, double* sum, double* sumsq,
+// Gather + direct light of one pass over the Morton-sorted queries, then the
+// per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
+// the FP64 sums; otherwise it is left in pix.  ev (optional, 2 events)
+// brackets the k_shade_sorted launch alone.
+hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
+                         int pfilter, int use_classic, int n_cu, double* sum, double* sumsq,
                          hipEvent_t* ev = nullptr);
 
 // Test hooks (radius query / gather on host-provided query points).
