@@ -148,3 +148,51 @@ def test_nonfinite_photon_rejected():
     with pytest.raises(P.PpmError) as e:
         ph.upload(np.array([[0, np.nan, 0]]), np.array([[0, 1, 0]]), np.array([0], np.uint8))
     assert e.value.status == -2
+
+
+def test_golden_map_bitexact_sets_and_radiance():
+    """Photon set dumped in the reference's text format (tests/golden):
+    read_map -> device map; FP64-decided index sets equal the committed
+    FP64 sets bit for bit; radiance within 1e-5 relative for every filter."""
+    import os
+    from ppmpa_amd import photonmap_io as M
+    say("golden map")
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    z = np.load(os.path.join(gold, "queries_config0.npz"), allow_pickle=False)
+    c = ctx()
+    m, ph = P.read_map(c, os.path.join(gold, "map_config0.txt"))
+    pm = M.parse_map(os.path.join(gold, "map_config0.txt"))
+    for j, r2 in enumerate(z["r2"]):
+        got = m.within(z["q"], r2, refine64=True)
+        off, idx = z[f"off_{j}"], z[f"idx_{j}"]
+        for i in range(len(z["q"])):
+            assert np.array_equal(got[i], idx[int(off[i]):int(off[i + 1])]), (j, i)
+        got32 = m.within(z["q"], r2, refine64=False)
+        want32 = O.within_brute(pm.pos, z["q"], r2, mode=1)
+        for i in range(len(z["q"])):
+            assert np.array_equal(got32[i], want32[i])
+        for pf in (0, 1, 2):
+            rad = m.estimate_radiance(z["q"], z["nrm"], r2, pfilter=pf, refine64=True)
+            want = z[f"rad_{j}_{pf}"]
+            nz = want != 0
+            assert np.all(rad[~nz] == 0)
+            assert np.max(np.abs(rad[nz] - want[nz]) / np.abs(want[nz])) < 1e-5
+
+
+def test_float_gather_production_arithmetic():
+    """The production gather decides membership in float (no refinement):
+    it matches the FP64 oracle within 1e-5 except for queries with a photon
+    within float rounding of the sphere (a boundary flip changes the sum by
+    one photon)."""
+    say("float gather")
+    pos, dir, wl = random_photons(30000, 31, clustered=True)
+    m, _ = _upload_and_build(pos, dir, wl, 1e-4)
+    om = O.OMap(pos, dir, wl, 1e-4)
+    rng = np.random.default_rng(32)
+    q = pos[rng.integers(0, len(wl), 3000)] + rng.normal(0, 0.003, (3000, 3))
+    nrm = rng.normal(size=(3000, 3))
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    got = m.estimate_radiance(q, nrm, 0.01 ** 2, refine64=False)
+    want = om.gather(q, nrm, 0.01 ** 2, 0)
+    rel = np.abs(got - want).max(1) / np.maximum(np.abs(want).max(1), 1e-300)
+    assert (rel < 1e-5).mean() > 0.99
