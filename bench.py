@@ -28,7 +28,10 @@ def main():
     ap.add_argument("--steps", type=int, default=512)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-passes", type=int, default=1)
+    ap.add_argument("--cpu-sample-passes", type=int, default=8)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the timed region, copy this many MB device-to-device (a kernel of known traffic "
+                         "to scale profiler byte counters against)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the PPM-PA passes would shard "
@@ -90,8 +93,12 @@ def main():
     achieved = gather_bytes / (ms["ms_gather"] / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_gather_sorted", "kernel_ms": round(ms["ms_gather"], 4),
+            "kernel": "k_shade_sorted", "kernel_ms": round(ms["ms_gather"], 4),
             "algorithmic_bytes": round(gather_bytes), "queries_per_pass": round(Q)}
+    traffic = profiled_traffic("k_shade_sorted")
+    if traffic is not None:
+        roof["traffic"] = traffic
+        roof["traffic_source"] = "profiles/pmc_traffic_config1.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
 
     out = {
         "metric": "Mphotons/s traced+gathered",
@@ -117,16 +124,43 @@ def main():
         "roofline": roof,
     }
 
+    if args.copy_probe_mb > 0:
+        import torch
+        x = torch.zeros(args.copy_probe_mb << 20, dtype=torch.uint8, device="cuda")
+        y = torch.empty_like(x)
+        torch.add(x, 1, out=y)  # an elementwise kernel: reads and writes copy_probe_mb MB
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = args.copy_probe_mb << 20
+
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.cpu_sample_passes)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    (two rocprofv3 --pmc runs of this same workload), scaled by the factors
+    recorded there; None when absent."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_config1.md")
+    if not os.path.exists(path):
+        return None
+    sf, sw = 1.0, 1.0
+    for line in open(path):
+        if line.startswith("scale_fetch:"):
+            sf = float(line.split(":")[1])
+        if line.startswith("scale_write:"):
+            sw = float(line.split(":")[1])
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 3 and cells[0] == kernel:
+            return round((float(cells[1]) * sf + float(cells[2]) * sw) * 1024)
+    return None
+
+
 def cpu_baseline(sc, cfgd, n_sample):
     """The FP64 oracle (a restatement of the reference's CPU path) on this
-    host's cores, on a bounded sample: pass 1 (the widest radius) of the same
-    workload."""
+    host's cores, on a bounded sample: passes 1..n_sample (the widest radii,
+    the most gather work) of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     import ppmpa_amd as P
@@ -138,7 +172,7 @@ def cpu_baseline(sc, cfgd, n_sample):
     _, _, stored, ms_trace, ms_eye = osc.run_passes(c, n_sample, threads=threads)
     T = time.perf_counter() - t0
     return {"value": round(cfgd["n_photon"] * n_sample / T / 1e6, 4), "unit": "Mphotons/s", "cores": threads,
-            "kind": "port", "sample": f"{n_sample} full pass(es) of the workload (pass 1), FP64 oracle, "
+            "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle, "
                                       f"{T:.1f} s; trace+kd {ms_trace / 1e3:.1f} s, eye+gather {ms_eye / 1e3:.1f} s"}
 
 
