@@ -1,7 +1,7 @@
 """Benchmark of the PPM-PA hot path on one MI355X.
 
-Workload (config[1] of the benchmark set): Cornell box with a diffuse, a mirror
-and a glass sphere, 1024x768 pixels, 2M photons per pass, max 16 bounces,
+Workload "caustic": a Cornell box of five 1 x 1 m quad walls with a diffuse, a
+mirror and a glass sphere, 1024x768 pixels, 2M photons per pass, max 16 bounces,
 r_1 = 0.005 m, alpha = 2/3, seed 1.  One step = one PPM-PA pass (photon trace
 + map build + eye rays + gather + FP64 accumulation) on device-resident data.
 
@@ -30,8 +30,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-passes", type=int, default=8)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="after the timed region, copy this many MB device-to-device (a kernel of known traffic "
-                         "to scale profiler byte counters against)")
+                    help="after the timed region, fill one buffer of this many MiB and copy it device-to-device "
+                         "(kernels of known traffic to scale profiler byte counters against; with HSA_ENABLE_SDMA=0 "
+                         "the copy runs as a blit kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the PPM-PA passes would shard "
@@ -44,7 +45,7 @@ def main():
     import ppmpa_amd as P
     from ppmpa_amd import _abi as A
 
-    cfgd = P.CONFIGS["config1"]
+    cfgd = P.CONFIGS["caustic"]
     sc = cfgd["scene"]()
     ctx = P.Context(0)
     ds = P.DeviceScene(ctx, sc)
@@ -83,8 +84,8 @@ def main():
     stage.close()
     ms = {k: st[k] / 4 for k in ("ms_photon_trace", "ms_map_build", "ms_eye_trace", "ms_gather", "ms_accumulate")}
 
-    # roofline of the gather kernel (k_gather_sorted, timed alone between
-    # HIP events on the context stream): algorithmic bytes per pass = the
+    # roofline of the shade kernel (k_shade: gather + direct light, timed
+    # alone between HIP events on the context stream): algorithmic bytes per pass = the
     # photon map read once (28 B/photon) + per query its record (48 B), its
     # sorted index (4 B) and its result (16 B).
     S = stored_per_pass
@@ -93,12 +94,12 @@ def main():
     achieved = gather_bytes / (ms["ms_gather"] / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_shade_sorted", "kernel_ms": round(ms["ms_gather"], 4),
+            "kernel": "k_shade", "kernel_ms": round(ms["ms_gather"], 4),
             "algorithmic_bytes": round(gather_bytes), "queries_per_pass": round(Q)}
-    traffic = profiled_traffic("k_shade_sorted")
+    traffic = profiled_traffic("k_shade")
     if traffic is not None:
         roof["traffic"] = traffic
-        roof["traffic_source"] = "profiles/pmc_traffic_config1.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
+        roof["traffic_source"] = "profiles/pmc_traffic_caustic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
 
     out = {
         "metric": "Mphotons/s traced+gathered",
@@ -113,7 +114,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "config1: Cornell box + diffuse/mirror/glass spheres, 1024x768, 2M photons/pass, "
+        "config": {"workload": "caustic: Cornell box (five 1x1 m quad walls) + diffuse/mirror/glass spheres, 1024x768, 2M photons/pass, "
                                "max 16 bounces, r1=0.005 m, alpha=2/3, seed 1",
                    "photons_per_pass": n_photon, "pixels": pix, "passes": args.steps,
                    "accumulation": "f64"},
@@ -125,12 +126,7 @@ def main():
     }
 
     if args.copy_probe_mb > 0:
-        import torch
-        x = torch.zeros(args.copy_probe_mb << 20, dtype=torch.uint8, device="cuda")
-        y = torch.empty_like(x)
-        torch.add(x, 1, out=y)  # an elementwise kernel: reads and writes copy_probe_mb MB
-        torch.cuda.synchronize()
-        out["copy_probe_bytes"] = args.copy_probe_mb << 20
+        out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.cpu_sample_passes)
@@ -138,11 +134,34 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def copy_probe(nbytes):
+    """hipMemset of nbytes (a fill kernel writing nbytes) then a
+    device-to-device hipMemcpy of them (reads and writes nbytes), through the
+    HIP runtime directly."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(C.byref(buf), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed in the copy probe")
+    try:
+        if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMemset failed in the copy probe")
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in the copy probe")
+        if hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed in the copy probe")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return nbytes
+
+
 def profiled_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the committed counter summary
     (two rocprofv3 --pmc runs of this same workload), scaled by the factors
     recorded there; None when absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_config1.md")
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_caustic.md")
     if not os.path.exists(path):
         return None
     sf, sw = 1.0, 1.0
@@ -152,8 +171,8 @@ def profiled_traffic(kernel):
         if line.startswith("scale_write:"):
             sw = float(line.split(":")[1])
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 3 and cells[0] == kernel:
-            return round((float(cells[1]) * sf + float(cells[2]) * sw) * 1024)
+        if len(cells) >= 5 and cells[0] == kernel:  # kernel | dispatches | FETCH KiB | WRITE KiB | MB
+            return round((float(cells[2]) * sf + float(cells[3]) * sw) * 1024)
     return None
 
 

@@ -7,7 +7,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libppm.so")
+LIB_PATH = os.environ.get("PPMPA_AMD_LIB", os.path.join(HERE, "libppm.so"))
 
 # status codes
 PPM_OK = 0

@@ -36,26 +36,28 @@ def _run(cmd):
     return proc.stdout
 
 
-def build(verbose=False):
-    """Compile every source to an object and link libppm.so next to this file."""
+def build(verbose=False, out=LIB, defines=()):
+    """Compile every source to an object and link libppm.so next to this file
+    (or at `out`, with extra -D `defines`)."""
     if not os.path.exists(HIPCC):
         raise RuntimeError("hipcc not found at " + HIPCC)
-    objdir = os.path.join(HERE, "_build")
+    objdir = os.path.join(HERE, "_build", os.path.basename(out))
     os.makedirs(objdir, exist_ok=True)
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
-        cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
+        flags = COMMON + ["-D" + d for d in defines]
+        cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
-            cmd = [HIPCC, "-x", "hip"] + COMMON + ["-c", os.path.join(CSRC, src), "-o", obj]
-        out = _run(cmd)
-        if verbose and out.strip():
-            sys.stderr.write(out)
+            cmd = [HIPCC, "-x", "hip"] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+        log = _run(cmd)
+        if verbose and log.strip():
+            sys.stderr.write(log)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
-    shutil.move(tmp, LIB)
-    return LIB
+    shutil.move(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

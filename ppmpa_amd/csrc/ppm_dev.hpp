@@ -434,55 +434,35 @@ __device__ __forceinline__ C3 light_radiance(const DevLight& L, float sqd) {
   return c3(L.color) * L.flux;  // SunLight
 }
 
+// Σ over the lights of the reference's direct term (tracer.rs:136-144).
+// Only a ParallelogramLight contributes: a PointLight or SunLight yields at
+// most one sample (light.rs:116-117,132-147), which the RADIANCE0 shift
+// pairs with zero radiance, so their term is exactly 0 and no shadow ray is
+// cast for them.
+// Sample s of the TSS grid (light.rs:164-170), row-major in the first
+// coordinate: (0.1 + 0.2 (s / 5), 0.1 + 0.2 (s % 5)) as the float literals
+// 0.1, 0.3, 0.5, 0.7, 0.9.
+__device__ __forceinline__ float tss(int i) {
+  return i == 0 ? 0.1f : (i == 1 ? 0.3f : (i == 2 ? 0.5f : (i == 3 ? 0.7f : 0.9f)));
+}
+__device__ __forceinline__ V3 tss_point(const DevLight& L, int s) {
+  return mk(L.pos) + tss(s / 5) * mk(L.d1) + tss(s % 5) * mk(L.d2);
+}
+
 __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
   C3 sum{0.0f, 0.0f, 0.0f};
   for (uint32_t li = 0; li < sc.n_lights; ++li) {
     const DevLight L = ldc(sc.lights, li);
+    if (L.kind != LIGHT_PARALLELOGRAM) continue;
+    const V3 ln = mk(L.nvec);
     C3 prev{0.0f, 0.0f, 0.0f};
-    if (L.kind == LIGHT_PARALLELOGRAM) {
-      V3 lp0 = mk(L.pos), d1 = mk(L.d1), d2 = mk(L.d2), ln = mk(L.nvec);
-      // TSS grid (light.rs:164-170), row-major in the first coordinate.
-      const float ts[5] = {0.1f, 0.3f, 0.5f, 0.7f, 0.9f};
-      for (int a = 0; a < 5; ++a) {
-        for (int b = 0; b < 5; ++b) {
-          float tx = ts[a], ty = ts[b];
-          V3 ld = (lp0 + tx * d1 + ty * d2) - p;
-          if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
-          float csq, sqd;
-          if (!sample_lit(sc, p, n, ld, csq, sqd)) continue;
-          sum = sum + prev * csq;
-          prev = light_radiance(L, sqd);
-        }
-      }
-    } else if (L.kind == LIGHT_POINT) {
-      V3 ld = mk(L.pos) - p;  // light.rs:116-117
+    for (int s = 0; s < 25; ++s) {
+      const V3 ld = tss_point(L, s) - p;
+      if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (sample_lit(sc, p, n, ld, csq, sqd)) {
-        sum = sum + prev * csq;
-        prev = light_radiance(L, sqd);
-      }
-    } else {  // SunLight (light.rs:132-147)
-      V3 d = mk(L.pos) - p;
-      if (!(dot(mk(L.nvec), d) > 0.0f)) {
-        V3 dt2 = -mk(L.dir);
-        V3 d1 = mk(L.d1), d2 = mk(L.d2);
-        V3 re2 = cross(dt2, d2);
-        float det = dot(re2, d1);
-        if (det != 0.0f) {
-          V3 pp = p - mk(L.pos);
-          V3 te1 = cross(pp, d1);
-          float inv = 1.0f / det;
-          float u = dot(re2, pp) * inv, v = dot(te1, dt2) * inv, t = dot(te1, d2) * inv;
-          if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > 2.0f)) {
-            V3 ld = t * dt2;
-            float csq, sqd;
-            if (sample_lit(sc, p, n, ld, csq, sqd)) {
-              sum = sum + prev * csq;
-              prev = light_radiance(L, sqd);
-            }
-          }
-        }
-      }
+      if (!sample_lit(sc, p, n, ld, csq, sqd)) continue;
+      sum = sum + prev * csq;
+      prev = light_radiance(L, sqd);
     }
   }
   return sum;

@@ -382,24 +382,28 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
   return v;
 }
 
-// Morton key (30 bits) of a position in the quantisation box; positions
-// outside get bit 30 and a code in a box 64x as large.
+// Sort key of a position: a 2-bit distance level above a 30-bit Morton code.
+// Level 0 = inside the quantisation box (1024 cells per axis); levels 1 and 2
+// = inside boxes 16x and 256x as large around the same centre; level 3 =
+// beyond (clamped).  Far photons (walks that leaked through a corner of the
+// geometry travel arbitrarily far) thus sort into leaves of their own instead
+// of sharing a leaf, and a huge bounding box, with photons near the scene.
 __device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, float4 bmin, float4 bscale, float fq) {
   float fx = (px - bmin.x) * bscale.x;
   float fy = (py - bmin.y) * bscale.y;
   float fz = (pz - bmin.z) * bscale.z;
-  uint32_t outside = 0;
-  if (!(fx >= 0.0f && fx < fq + 1.0f && fy >= 0.0f && fy < fq + 1.0f && fz >= 0.0f && fz < fq + 1.0f)) {
-    const float h = 0.5f * (fq + 1.0f);
-    fx = (fx - h) * (1.0f / 64.0f) + h;
-    fy = (fy - h) * (1.0f / 64.0f) + h;
-    fz = (fz - h) * (1.0f / 64.0f) + h;
-    outside = 1u << 30;
+  const float n = fq + 1.0f, h = 0.5f * n;
+  uint32_t level = 0;
+  float s = 1.0f;
+  while (level < 3 && !((fx - h) * s + h >= 0.0f && (fx - h) * s + h < n && (fy - h) * s + h >= 0.0f &&
+                        (fy - h) * s + h < n && (fz - h) * s + h >= 0.0f && (fz - h) * s + h < n)) {
+    ++level;
+    s *= (1.0f / 16.0f);
   }
-  fx = fminf(fmaxf(fx, 0.0f), fq);
-  fy = fminf(fmaxf(fy, 0.0f), fq);
-  fz = fminf(fmaxf(fz, 0.0f), fq);
-  return outside | (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) |
+  fx = fminf(fmaxf((fx - h) * s + h, 0.0f), fq);
+  fy = fminf(fmaxf((fy - h) * s + h, 0.0f), fq);
+  fz = fminf(fmaxf((fz - h) * s + h, 0.0f), fq);
+  return (level << 30) | (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) |
          expand_bits10((uint32_t)fz);
 }
 
@@ -413,10 +417,7 @@ static float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax) {
 }
 
 
-// Key = 30-bit Morton code of the position quantised in the box; photons
-// outside the box (e.g. walks that leaked through a corner of infinite
-// planes) get bit 30 and a Morton code in a box 64x as large, so they sort
-// into leaves of their own instead of widening the leaves at the faces.
+// Keys of the stored photons (morton_key) and their dense -> region index.
 __global__ __launch_bounds__(256) void k_morton(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
                                                 uint32_t region_cap, const PassState* st, uint32_t* keys,
                                                 uint32_t* vals, float4 bmin, float4 bscale, uint32_t qmax,
@@ -626,7 +627,7 @@ __global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const
       const uint32_t j = g * SUB_LEAVES + jl;
       float lx = INF_F, ly = INF_F, lz = INF_F, hx = -INF_F, hy = -INF_F, hz = -INF_F;
       const uint32_t e = j * LEAF + lane;
-      if (j < L && e < n) {
+      if (lane < (uint32_t)LEAF && j < L && e < n) {
         const uint32_t src = order[e];
         const PhotonRec r = in[src];
         out[e] = r;
@@ -712,7 +713,7 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   int cur = 0;
-  e = radix_sort(s, m.keys, m.vals, &st->n_stored, m.ghist, m.totals, m.sort_blocks, 31u, &cur);
+  e = radix_sort(s, m.keys, m.vals, &st->n_stored, m.ghist, m.totals, m.sort_blocks, 32u, &cur);
   if (e != hipSuccess) return e;
   uint32_t groups = (m.lpad_max + SUB_LEAVES - 1) / SUB_LEAVES;
   if (groups > 16384) groups = 16384;
@@ -794,7 +795,7 @@ __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf
         const C3 wd = cur.w * W.wd;
         if (nonzero(wd)) {
           // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
-          // both are evaluated per query by k_shade_sorted
+          // both are evaluated per query by k_shade
           uint32_t slot = t;
           if (depth == 0) primary_q = true;
           if (depth > 0) {
@@ -986,22 +987,25 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
   }
 }
 
-// Shading of the queries in Morton order: wave w takes sorted queries
-// [64 w, 64 w + 64), which lie close together in space, so the union of
-// their balls touches few leaves and their shadow rays are coherent.
+// Shading of the queries in Morton order.  A wave takes 64 consecutive
+// sorted queries (close together in space), one per lane.  The wave
+// traverses the tree for the union of its queries' balls; each overlapped
+// leaf is staged once into LDS with coalesced loads and every lane whose
+// ball overlaps the leaf box tests its photons from LDS (broadcast reads).
 //   res[slot] = w * (direct light (when uc) + scale * Σ filter * max(0,-n.dir))
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
 // BSDF factor and path throughput.
-__global__ __launch_bounds__(256) void k_shade_sorted(SceneView sc, const Node* nodes, const PhotonRec* ph,
-                                                      const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
-                                                      int uc) {
+__global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, const PhotonRec* ph,
+                                               const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
+                                               int uc) {
   __shared__ LeafLds lds[4];
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
   const uint32_t n = st->n_stored, depth = st->depth;
-  const float r2 = st->r2f, scale = st->scale;
+  const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
   const uint32_t groups = (nq + 63) / 64;
   const uint32_t waves = gridDim.x * 4;
+  const uint32_t first_leaf = (1u << depth) - 1u;
   for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
     const uint32_t i = g * 64 + lane;
     const bool active = i < nq;
@@ -1011,12 +1015,55 @@ __global__ __launch_bounds__(256) void k_shade_sorted(SceneView sc, const Node* 
       slot = order[i];
       qq = eb.q[slot];
     }
-    const GatherAcc a = gather_query(nodes, ph, n, depth, active, qq, r2, pf, lds[wib]);
-    if (active) {
-      C3 di = C3{a.r * scale, a.g * scale, a.b * scale};
-      if (uc) di = di + direct_light(sc, mk(qq.a), mk(qq.b));
-      eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
+    const float qx = qq.a.x, qy = qq.a.y, qz = qq.a.z;
+    const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    uint32_t node = 0;
+    while (n > 0) {
+      node = __builtin_amdgcn_readfirstlane(node);
+      const Node nd = ldc(nodes, node);
+      const bool ov = active && box_dist2_nc(qx, qy, qz, nd.lo, nd.hi) <= r2;
+      if (__ballot(ov)) {
+        if (node < first_leaf) {
+          node = 2 * node + 1;
+          continue;
+        }
+        const uint32_t base = (node - first_leaf) * LEAF;
+        const uint32_t cnt = base < n ? min((uint32_t)LEAF, n - base) : 0u;
+        if (lane < cnt) {
+          const PhotonRec r = ph[base + lane];
+          lds[wib].pm[lane] = r.pm;
+          lds[wib].dm[lane] = r.dm;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ov) {
+          for (uint32_t j = 0; j < cnt; ++j) {
+            const float4 pm = lds[wib].pm[j];
+            const float d2 = dist2_nc(qx, qy, qz, pm.x, pm.y, pm.z);
+            if (d2 <= r2) {
+              const float4 dm = lds[wib].dm[j];
+              const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
+              const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) : 0.0f;
+              const uint32_t wl = __float_as_uint(pm.w) & 3u;
+              ar += wl == 0 ? v : 0.0f;
+              ag += wl == 1 ? v : 0.0f;
+              ab += wl == 2 ? v : 0.0f;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      while (node != 0 && (node & 1u) == 0) node = (node - 1) >> 1;
+      if (node == 0) break;
+      node += 1;
     }
+    C3 di = C3{ar * scale, ag * scale, ab * scale};
+    if (uc) di = di + direct_light(sc, mk(qq.a), mk(qq.b));
+    if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
 }
 
@@ -1085,11 +1132,12 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   uint32_t* qk[2] = {eb.qkeys[0], eb.qkeys[1]};
   uint32_t* qv[2] = {eb.qvals[0], eb.qvals[1]};
   int cur = 0;
-  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 31u, &cur);
+  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 32u, &cur);
   if (e != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_shade_sorted, dim3(blocks), dim3(256), 0, s, make_view(sc), m.nodes, m.sorted, st, eb, qv[cur],
-                     pfilter, use_classic);
+  const SceneView v = make_view(sc);
+  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
+                     use_classic);
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);

@@ -97,7 +97,7 @@ hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const Ey
 // Gather + direct light of one pass over the Morton-sorted queries, then the
 // per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
 // the FP64 sums; otherwise it is left in pix.  ev (optional, 2 events)
-// brackets the k_shade_sorted launch alone.
+// brackets the k_shade launch alone.
 hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
                          int pfilter, int use_classic, int n_cu, double* sum, double* sumsq,
                          hipEvent_t* ev = nullptr);

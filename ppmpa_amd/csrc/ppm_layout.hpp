@@ -41,7 +41,10 @@ enum : uint32_t { LIGHT_POINT = 0, LIGHT_PARALLELOGRAM = 1, LIGHT_SUN = 2 };
 enum : int { FILTER_NONE = 0, FILTER_CONE = 1, FILTER_GAUSS = 2 };
 
 constexpr int MAX_DEPTH_LIMIT = 16;  // deepest MAX_TRACE the kernels accept
-constexpr int LEAF = 64;             // photons per map leaf = one wavefront
+#ifndef PPM_LEAF
+#define PPM_LEAF 32
+#endif
+constexpr int LEAF = PPM_LEAF;       // photons per map leaf (<= one wavefront)
 
 // --- scene records (80 B) -------------------------------------------------
 struct alignas(16) DevShape {

@@ -253,14 +253,21 @@ class Scene:
 # ---------------------------------------------------------------------------
 # Synthetic scenes of the benchmark configurations.
 
-BOX_BOUNDS = ((-0.01, -0.01, -1.0), (1.01, 1.01, 1.01))
+BOX_BOUNDS = ((-0.01, -0.01, -0.01), (1.01, 1.01, 1.01))
+TUNNEL_BOUNDS = ((-0.01, -0.01, -1.0), (1.01, 1.01, 1.01))
 
 
 def cornell_box(sc: Scene, left=(0.75, 0.25, 0.25), right=(0.25, 0.75, 0.25), grey=0.75, flux=100.0,
-                light_size=0.25):
-    """The 1x1x1 m box of five Lambertian planes (floor y=0, ceiling y=1, back
+                light_size=0.25, walls="quad"):
+    """The 1x1x1 m box of five Lambertian walls (floor y=0, ceiling y=1, back
     z=1, left x=0, right x=1; open at z=0) with a light_size^2 ceiling area
     light of `flux` watts of white light at y = 0.999.
+
+    walls="quad": each wall is a 1 x 1 m Parallelogram (geometry.rs:84-89,
+    the reference's finite planar shape), so photons leaving through the
+    open front escape as from a real box.  walls="plane": infinite Plain
+    shapes (geometry.rs:70-73), which turn the open front into an endless
+    tunnel that photons wander and are stored in.
 
     The light is a ParallelogramLight plus an emissive Parallelogram object at
     the same place: the reference's direct-light test needs the shadow ray to
@@ -273,13 +280,25 @@ def cornell_box(sc: Scene, left=(0.75, 0.25, 0.25), right=(0.25, 0.75, 0.25), gr
     g = sc.add_material(lambertian(grey))
     l = sc.add_material(lambertian(left))
     r = sc.add_material(lambertian(right))
-    sc.shapes += [
-        plain((0, 1, 0), 0.0, g),    # floor
-        plain((0, -1, 0), 1.0, g),   # ceiling
-        plain((0, 0, -1), 1.0, g),   # back wall
-        plain((1, 0, 0), 0.0, l),    # left wall
-        plain((-1, 0, 0), 1.0, r),   # right wall
-    ]
+    if walls == "plane":
+        sc.shapes += [
+            plain((0, 1, 0), 0.0, g),    # floor
+            plain((0, -1, 0), 1.0, g),   # ceiling
+            plain((0, 0, -1), 1.0, g),   # back wall
+            plain((1, 0, 0), 0.0, l),    # left wall
+            plain((-1, 0, 0), 1.0, r),   # right wall
+        ]
+        sc.bounds_min, sc.bounds_max = TUNNEL_BOUNDS
+    else:
+        # corners ordered so that dir1 x dir2 points into the box
+        sc.shapes += [
+            parallelogram((0, 0, 0), (0, 0, 1), (1, 0, 0), g),  # floor
+            parallelogram((0, 1, 0), (1, 1, 0), (0, 1, 1), g),  # ceiling
+            parallelogram((0, 0, 1), (0, 1, 1), (1, 0, 1), g),  # back wall
+            parallelogram((0, 0, 0), (0, 1, 0), (0, 0, 1), l),  # left wall
+            parallelogram((1, 0, 0), (1, 0, 1), (1, 1, 0), r),  # right wall
+        ]
+        sc.bounds_min, sc.bounds_max = BOX_BOUNDS
     s = light_size
     x0 = 0.5 - s / 2
     p0 = (x0, 0.999, x0)
@@ -288,7 +307,6 @@ def cornell_box(sc: Scene, left=(0.75, 0.25, 0.25), right=(0.25, 0.75, 0.25), gr
     lm = sc.add_material(simple((0, 0, 0), (0.8, 0.8, 0.8), 0.0, 0.0, 0.0, emittance=(em, em, em)))
     sc.shapes.append(parallelogram(p0, (x0 + s, 0.999, x0), (x0, 0.999, x0 + s), lm, nvec=(0, -1, 0)))
     sc.lights.append(parallelogram_light((1, 1, 1), flux, p0, (0, -1, 0), (s, 0, 0), (0, 0, s)))
-    sc.bounds_min, sc.bounds_max = BOX_BOUNDS
     return sc
 
 
@@ -298,10 +316,10 @@ def box_camera(xreso, yreso, antialias=True) -> Camera:
     return pinhole(xreso, yreso, (0.5, 0.5, -0.87), (0.5, 0.5, 1.0), 60.0, antialias=antialias)
 
 
-def config0_scene(xreso=256, yreso=256) -> Scene:
+def two_spheres_scene(xreso=256, yreso=256, walls="quad") -> Scene:
     """Cornell box + a diffuse sphere (r 0.2, albedo 0.75) + a perfect-mirror
     sphere (r 0.2); pinhole camera, 60 deg FOV."""
-    sc = cornell_box(Scene())
+    sc = cornell_box(Scene(), walls=walls)
     d = sc.add_material(lambertian(0.75))
     m = sc.add_material(mirror())
     sc.shapes += [sphere((0.3, 0.2, 0.65), 0.2, d), sphere((0.72, 0.2, 0.45), 0.2, m)]
@@ -309,10 +327,10 @@ def config0_scene(xreso=256, yreso=256) -> Scene:
     return sc
 
 
-def config1_scene(xreso=1024, yreso=768) -> Scene:
-    """config0's box with a diffuse, a mirror and a glass (IOR 1.5) sphere
+def caustic_scene(xreso=1024, yreso=768, walls="quad") -> Scene:
+    """The box of two_spheres_scene with a diffuse, a mirror and a glass (IOR 1.5) sphere
     (r 0.2 each); the glass sphere sits under the light to cast a caustic."""
-    sc = cornell_box(Scene())
+    sc = cornell_box(Scene(), walls=walls)
     d = sc.add_material(lambertian(0.75))
     m = sc.add_material(mirror())
     gl = sc.add_material(glass(1.5))
@@ -350,10 +368,11 @@ def reference_room() -> Scene:
     return sc
 
 
-# Pass schedules of the configurations (BASELINE configs 0 and 1).
+# Pass schedules of the two workloads: the small CPU-runnable one and the
+# 2M-photon, 512-pass caustic scene the throughput is quoted on.
 CONFIGS = {
-    "config0": dict(scene=config0_scene, n_photon=200_000, passes=16, radius0=0.01, alpha=2.0 / 3.0, max_depth=8,
+    "two_spheres": dict(scene=two_spheres_scene, n_photon=200_000, passes=16, radius0=0.01, alpha=2.0 / 3.0, max_depth=8,
                     seed=1),
-    "config1": dict(scene=config1_scene, n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0,
+    "caustic": dict(scene=caustic_scene, n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0,
                     max_depth=16, seed=1),
 }

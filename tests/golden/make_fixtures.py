@@ -28,11 +28,11 @@ R2S = (0.01 ** 2, 0.03 ** 2)
 
 
 def main():
-    sc = P.config0_scene(32, 32)
+    sc = P.two_spheres_scene(32, 32)
     osc = O.OScene(sc)
     pos, dir, wl, meta, power = osc.trace_photons(seed=1, pass_=1, n_photon=N_PHOTON, uc=True, maxd=8)
     text = M.format_map(N_PHOTON, power, pos, dir, wl)
-    with open(os.path.join(HERE, "map_config0.txt"), "w") as f:
+    with open(os.path.join(HERE, "map_two_spheres.txt"), "w") as f:
         f.write(text)
     m = M.parse_map(text)  # what read_map sees (exact round trip of the doubles)
     rng = np.random.default_rng(20261202)
@@ -50,7 +50,7 @@ def main():
         out[f"off_{j}"] = np.cumsum([0] + [len(s) for s in sets]).astype(np.uint64)
         for pf in (0, 1, 2):
             out[f"rad_{j}_{pf}"] = om.gather(q, nrm, r2, pf)
-    np.savez(os.path.join(HERE, "queries_config0.npz"), **out)
+    np.savez(os.path.join(HERE, "queries_two_spheres.npz"), **out)
     print(f"{len(m.wl)} photons, {len(q)} queries")
 
 

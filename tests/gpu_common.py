@@ -27,10 +27,10 @@ def say(*a):
     sys.stdout.flush()
 
 
-def small_scene(kind="config0", xreso=64, yreso=48):
-    if kind == "config0":
-        return P.config0_scene(xreso, yreso)
-    return P.config1_scene(xreso, yreso)
+def small_scene(kind="two_spheres", xreso=64, yreso=48):
+    if kind == "two_spheres":
+        return P.two_spheres_scene(xreso, yreso)
+    return P.caustic_scene(xreso, yreso)
 
 
 def random_photons(n, seed, lo=0.0, hi=1.0, clustered=False):

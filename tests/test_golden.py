@@ -14,16 +14,16 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 @pytest.fixture(scope="module")
 def gold():
-    m = M.parse_map(os.path.join(GOLD, "map_config0.txt"))
-    z = np.load(os.path.join(GOLD, "queries_config0.npz"), allow_pickle=False)
+    m = M.parse_map(os.path.join(GOLD, "map_two_spheres.txt"))
+    z = np.load(os.path.join(GOLD, "queries_two_spheres.npz"), allow_pickle=False)
     return m, {k: z[k] for k in z.files}
 
 
 def test_map_text_round_trip(gold):
     m, _ = gold
     assert m.n_emitted == 4000 and m.power == 0.025
-    assert len(m.wl) > 5000
-    text = open(os.path.join(GOLD, "map_config0.txt")).read()
+    assert len(m.wl) > 3000
+    text = open(os.path.join(GOLD, "map_two_spheres.txt")).read()
     assert M.format_map(m.n_emitted, m.power, m.pos, m.dir, m.wl) == text
 
 

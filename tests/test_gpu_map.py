@@ -158,10 +158,10 @@ def test_golden_map_bitexact_sets_and_radiance():
     from ppmpa_amd import photonmap_io as M
     say("golden map")
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-    z = np.load(os.path.join(gold, "queries_config0.npz"), allow_pickle=False)
+    z = np.load(os.path.join(gold, "queries_two_spheres.npz"), allow_pickle=False)
     c = ctx()
-    m, ph = P.read_map(c, os.path.join(gold, "map_config0.txt"))
-    pm = M.parse_map(os.path.join(gold, "map_config0.txt"))
+    m, ph = P.read_map(c, os.path.join(gold, "map_two_spheres.txt"))
+    pm = M.parse_map(os.path.join(gold, "map_two_spheres.txt"))
     for j, r2 in enumerate(z["r2"]):
         got = m.within(z["q"], r2, refine64=True)
         off, idx = z[f"off_{j}"], z[f"idx_{j}"]

@@ -23,7 +23,7 @@ def _block(img, b):
 
 def test_images_agree_within_3_sigma():
     say("3 sigma")
-    sc = P.config0_scene(64, 64)
+    sc = P.two_spheres_scene(64, 64)
     K = 24
     c = ctx()
     ds = P.DeviceScene(c, sc)

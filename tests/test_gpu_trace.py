@@ -21,7 +21,7 @@ def _keyed(meta, pos, dir, wl):
     return keys[order], pos[order], dir[order], wl[order]
 
 
-@pytest.mark.parametrize("kind,uc,maxd", [("config0", True, 8), ("config1", True, 16), ("config1", False, 10)])
+@pytest.mark.parametrize("kind,uc,maxd", [("two_spheres", True, 8), ("caustic", True, 16), ("caustic", False, 10)])
 def test_photon_trace_crn(kind, uc, maxd):
     say("photon trace crn", kind, uc, maxd)
     sc = small_scene(kind)
@@ -110,7 +110,7 @@ def test_channel_frequencies():
 def test_eye_pass_crn():
     """One eye pass on the same photon set: GPU float vs oracle FP64."""
     say("eye pass crn")
-    sc = small_scene("config1", 96, 72)
+    sc = small_scene("caustic", 96, 72)
     c = ctx()
     osc = O.OScene(sc)
     pos, dir, wl, meta, pw = osc.trace_photons(5, 1, 40000, True, 16)
@@ -132,7 +132,7 @@ def test_eye_pass_crn():
 def test_session_matches_oracle_crn():
     """A few PPM-PA passes: device session vs oracle passes (same seed)."""
     say("session crn")
-    sc = small_scene("config0", 64, 64)
+    sc = small_scene("two_spheres", 64, 64)
     c = ctx()
     ds = P.DeviceScene(c, sc)
     cfg = P.pass_cfg(seed=3, n_photon=20000, radius0=0.03, alpha=2 / 3, max_depth=8)
