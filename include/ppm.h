@@ -131,8 +131,17 @@ typedef struct {
   double  eex[3], eey[3];
 } ppm_camera_desc;
 
+/* ppm_scene_desc.flags: how rays find their nearest shape.  The reference
+ * tests every object (calc_intersection, tracer.rs:306-335); a BVH built on
+ * the device gives the same nearest hit (ties go to the lower shape index,
+ * as the reference's stable sort keeps them). */
+#define PPM_SCENE_BVH_AUTO 0 /* BVH when there are >= 64 spheres/polygons/parallelograms */
+#define PPM_SCENE_BVH_OFF  1 /* every shape tested one by one */
+#define PPM_SCENE_BVH_ON   2 /* BVH over the bounded shapes; planes one by one */
+
 typedef struct {
-  int32_t n_shapes, n_materials, n_lights, reserved;
+  int32_t n_shapes, n_materials, n_lights;
+  int32_t flags;                /* PPM_SCENE_BVH_* */
   const ppm_shape_desc*    shapes;
   const ppm_material_desc* materials;
   const ppm_light_desc*    lights;
@@ -199,6 +208,21 @@ ppm_status  ppm_scene_create(ppm_ctx* ctx, const ppm_scene_desc* desc,
                              ppm_scene** out);
 void        ppm_scene_destroy(ppm_scene* scene);
 
+/* How the scene was laid out: shapes tested one by one, shapes in the BVH
+ * and the BVH's leaf level (0 with no BVH).  No counterpart in the reference
+ * (which has no BVH); for callers and tests that check the layout. */
+ppm_status  ppm_scene_stats(const ppm_scene* scene, uint32_t* n_linear,
+                            uint32_t* n_bvh, uint32_t* bvh_depth);
+
+/* calc_intersection (tracer.rs:306-335) for n rays: origins and directions
+ * are n x 3 doubles (directions are normalised first, as
+ * Ray::new_from_elem does, geometry.rs:46-52); t_out receives the distance of the nearest hit with
+ * t >= 1e-4 (tracer.rs:324) and obj_out the shape's index in the scene, or
+ * (inf, -1) when nothing is hit.  Computed in float, like the passes. */
+ppm_status  ppm_intersect(ppm_ctx* ctx, const ppm_scene* scene,
+                          const double* origins, const double* dirs,
+                          uint64_t n, double* t_out, int32_t* obj_out);
+
 /* ---- photons (H1, H2) ---------------------------------------------------- */
 
 /* A device-resident photon set of fixed capacity. */
@@ -242,7 +266,7 @@ ppm_status  ppm_map_create(ppm_ctx* ctx, uint64_t capacity, ppm_map** out);
 void        ppm_map_destroy(ppm_map* map);
 
 /* build_photonmap (photonmap.rs:23-29): Morton keys within the scene's
- * bounds, radix sort, and a balanced tree of 64-photon leaves.  power is the
+ * bounds, radix sort, and a balanced tree of 32-photon leaves.  power is the
  * per-photon power (PhotonMap.power).  scene may be NULL: the bounds are then
  * those recorded with the photon set (the bounding box of an uploaded set, or
  * the bounds of the scene that traced it). */

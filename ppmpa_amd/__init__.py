@@ -4,7 +4,8 @@ radiance gather, eye rays, and the PPM-PA pass schedule with FP64
 accumulation, as hand-written HIP kernels behind the C ABI of include/ppm.h.
 """
 from . import _abi
-from .scene import (Camera, Light, Material, Scene, Shape, CONFIGS, two_spheres_scene, caustic_scene, cornell_box,
+from .scene import (Camera, Light, Material, Mesh, Scene, Shape, CONFIGS, two_spheres_scene, caustic_scene, cornell_box,
+                    icosphere, mesh_scene,
                     glass, lambertian, mirror, parallelogram, parallelogram_light, pinhole, plain, point_light,
                     polygon, read_camera, reference_room, simple, sphere, sun_light, ts)
 from .tracer import (Context, DeviceScene, PhotonMap, PhotonSet, PpmError, Session, build_photonmap, pass_cfg,

@@ -25,6 +25,8 @@ LIGHT_POINT, LIGHT_PARALLELOGRAM, LIGHT_SUN = range(3)
 FILTER_NONE, FILTER_CONE, FILTER_GAUSS = range(3)
 WL_RED, WL_GREEN, WL_BLUE = range(3)
 
+SCENE_BVH_AUTO, SCENE_BVH_OFF, SCENE_BVH_ON = range(3)
+
 SESSION_STAGE_TIMING = 1
 SESSION_NO_GRAPH = 2
 
@@ -95,7 +97,7 @@ class SceneDesc(C.Structure):
         ("n_shapes", C.c_int32),
         ("n_materials", C.c_int32),
         ("n_lights", C.c_int32),
-        ("reserved", C.c_int32),
+        ("flags", C.c_int32),
         ("shapes", C.POINTER(ShapeDesc)),
         ("materials", C.POINTER(MaterialDesc)),
         ("lights", C.POINTER(LightDesc)),
@@ -154,6 +156,8 @@ PROTOTYPES = {
                               C.c_double, C.c_double, C.c_double]),
     "ppm_scene_create": (st, [vp, C.POINTER(SceneDesc), C.POINTER(vp)]),
     "ppm_scene_destroy": (None, [vp]),
+    "ppm_scene_stats": (st, [vp, u32p, u32p, u32p]),
+    "ppm_intersect": (st, [vp, vp, dp, dp, C.c_uint64, dp, C.POINTER(C.c_int32)]),
     "ppm_photons_create": (st, [vp, C.c_uint64, C.POINTER(vp)]),
     "ppm_photons_destroy": (None, [vp]),
     "ppm_trace_photons": (st, [vp, vp, C.c_uint64, C.c_uint32, C.c_uint64, C.c_int32, C.c_int32, vp]),

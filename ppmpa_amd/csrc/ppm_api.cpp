@@ -352,6 +352,103 @@ ppm_status ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso, 
   return PPM_OK;
 }
 
+namespace {
+
+// Shapes tested one by one and, when the scene asks for one (or has at least
+// BVH_MIN_PRIMS bounded shapes in automatic mode), the device-built BVH over
+// the bounded ones.  shapes[] is already on the device as sc->buf.shapes.
+ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
+  SceneBuf& b = sc->buf;
+  std::vector<uint32_t> bounded;
+  std::vector<DevShape> lin;
+  for (size_t i = 0; i < shapes.size(); ++i) {
+    const uint32_t k = shapes[i].kind;
+    if (k == SHAPE_SPHERE || k == SHAPE_POLYGON || k == SHAPE_PARALLELOGRAM)
+      bounded.push_back((uint32_t)i);
+    else
+      lin.push_back(shapes[i]);
+  }
+  const bool use_bvh = flags == PPM_SCENE_BVH_ON || (flags == PPM_SCENE_BVH_AUTO && bounded.size() >= BVH_MIN_PRIMS);
+  if (!use_bvh || bounded.empty()) {
+    b.lin = b.shapes;
+    b.n_lin = b.n_shapes;
+    return PPM_OK;
+  }
+  const uint64_t n = bounded.size();
+  const uint64_t leaves = (n + BVH_LEAF - 1) / BVH_LEAF;
+  uint32_t depth = 0;
+  while ((1ull << depth) < leaves) ++depth;
+  if (depth > BVH_MAX_DEPTH) return fail(c, PPM_E_RANGE, "too many shapes for the BVH (at most 16M)");
+  // centroid bounds (host, from the float records the device will see)
+  double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i : bounded) {
+    const DevShape& s = shapes[i];
+    const double p[3] = {s.p.x, s.p.y, s.p.z};
+    double lo[3], hi[3];
+    if (s.kind == SHAPE_SPHERE) {
+      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = p[a];
+    } else {
+      const double d1[3] = {s.d1.x, s.d1.y, s.d1.z}, d2[3] = {s.d2.x, s.d2.y, s.d2.z};
+      const double q = s.kind == SHAPE_PARALLELOGRAM ? 1.0 : 0.0;  // its fourth corner p + d1 + d2
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
+        hi[a] = std::max({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
+      }
+    }
+    for (int a = 0; a < 3; ++a) {
+      const double cc = 0.5 * (lo[a] + hi[a]);
+      cmin[a] = std::min(cmin[a], cc);
+      cmax[a] = std::max(cmax[a], cc);
+    }
+  }
+  float4 fmin = make_float4((float)cmin[0], (float)cmin[1], (float)cmin[2], 0.0f);
+  float4 fscale;
+  float* sc3[3] = {&fscale.x, &fscale.y, &fscale.z};
+  for (int a = 0; a < 3; ++a) {
+    const double ext = cmax[a] - cmin[a];
+    *sc3[a] = ext > 0.0 ? (float)(1023.0 / ext) : 0.0f;
+  }
+  fscale.w = 0.0f;
+  const uint32_t sort_blocks =
+      (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)c->n_cu * 4, (n + 2047) / 2048)));
+  uint32_t* ids = nullptr;
+  uint32_t *keys[2] = {nullptr, nullptr}, *vals[2] = {nullptr, nullptr};
+  uint32_t *np = nullptr, *ghist = nullptr, *totals = nullptr;
+  Node* pbox = nullptr;
+  const uint32_t n32 = (uint32_t)n;
+  hipError_t e;
+  if ((e = dalloc(&ids, n)) == hipSuccess && (e = dalloc(&keys[0], n)) == hipSuccess &&
+      (e = dalloc(&keys[1], n)) == hipSuccess && (e = dalloc(&vals[0], n)) == hipSuccess &&
+      (e = dalloc(&vals[1], n)) == hipSuccess && (e = dalloc(&np, 1)) == hipSuccess &&
+      (e = dalloc(&ghist, 256ull * sort_blocks)) == hipSuccess && (e = dalloc(&totals, 256)) == hipSuccess &&
+      (e = dalloc(&pbox, n)) == hipSuccess && (e = dalloc(&b.bprims, n)) == hipSuccess &&
+      (e = dalloc(&b.bvh, (2ull << depth) - 1)) == hipSuccess &&
+      (e = dalloc(&b.lin, lin.size())) == hipSuccess &&
+      (e = hipMemcpy(ids, bounded.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = hipMemcpy(np, &n32, sizeof(n32), hipMemcpyHostToDevice)) == hipSuccess &&
+      (lin.empty() ||
+       (e = hipMemcpy(b.lin, lin.data(), lin.size() * sizeof(DevShape), hipMemcpyHostToDevice)) == hipSuccess) &&
+      (e = launch_bvh_build(c->stream, b.shapes, ids, n32, fmin, fscale, depth, keys, vals, np, ghist, totals,
+                            sort_blocks, pbox, b.bprims, b.bvh)) == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  dfree(ids);
+  dfree(keys[0]);
+  dfree(keys[1]);
+  dfree(vals[0]);
+  dfree(vals[1]);
+  dfree(np);
+  dfree(ghist);
+  dfree(totals);
+  dfree(pbox);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  b.n_lin = (uint32_t)lin.size();
+  b.n_bprims = n32;
+  b.bvh_depth = depth;
+  return PPM_OK;
+}
+
+}  // namespace
+
 ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out) {
   if (!c || !d || !out) return fail(c, PPM_E_INVALID, "null argument");
   *out = nullptr;
@@ -387,7 +484,10 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
       if (!norm3(cr, n)) return fail(c, PPM_E_INVALID, "degenerate polygon");
     }
     o.n = f4(n);
+    const uint32_t idx = (uint32_t)i;
+    std::memcpy(&o.p.w, &idx, sizeof(idx));
   }
+  if (d->flags < PPM_SCENE_BVH_AUTO || d->flags > PPM_SCENE_BVH_ON) return fail(c, PPM_E_INVALID, "bad scene flags");
   std::vector<DevMaterial> mats(d->n_materials + 1);
   for (int i = 0; i <= d->n_materials; ++i) {
     DevMaterial& o = mats[i];
@@ -476,6 +576,11 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
     ppm_scene_destroy(sc);
     return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
   }
+  const ppm_status bs = scene_layout(c, sc, shapes, d->flags);
+  if (bs != PPM_OK) {
+    ppm_scene_destroy(sc);
+    return bs;
+  }
   *out = sc;
   return PPM_OK;
 }
@@ -483,10 +588,58 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
 void ppm_scene_destroy(ppm_scene* sc) {
   if (!sc) return;
   if (sc->ctx) (void)hipStreamSynchronize(sc->ctx->stream);
+  if (sc->buf.lin != sc->buf.shapes) dfree(sc->buf.lin);
+  sc->buf.lin = nullptr;
   dfree(sc->buf.shapes);
   dfree(sc->buf.mats);
   dfree(sc->buf.lights);
+  dfree(sc->buf.bvh);
+  dfree(sc->buf.bprims);
   delete sc;
+}
+
+ppm_status ppm_scene_stats(const ppm_scene* sc, uint32_t* n_linear, uint32_t* n_bvh, uint32_t* bvh_depth) {
+  if (!sc) return PPM_E_INVALID;
+  if (n_linear) *n_linear = sc->buf.n_lin;
+  if (n_bvh) *n_bvh = sc->buf.n_bprims;
+  if (bvh_depth) *bvh_depth = sc->buf.bvh_depth;
+  return PPM_OK;
+}
+
+ppm_status ppm_intersect(ppm_ctx* c, const ppm_scene* sc, const double* origins, const double* dirs, uint64_t n,
+                         double* t_out, int32_t* obj_out) {
+  if (!c || !sc || (n && (!origins || !dirs || !t_out || !obj_out))) return fail(c, PPM_E_INVALID, "null argument");
+  if (n > (1ull << 26)) return fail(c, PPM_E_RANGE, "at most 2^26 rays per call");
+  if (n == 0) return PPM_OK;
+  std::vector<float4> ho(n), hd(n);
+  for (uint64_t i = 0; i < n; ++i) {
+    const double* o = origins + 3 * i;
+    double dn[3];
+    if (!finite3(o) || !finite3(dirs + 3 * i)) return fail(c, PPM_E_NONFINITE, "non-finite ray");
+    if (!norm3(dirs + 3 * i, dn)) return fail(c, PPM_E_INVALID, "zero-length ray direction");
+    ho[i] = f4(o);
+    hd[i] = f4(dn);
+  }
+  float4 *d_o = nullptr, *dD = nullptr;
+  float* dt = nullptr;
+  int32_t* dobj = nullptr;
+  std::vector<float> ht(n);
+  hipError_t e;
+  if ((e = dalloc(&d_o, n)) == hipSuccess && (e = dalloc(&dD, n)) == hipSuccess && (e = dalloc(&dt, n)) == hipSuccess &&
+      (e = dalloc(&dobj, n)) == hipSuccess &&
+      (e = hipMemcpyAsync(d_o, ho.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream)) == hipSuccess &&
+      (e = hipMemcpyAsync(dD, hd.data(), n * sizeof(float4), hipMemcpyHostToDevice, c->stream)) == hipSuccess &&
+      (e = launch_intersect(c->stream, sc->buf, d_o, dD, (uint32_t)n, dt, dobj)) == hipSuccess &&
+      (e = hipMemcpyAsync(ht.data(), dt, n * sizeof(float), hipMemcpyDeviceToHost, c->stream)) == hipSuccess &&
+      (e = hipMemcpyAsync(obj_out, dobj, n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream)) == hipSuccess)
+    e = hipStreamSynchronize(c->stream);
+  dfree(d_o);
+  dfree(dD);
+  dfree(dt);
+  dfree(dobj);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  for (uint64_t i = 0; i < n; ++i) t_out[i] = (double)ht[i];
+  return PPM_OK;
 }
 
 // ---------------------------------------------------------------------------

@@ -113,9 +113,15 @@ struct Rng {
 // Scene view passed by value to every kernel.
 
 struct SceneView {
-  const DevShape* shapes;
+  const DevShape* shapes;   // in scene order
   const DevMaterial* mats;  // n_mats + 1 entries: the last one is M_AIR
   const DevLight* lights;
+  const DevShape* lin;      // shapes tested one by one (all of them, or the unbounded ones beside a BVH)
+  const Node* bvh;          // BVH nodes (null: no BVH)
+  const DevShape* bprims;   // BVH primitives in leaf order
+  uint32_t n_lin;
+  uint32_t n_bprims;
+  uint32_t bvh_depth;       // leaves at level bvh_depth
   uint32_t n_shapes;
   uint32_t n_mats;          // index of M_AIR
   uint32_t n_lights;
@@ -132,20 +138,24 @@ struct Hit {
   int32_t obj;
 };
 
+// Nearest-hit update: the earlier of two equal distances is the shape that
+// comes first in the scene, as the stable sort of tracer.rs:335 keeps it.
+__device__ __forceinline__ void take(Hit& h, float t, int32_t idx) {
+  if (t >= NEARLY0_F && (t < h.t || (t == h.t && idx < h.obj))) {
+    h.t = t;
+    h.obj = idx;
+  }
+}
+
 // distance_plain (geometry.rs:170-177), distance_sphere (:179-193) and
 // method_moller (:149-165) folded into a nearest-hit update.  Candidates with
-// t < NEARLY0 are dropped (tracer.rs:324); ties keep the earlier object and
-// the earlier candidate, as the stable sort of tracer.rs:335 does.
+// t < NEARLY0 are dropped (tracer.rs:324).
 __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, V3 d, Hit& h) {
-  const float eps = NEARLY0_F;
   switch (s.kind) {
     case SHAPE_PLAIN: {
       V3 n = mk(s.n);
       float c = dot(n, d);
-      if (c != 0.0f) {
-        float t = (s.dist + dot(n, o)) / -c;
-        if (t >= eps && t < h.t) { h.t = t; h.obj = idx; }
-      }
+      if (c != 0.0f) take(h, (s.dist + dot(n, o)) / -c, idx);
       break;
     }
     case SHAPE_SPHERE: {
@@ -155,11 +165,10 @@ __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, 
       if (t1 > 0.0f) {
         float t2 = sqrtf(t1);
         if (t2 == 0.0f) {
-          if (t0 >= eps && t0 < h.t) { h.t = t0; h.obj = idx; }
+          take(h, t0, idx);
         } else {
-          float ta = t0 - t2, tb = t0 + t2;
-          if (ta >= eps && ta < h.t) { h.t = ta; h.obj = idx; }
-          if (tb >= eps && tb < h.t) { h.t = tb; h.obj = idx; }
+          take(h, t0 - t2, idx);
+          take(h, t0 + t2, idx);
         }
       }
       break;
@@ -177,9 +186,7 @@ __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, 
         float v = dot(te1, d) * inv;
         float t = dot(te1, d2) * inv;
         float lim = s.kind == SHAPE_POLYGON ? 1.0f : 2.0f;
-        if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > lim)) {
-          if (t >= eps && t < h.t) { h.t = t; h.obj = idx; }
-        }
+        if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > lim)) take(h, t, idx);
       }
       break;
     }
@@ -200,12 +207,78 @@ __device__ __forceinline__ T ldc(const T* p, uint32_t i) {
 #endif
 }
 
-__device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d) {
-  Hit h{INF_F, -1};
-  for (uint32_t i = 0; i < sc.n_shapes; ++i) {
-    const DevShape s = ldc(sc.shapes, i);
-    shape_hit(s, (int32_t)i, o, d, h);
+// Slab test of a node box: the ray (o, 1/d) enters the box at tn within
+// [0, tmax].  An axis whose two plane distances are NaN (ray in the plane of
+// a flat box) does not constrain: fminf/fmaxf drop NaN operands.
+__device__ __forceinline__ bool slab(const Node& b, V3 o, V3 inv, float tmax, float& tn) {
+  const float x0 = (b.lo.x - o.x) * inv.x, x1 = (b.hi.x - o.x) * inv.x;
+  const float y0 = (b.lo.y - o.y) * inv.y, y1 = (b.hi.y - o.y) * inv.y;
+  const float z0 = (b.lo.z - o.z) * inv.z, z1 = (b.hi.z - o.z) * inv.z;
+  const float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
+  const float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
+  tn = t0;
+  return t0 <= t1;
+}
+
+// Nearest hit over the BVH primitives: ordered depth-first traversal (the
+// nearer child first), the farther child on the lane's LDS stack `stk`
+// (BVH_STACK entries 256 words apart: one column per lane of a 256-thread
+// block, so the lanes of a wave hit distinct banks).  A node is entered while its box starts no later than
+// the best hit, so equal distances still reach the lower shape index.
+__device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h, uint32_t* stk) {
+  const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
+  uint32_t node = 0, sp = 0;
+  float tn;
+  if (!slab(sc.bvh[0], o, inv, h.t, tn)) return;
+  while (true) {
+    if (node >= first_leaf) {
+      const uint32_t base = (node - first_leaf) * BVH_LEAF;
+      const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
+      for (uint32_t j = base; j < end; ++j) {
+        const DevShape s = sc.bprims[j];
+        shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
+      }
+    } else {
+      const uint32_t l = 2u * node + 1u;
+      float tl, tr;
+      const bool hl = slab(sc.bvh[l], o, inv, h.t, tl);
+      const bool hr = slab(sc.bvh[l + 1u], o, inv, h.t, tr);
+      if (hl && hr) {
+        const bool left_first = tl <= tr;
+        stk[sp * 256u] = left_first ? l + 1u : l;
+        ++sp;
+        node = left_first ? l : l + 1u;
+        continue;
+      }
+      if (hl || hr) {
+        node = hl ? l : l + 1u;
+        continue;
+      }
+    }
+    bool more = false;
+    while (sp > 0) {
+      --sp;
+      node = stk[sp * 256u];
+      if (slab(sc.bvh[node], o, inv, h.t, tn)) {
+        more = true;
+        break;
+      }
+    }
+    if (!more) break;
   }
+}
+
+// calc_intersection (tracer.rs:306-335): the nearest hit over every shape.
+// BVH = false: the scene has no BVH and `lin` holds all shapes.
+template <bool BVH>
+__device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d, uint32_t* stk) {
+  Hit h{INF_F, -1};
+  for (uint32_t i = 0; i < sc.n_lin; ++i) {
+    const DevShape s = ldc(sc.lin, i);
+    shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
+  }
+  if (BVH) bvh_nearest(sc, o, d, h, stk);
   return h;
 }
 
@@ -237,8 +310,9 @@ __device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit 
   return true;
 }
 
-__device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is) {
-  return finish_hit(sc, o, d, nearest(sc, o, d), is);
+template <bool BVH>
+__device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is, uint32_t* stk) {
+  return finish_hit(sc, o, d, nearest<BVH>(sc, o, d, stk), is);
 }
 
 // ---------------------------------------------------------------------------
@@ -405,14 +479,15 @@ __device__ __forceinline__ BsdfW bsdf_weights(const DevMaterial& m, float cos0) 
 // the radiance list before the zip (light.rs:132), so sample i's cos pairs
 // with sample i-1's radiance and the last radiance is dropped: the running
 // `prev` carries that shift.
+template <bool BVH>
 __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, float& cos0sq,
-                                           float& sqd) {
+                                           float& sqd, uint32_t* stk) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
   if (c0 < 0.0f) return false;
   Isect is;
-  if (!intersect(sc, p, ld3, is)) return false;
+  if (!intersect<BVH>(sc, p, ld3, is, stk)) return false;
   float sq_l = dot(ld, ld);
   V3 od = is.pos - p;
   float sq_o = dot(od, od);
@@ -449,7 +524,8 @@ __device__ __forceinline__ V3 tss_point(const DevLight& L, int s) {
   return mk(L.pos) + tss(s / 5) * mk(L.d1) + tss(s % 5) * mk(L.d2);
 }
 
-__device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
+template <bool BVH>
+__device__ C3 direct_light(const SceneView& sc, V3 p, V3 n, uint32_t* stk) {
   C3 sum{0.0f, 0.0f, 0.0f};
   for (uint32_t li = 0; li < sc.n_lights; ++li) {
     const DevLight L = ldc(sc.lights, li);
@@ -460,7 +536,7 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit(sc, p, n, ld, csq, sqd)) continue;
+      if (!sample_lit<BVH>(sc, p, n, ld, csq, sqd, stk)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }

@@ -24,6 +24,12 @@ static SceneView make_view(const SceneBuf& b) {
   v.shapes = b.shapes;
   v.mats = b.mats;
   v.lights = b.lights;
+  v.lin = b.lin;
+  v.bvh = b.bvh;
+  v.bprims = b.bprims;
+  v.n_lin = b.n_lin;
+  v.n_bprims = b.n_bprims;
+  v.bvh_depth = b.bvh_depth;
   v.n_shapes = b.n_shapes;
   v.n_mats = b.n_mats;
   v.n_lights = b.n_lights;
@@ -129,12 +135,13 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
 
 // One step of trace_photon.  Returns whether the photon lives on; `store`
 // reports a photon-map record (position, INCOMING direction, channel).
+template <bool BVH>
 __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int maxd, PhotonState& p, Rng& rng,
-                                              bool& store, V3& spos) {
+                                              bool& store, V3& spos, uint32_t* stk) {
   store = false;
   if ((int)p.depth >= maxd) return false;  // tracer.rs:32
   Isect is;
-  if (!intersect(sc, p.o, p.d, is)) return false;  // tracer.rs:36-39
+  if (!intersect<BVH>(sc, p.o, p.d, is, stk)) return false;  // tracer.rs:36-39
   const DevMaterial& M = sc.mats[is.mat];
   // tracer.rs:77-79
   store = (uc == 0 || p.depth > 0) && surf_store(M);
@@ -216,9 +223,11 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
 // + mbcnt, a wave-uniform counter in registers), and stored records go to
 // consecutive slots of the region (coalesced 32-byte writes).  No global
 // atomics: the record order depends only on the data, so it is reproducible.
+template <bool BVH>
 __global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
                                                        uint64_t seed, int uc, int maxd) {
+  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (w >= n_waves) return;
   const uint32_t pass = st->pass;
@@ -254,7 +263,7 @@ __global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* 
     V3 spos = mk(0.0f, 0.0f, 0.0f);
     const V3 sdir = p.d;
     const uint32_t smeta = pack_meta(p.wl, p.depth, p.pid);
-    if (alive) alive = photon_bounce(sc, uc, maxd, p, rng, store, spos);
+    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, stk + threadIdx.x);
     const uint64_t sm = __ballot(store);
     if (sm) {
       if (store) {
@@ -329,8 +338,12 @@ hipError_t launch_trace_photons(hipStream_t s, const SceneBuf& sc, PassState* st
                                 int use_classic, int max_depth) {
   const uint32_t n_waves = pb.n_regions;
   const uint32_t per_wave = (uint32_t)((sc.n_emit_total + (uint64_t)n_waves - 1) / n_waves);
-  hipLaunchKernelGGL(k_trace_photons, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
-                     pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_trace_photons<true>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+                       pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
+  else
+    hipLaunchKernelGGL(k_trace_photons<false>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+                       pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, pb.count, pb.offset, n_waves, &st->n_stored,
@@ -724,6 +737,120 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 }
 
 // ===========================================================================
+// Scene BVH, built on the device once per scene.  The reference tests every
+// object for every ray (calc_intersection, tracer.rs:306-335); here the
+// bounded shapes (spheres, polygons, parallelograms) sit in a heap-ordered
+// complete binary tree over leaves of BVH_LEAF shapes in Morton order of
+// their centroids, the same implicit layout as the photon map: node i has
+// children 2i+1 and 2i+2 and needs no pointers.  Unbounded shapes (planes)
+// stay in the linear list.
+
+// Box of a bounded shape, padded by 1e-5 of its coordinate magnitude so that
+// the float slab test never rejects a ray that the shape's own float test
+// reports as a hit (flat boxes of axis-aligned quads included).
+__device__ __forceinline__ Node prim_box(const DevShape& s) {
+  V3 lo, hi;
+  const V3 p = mk(s.p);
+  if (s.kind == SHAPE_SPHERE) {
+    const float r = fabsf(s.radius);
+    lo = p - mk(r, r, r);
+    hi = p + mk(r, r, r);
+  } else {
+    const V3 a = p + mk(s.d1), b = p + mk(s.d2);
+    lo = mk(fminf(p.x, fminf(a.x, b.x)), fminf(p.y, fminf(a.y, b.y)), fminf(p.z, fminf(a.z, b.z)));
+    hi = mk(fmaxf(p.x, fmaxf(a.x, b.x)), fmaxf(p.y, fmaxf(a.y, b.y)), fmaxf(p.z, fmaxf(a.z, b.z)));
+    if (s.kind == SHAPE_PARALLELOGRAM) {
+      const V3 c = a + mk(s.d2);
+      lo = mk(fminf(lo.x, c.x), fminf(lo.y, c.y), fminf(lo.z, c.z));
+      hi = mk(fmaxf(hi.x, c.x), fmaxf(hi.y, c.y), fmaxf(hi.z, c.z));
+    }
+  }
+  const float m = fmaxf(fmaxf(fmaxf(fabsf(lo.x), fabsf(lo.y)), fmaxf(fabsf(lo.z), fabsf(hi.x))),
+                        fmaxf(fabsf(hi.y), fabsf(hi.z)));
+  const float pad = 1e-5f * (1.0f + m);
+  Node b;
+  b.lo = make_float4(lo.x - pad, lo.y - pad, lo.z - pad, 0.0f);
+  b.hi = make_float4(hi.x + pad, hi.y + pad, hi.z + pad, 0.0f);
+  return b;
+}
+
+__device__ __forceinline__ Node empty_node() {
+  Node b;
+  b.lo = make_float4(BVH_EMPTY, BVH_EMPTY, BVH_EMPTY, 0.0f);
+  b.hi = b.lo;
+  return b;
+}
+
+// Per bounded shape i (scene index ids[i]): its padded box and the 30-bit
+// Morton key of its centroid in the centroid bounds.
+__global__ __launch_bounds__(256) void k_bvh_keys(const DevShape* shapes, const uint32_t* ids, uint32_t n,
+                                                  float4 cmin, float4 cscale, uint32_t* keys, uint32_t* vals,
+                                                  Node* pbox) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Node b = prim_box(shapes[ids[i]]);
+  pbox[i] = b;
+  const float cx = 0.5f * (b.lo.x + b.hi.x), cy = 0.5f * (b.lo.y + b.hi.y), cz = 0.5f * (b.lo.z + b.hi.z);
+  const uint32_t qx = (uint32_t)fminf(fmaxf((cx - cmin.x) * cscale.x, 0.0f), 1023.0f);
+  const uint32_t qy = (uint32_t)fminf(fmaxf((cy - cmin.y) * cscale.y, 0.0f), 1023.0f);
+  const uint32_t qz = (uint32_t)fminf(fmaxf((cz - cmin.z) * cscale.z, 0.0f), 1023.0f);
+  keys[i] = (expand_bits10(qx) << 2) | (expand_bits10(qy) << 1) | expand_bits10(qz);
+  vals[i] = i;
+}
+
+// One thread per leaf slot: the leaf's shapes in sorted order into bprims,
+// the union of their boxes into the leaf node (slots past the shapes get the
+// empty node).
+__global__ __launch_bounds__(256) void k_bvh_leaves(const DevShape* shapes, const uint32_t* ids,
+                                                    const uint32_t* order, const Node* pbox, uint32_t n,
+                                                    uint32_t depth, DevShape* bprims, Node* nodes) {
+  const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
+  if (leaf >= (1u << depth)) return;
+  const uint32_t base = leaf * BVH_LEAF;
+  Node u = empty_node();
+  bool any = false;
+  for (uint32_t j = 0; j < BVH_LEAF; ++j) {
+    const uint32_t k = base + j;
+    if (k >= n) break;
+    const uint32_t pi = order[k];
+    const Node b = pbox[pi];
+    u = any ? node_union(u, b) : b;
+    any = true;
+    bprims[k] = shapes[ids[pi]];
+  }
+  nodes[(1u << depth) - 1u + leaf] = u;
+}
+
+// Nodes of one level: the union of the two children, skipping empty ones.
+__global__ __launch_bounds__(256) void k_bvh_level(Node* nodes, uint32_t level) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (1u << level)) return;
+  const uint32_t i = (1u << level) - 1u + k;
+  const Node a = nodes[2u * i + 1u], b = nodes[2u * i + 2u];
+  const bool ea = a.lo.x == BVH_EMPTY, eb = b.lo.x == BVH_EMPTY;
+  nodes[i] = ea ? b : (eb ? a : node_union(a, b));
+}
+
+hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_t* ids, uint32_t n, float4 cmin,
+                            float4 cscale, uint32_t depth, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np,
+                            uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,
+                            Node* nodes) {
+  hipLaunchKernelGGL(k_bvh_keys, dim3((n + 255) / 256), dim3(256), 0, s, shapes, ids, n, cmin, cscale, keys[0],
+                     vals[0], pbox);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int cur = 0;
+  e = radix_sort(s, keys, vals, np, ghist, totals, sort_blocks, 32u, &cur);
+  if (e != hipSuccess) return e;
+  const uint32_t slots = 1u << depth;
+  hipLaunchKernelGGL(k_bvh_leaves, dim3((slots + 255) / 256), dim3(256), 0, s, shapes, ids, vals[cur], pbox, n,
+                     depth, bprims, nodes);
+  for (int level = (int)depth - 1; level >= 0; --level)
+    hipLaunchKernelGGL(k_bvh_level, dim3(((1u << level) + 255) / 256), dim3(256), 0, s, nodes, (uint32_t)level);
+  return hipGetLastError();
+}
+
+// ===========================================================================
 // Eye rays (tracer.rs:129-177).  The reference recurses over a binary tree
 // (reflection child + refraction child).  The BSDF is linear in the child
 // radiances, so the tree is walked depth-first with a per-lane throughput:
@@ -738,8 +865,10 @@ struct EyeNode {
   uint32_t meta;  // depth (5 bits) | medium (11 bits) << 5 | path (16 bits) << 16
 };
 
+template <bool BVH>
 __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd) {
+  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t wave = t >> 6;  // = tile index
   const uint32_t pass = st->pass;
@@ -770,7 +899,7 @@ __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf
       bool next_set = false;
       EyeNode next;
       Isect is;
-      if ((int)depth < maxd && intersect(sc, cur.o, cur.d, is)) {
+      if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is, stk + threadIdx.x)) {
         const DevMaterial& M = sc.mats[is.mat];
         const DevMaterial& Mm = sc.mats[med];
         Rng nr;
@@ -859,7 +988,11 @@ __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth) {
   const uint32_t blocks = (eb.q_primary + 255) / 256;
-  hipLaunchKernelGGL(k_eye, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic, max_depth);
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_eye<true>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic, max_depth);
+  else
+    hipLaunchKernelGGL(k_eye<false>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
+                       max_depth);
   return hipGetLastError();
 }
 
@@ -995,10 +1128,12 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 //   res[slot] = w * (direct light (when uc) + scale * Σ filter * max(0,-n.dir))
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
 // BSDF factor and path throughput.
+template <bool BVH>
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc) {
   __shared__ LeafLds lds[4];
+  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
   const uint32_t n = st->n_stored, depth = st->depth;
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
@@ -1062,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, 
       node += 1;
     }
     C3 di = C3{ar * scale, ag * scale, ab * scale};
-    if (uc) di = di + direct_light(sc, mk(qq.a), mk(qq.b));
+    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b), stk + threadIdx.x);
     if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
 }
@@ -1136,8 +1271,12 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   if (e != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
   const SceneView v = make_view(sc);
-  hipLaunchKernelGGL(k_shade, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
-                     use_classic);
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
+                       use_classic);
+  else
+    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
+                       use_classic);
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);
@@ -1240,6 +1379,35 @@ hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const flo
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_query, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, m.sorted64, m.src_idx, st, qf, q64,
                      qn, nq, r2, refine64, mode, pfilter, counts, offsets, idx_out, rgb_out);
+  return hipGetLastError();
+}
+
+}  // namespace ppm
+
+namespace ppm {
+
+// Test hook: nearest hit of host-given rays (calc_intersection,
+// tracer.rs:306-335): t and scene index of the shape, or (inf, -1).
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_intersect(SceneView sc, const float4* o, const float4* d, uint32_t n,
+                                                   float* t_out, int32_t* obj_out) {
+  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Hit h = nearest<BVH>(sc, mk(o[i]), mk(d[i]), stk + threadIdx.x);
+  t_out[i] = h.obj >= 0 ? h.t : INF_F;
+  obj_out[i] = h.obj;
+}
+
+hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
+                            float* t_out, int32_t* obj_out) {
+  if (n == 0) return hipSuccess;
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_intersect<true>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
+                       obj_out);
+  else
+    hipLaunchKernelGGL(k_intersect<false>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
+                       obj_out);
   return hipGetLastError();
 }
 

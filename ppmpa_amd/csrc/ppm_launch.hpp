@@ -14,6 +14,10 @@ struct SceneBuf {
   DevShape* shapes = nullptr;
   DevMaterial* mats = nullptr;  // n_mats + 1 (M_AIR last)
   DevLight* lights = nullptr;
+  DevShape* lin = nullptr;      // shapes tested one by one (== shapes when there is no BVH)
+  Node* bvh = nullptr;          // 2^(bvh_depth+1) - 1 nodes, or null
+  DevShape* bprims = nullptr;   // n_bprims shapes in leaf order
+  uint32_t n_lin = 0, n_bprims = 0, bvh_depth = 0;
   uint32_t n_shapes = 0, n_mats = 0, n_lights = 0;
   uint32_t n_emit_total = 0;  // photons per pass over all lights
   DevCamera cam{};
@@ -107,5 +111,17 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
 hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const float4* qf, const double* q64,
                         const float4* qn, uint32_t nq, double r2, int refine64, int mode, int pfilter,
                         uint32_t* counts, const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out, int n_cu);
+
+// Scene BVH over the n bounded shapes shapes[ids[i]]: Morton keys of the
+// centroids (cmin, cscale = 1023 / extent), a radix sort (count *np on the
+// device), leaves of BVH_LEAF shapes copied into bprims, levels folded up.
+hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_t* ids, uint32_t n, float4 cmin,
+                            float4 cscale, uint32_t depth, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np,
+                            uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,
+                            Node* nodes);
+
+// Test hook: nearest hit of n rays (unit directions).
+hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
+                            float* t_out, int32_t* obj_out);
 
 }  // namespace ppm

@@ -46,13 +46,23 @@ constexpr int MAX_DEPTH_LIMIT = 16;  // deepest MAX_TRACE the kernels accept
 #endif
 constexpr int LEAF = PPM_LEAF;       // photons per map leaf (<= one wavefront)
 
+// Scene BVH: a heap-ordered complete binary tree over leaves of BVH_LEAF
+// primitives in Morton order of their centroids.  Traversal keeps one
+// BVH_STACK-entry stack per lane in LDS; a descent pushes at most one node
+// per level, so BVH_STACK >= BVH_MAX_DEPTH suffices.
+constexpr uint32_t BVH_LEAF = 4;
+constexpr uint32_t BVH_MAX_DEPTH = 22;   // up to 4M leaves (16M primitives)
+constexpr uint32_t BVH_STACK = 24;
+constexpr uint32_t BVH_MIN_PRIMS = 64;   // automatic mode: fewer bounded shapes are tested one by one
+constexpr float BVH_EMPTY = 3.0e38f;     // an empty node is the point box (BVH_EMPTY,)^3
+
 // --- scene records (80 B) -------------------------------------------------
 struct alignas(16) DevShape {
   uint32_t kind;
   uint32_t mat;
   float dist;    // Plain
   float radius;  // Sphere
-  float4 p;      // position / centre / p0
+  float4 p;      // position / centre / p0; w = the shape's index in the scene (bits)
   float4 n;      // unit normal
   float4 d1;
   float4 d2;

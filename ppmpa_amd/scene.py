@@ -205,6 +205,19 @@ def read_camera() -> Camera:
                   f_number=4.0, progressive=True, antialias=True, blur=True)
 
 
+# numpy mirror of ppm_shape_desc (same field offsets, 120 bytes)
+SHAPE_DTYPE = np.dtype([("kind", "<i4"), ("material", "<i4"), ("pos", "<f8", 3), ("nvec", "<f8", 3),
+                        ("dir1", "<f8", 3), ("dir2", "<f8", 3), ("dist", "<f8"), ("radius", "<f8")], align=True)
+
+
+@dataclass
+class Mesh:
+    """Triangles (n x 3 x 3 vertex coordinates) of one material: each one a
+    Shape::Polygon (geometry.rs:93-103) with the unit normal (v1-v0)x(v2-v0)."""
+    tris: np.ndarray
+    material: int
+
+
 @dataclass
 class Scene:
     materials: List[Material] = field(default_factory=list)
@@ -213,19 +226,43 @@ class Scene:
     camera: Camera = None
     bounds_min: Vec = (0.0, 0.0, 0.0)
     bounds_max: Vec = (1.0, 1.0, 1.0)
+    meshes: List[Mesh] = field(default_factory=list)  # their triangles follow `shapes` in scene order
+    bvh: int = A.SCENE_BVH_AUTO
 
     def add_material(self, m: Material) -> int:
         self.materials.append(m)
         return len(self.materials) - 1
 
+    @property
+    def n_shapes(self) -> int:
+        return len(self.shapes) + sum(len(m.tris) for m in self.meshes)
+
+    def shape_array(self) -> np.ndarray:
+        """All shapes in scene order as a SHAPE_DTYPE array."""
+        out = np.zeros(self.n_shapes, SHAPE_DTYPE)
+        for i, s in enumerate(self.shapes):
+            out[i] = (int(s.kind), int(s.material), s.pos, s.nvec, s.dir1, s.dir2, float(s.dist), float(s.radius))
+        k = len(self.shapes)
+        for m in self.meshes:
+            t = np.asarray(m.tris, np.float64).reshape(-1, 3, 3)
+            d1, d2 = t[:, 1] - t[:, 0], t[:, 2] - t[:, 0]
+            n = np.cross(d1, d2)
+            ln = np.linalg.norm(n, axis=1)
+            if np.any(ln == 0.0):
+                raise ValueError("degenerate triangle in mesh")
+            sl = out[k:k + len(t)]
+            sl["kind"], sl["material"] = A.SHAPE_POLYGON, m.material
+            sl["pos"], sl["dir1"], sl["dir2"], sl["nvec"] = t[:, 0], d1, d2, n / ln[:, None]
+            k += len(t)
+        return out
+
     def desc(self):
         """ppm_scene_desc; the returned tuple keeps the arrays alive."""
-        shapes = (A.ShapeDesc * max(1, len(self.shapes)))()
-        for i, s in enumerate(self.shapes):
-            d = shapes[i]
-            d.kind, d.material = int(s.kind), int(s.material)
-            d.pos, d.nvec, d.dir1, d.dir2 = A.D3(*s.pos), A.D3(*s.nvec), A.D3(*s.dir1), A.D3(*s.dir2)
-            d.dist, d.radius = float(s.dist), float(s.radius)
+        arr = self.shape_array()
+        if len(arr) == 0:
+            arr = np.zeros(1, SHAPE_DTYPE)
+        assert arr.dtype.itemsize == C.sizeof(A.ShapeDesc)
+        shapes = arr
         mats = (A.MaterialDesc * max(1, len(self.materials)))()
         for i, m in enumerate(self.materials):
             d = mats[i]
@@ -241,8 +278,9 @@ class Scene:
             d.pos, d.nvec, d.dir1, d.dir2, d.dir = (A.D3(*l.pos), A.D3(*l.nvec), A.D3(*l.dir1), A.D3(*l.dir2),
                                                     A.D3(*l.dir))
         sd = A.SceneDesc()
-        sd.n_shapes, sd.n_materials, sd.n_lights = len(self.shapes), len(self.materials), len(self.lights)
-        sd.shapes = C.cast(shapes, C.POINTER(A.ShapeDesc))
+        sd.n_shapes, sd.n_materials, sd.n_lights = self.n_shapes, len(self.materials), len(self.lights)
+        sd.flags = int(self.bvh)
+        sd.shapes = shapes.ctypes.data_as(C.POINTER(A.ShapeDesc))
         sd.materials = C.cast(mats, C.POINTER(A.MaterialDesc))
         sd.lights = C.cast(lights, C.POINTER(A.LightDesc))
         sd.camera = self.camera.desc()
@@ -340,6 +378,68 @@ def caustic_scene(xreso=1024, yreso=768, walls="quad") -> Scene:
     return sc
 
 
+def icosphere(subdiv: int, center=(0.0, 0.0, 0.0), radius=1.0, noise=0.0, seed=1):
+    """Triangles of an icosahedron subdivided `subdiv` times (20 * 4^subdiv
+    faces, outward winding) projected onto a sphere; each vertex is moved
+    radially by a factor 1 + U(-noise, +noise) drawn from a Philox stream
+    keyed by `seed`.  Returns (vertices (V x 3), faces (F x 3 int))."""
+    g = (1.0 + 5.0 ** 0.5) / 2.0
+    v = np.array([(-1, g, 0), (1, g, 0), (-1, -g, 0), (1, -g, 0), (0, -1, g), (0, 1, g), (0, -1, -g), (0, 1, -g),
+                  (g, 0, -1), (g, 0, 1), (-g, 0, -1), (-g, 0, 1)], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+                  (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+                  (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], np.int64)
+    for _ in range(subdiv):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        es = np.sort(e, axis=1)
+        uniq, inv = np.unique(es, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = len(v)
+        v = np.concatenate([v, mid])
+        nf = len(f)
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = base + inv[:nf], base + inv[nf:2 * nf], base + inv[2 * nf:]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    if noise:
+        rng = np.random.Generator(np.random.Philox(key=int(seed)))
+        v = v * (1.0 + rng.uniform(-noise, noise, (len(v), 1)))
+    return np.asarray(center, np.float64) + radius * v, f
+
+
+def mesh_scene(xreso=1920, yreso=1080, subdiv=7, n_spheres=64, seed=1, walls="quad") -> Scene:
+    """The Cornell box of two_spheres_scene holding a displaced icosphere
+    (subdivision `subdiv`: 327,680 triangles at 7; radius 0.22 m about
+    (0.5, 0.3, 0.6), radial noise +-2%, diffuse albedo 0.75) and `n_spheres`
+    spheres of radius U(0.02, 0.08) at seeded non-overlapping places inside
+    the box, each diffuse (0.75), mirror or glass (IOR 1.5) with equal
+    probability.  Camera as in the other box scenes, 1920x1080."""
+    sc = cornell_box(Scene(), walls=walls)
+    dm = sc.add_material(lambertian(0.75))
+    mm = sc.add_material(mirror())
+    gm = sc.add_material(glass(1.5))
+    ctr, rad = np.array([0.5, 0.3, 0.6]), 0.22
+    vert, face = icosphere(subdiv, ctr, rad, noise=0.02, seed=seed)
+    rng = np.random.Generator(np.random.Philox(key=int(seed) + 1))
+    placed = []
+    while len(placed) < n_spheres:
+        r = rng.uniform(0.02, 0.08)
+        c = rng.uniform(r + 0.01, 1.0 - r - 0.01, 3)
+        if np.linalg.norm(c - ctr) < rad * 1.02 + r + 0.005:
+            continue
+        if any(np.linalg.norm(c - q) < r + s + 0.005 for q, s, _ in placed):
+            continue
+        placed.append((c, r, (dm, mm, gm)[int(rng.integers(0, 3))]))
+    for c, r, m in placed:
+        sc.shapes.append(sphere(c, r, m))
+    sc.meshes.append(Mesh(vert[face], dm))
+    sc.camera = box_camera(xreso, yreso)
+    return sc
+
+
 def reference_room() -> Scene:
     """A room in the style of the reference's built-in scene (scene.rs:20-447):
     4 x 4 x 11 m walls of Plain shapes, a 1.33 m ceiling parallelogram light of
@@ -375,4 +475,6 @@ CONFIGS = {
                     seed=1),
     "caustic": dict(scene=caustic_scene, n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0,
                     max_depth=16, seed=1),
+    "mesh": dict(scene=mesh_scene, n_photon=8_000_000, passes=256, radius0=0.005, alpha=2.0 / 3.0, max_depth=16,
+                 seed=1),
 }

@@ -78,6 +78,24 @@ class DeviceScene:
     def pixels(self):
         return self.scene.camera.xreso * self.scene.camera.yreso
 
+    def stats(self):
+        """(shapes tested one by one, shapes in the BVH, BVH leaf level)."""
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.lib.ppm_scene_stats(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def intersect(self, origins, dirs):
+        """calc_intersection (tracer.rs:306-335) for n rays: (t, shape index),
+        (inf, -1) where nothing is hit."""
+        o = np.ascontiguousarray(origins, np.float64).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+        n = len(o)
+        t = np.empty(n, np.float64)
+        obj = np.empty(n, np.int32)
+        self.ctx.check(self.ctx.lib.ppm_intersect(self.ctx.h, self.h, _dp(o), _dp(d), n, _dp(t),
+                                                  obj.ctypes.data_as(C.POINTER(C.c_int32))))
+        return t, obj
+
     def close(self):
         if self.h:
             self.ctx.lib.ppm_scene_destroy(self.h)
