@@ -1,11 +1,14 @@
 """Benchmark of the PPM-PA hot path on one MI355X.
 
-Workload "caustic": a Cornell box of five 1 x 1 m quad walls with a diffuse, a
-mirror and a glass sphere, 1024x768 pixels, 2M photons per pass, max 16 bounces,
-r_1 = 0.005 m, alpha = 2/3, seed 1.  One step = one PPM-PA pass (photon trace
-+ map build + eye rays + gather + FP64 accumulation) on device-resident data.
+Default workload "caustic": a Cornell box of five 1 x 1 m quad walls with a
+diffuse, a mirror and a glass sphere, 1024x768 pixels, 2M photons per pass,
+max 16 bounces, r_1 = 0.005 m, alpha = 2/3, seed 1.  One step = one PPM-PA pass
+(photon trace + map build + eye rays + gather + FP64 accumulation) on
+device-resident data.  Other workloads: "mesh" (the box with a 327,680-triangle
+icosphere and 64 spheres, 1920x1080, 8M photons per pass, through the scene
+BVH) and "two_spheres" (256x256, 200k photons per pass).
 
-  python bench.py [--gpus 1] [--steps 512] [--warmup 8]
+  python bench.py [--gpus 1] [--steps K] [--warmup 8] [--workload caustic]
 
 Prints one JSON line.  value = photons emitted and gathered per second over
 the timed passes (Mphotons/s).
@@ -21,13 +24,33 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
+DESCRIPTION = {
+    "caustic": "caustic: Cornell box (five 1x1 m quad walls) + diffuse/mirror/glass spheres, 1024x768, "
+               "2M photons/pass, max 16 bounces, r1=0.005 m, alpha=2/3, seed 1",
+    "mesh": "mesh: Cornell box + icosphere of 327,680 triangles (radial noise 2%) + 64 spheres "
+            "(diffuse/mirror/glass), 1920x1080, 8M photons/pass, max 16 bounces, r1=0.005 m, alpha=2/3, seed 1, "
+            "scene BVH",
+    "two_spheres": "two_spheres: Cornell box + diffuse and mirror spheres, 256x256, 200k photons/pass, "
+                   "max 8 bounces, r1=0.01 m, alpha=2/3, seed 1",
+}
+
+# stage -> (kernel, algorithmic bytes per pass from S stored photons, Q
+# queries and P pixels)
+STAGE_KERNEL = {
+    "ms_photon_trace": ("k_trace_photons", lambda S, Q, P: 32.0 * S),
+    "ms_eye_trace": ("k_eye", lambda S, Q, P: 48.0 * Q + 16.0 * P),
+    "ms_gather": ("k_shade", lambda S, Q, P: 28.0 * S + 68.0 * Q),
+}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=None, help="timed passes (default: the workload's schedule)")
     ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--workload", choices=sorted(DESCRIPTION), default="caustic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-3sigma", action="store_true", help="skip the time-to-3-sigma comparison")
     ap.add_argument("--cpu-sample-passes", type=int, default=8)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed region, fill one buffer of this many MiB and copy it device-to-device "
@@ -40,88 +63,56 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
     import ppmpa_amd as P
-    from ppmpa_amd import _abi as A
 
-    cfgd = P.CONFIGS["caustic"]
+    cfgd = P.CONFIGS[args.workload]
+    steps = args.steps if args.steps is not None else cfgd["passes"]
     sc = cfgd["scene"]()
     ctx = P.Context(0)
     ds = P.DeviceScene(ctx, sc)
-
-    def cfg(flags=0, first=1):
-        return P.pass_cfg(seed=cfgd["seed"], n_photon=cfgd["n_photon"], radius0=cfgd["radius0"],
-                          alpha=cfgd["alpha"], max_depth=cfgd["max_depth"], first_pass=first, flags=flags)
-
-    # warmup: a separate session (also builds the pass graph path once)
-    if args.warmup > 0:
-        w = P.Session(ctx, ds, cfg())
-        w.run(args.warmup)
-        w.close()
-
-    # timed: passes 1..K of the schedule, hipGraph replay, device-resident
-    s = P.Session(ctx, ds, cfg())
-    ctx.sync()
-    t0 = time.perf_counter()
-    s.run(args.steps)
-    ctx.sync()
-    wall = time.perf_counter() - t0
-    t = s.timings()
-    mean, var, npass = s.image()
-    s.close()
-    dev_s = t["ms_total"] / 1e3
-    T = max(wall, dev_s)
-    n_photon = cfgd["n_photon"]
+    r = run_workload(P, ctx, ds, cfgd, steps, args.warmup)
     pix = sc.camera.xreso * sc.camera.yreso
-    value = n_photon * args.steps / T / 1e6
-    stored_per_pass = t["photons_stored"] / max(1, args.steps)
+    T = r["T"]
+    value = cfgd["n_photon"] * steps / T / 1e6
+    ms = r["ms"]
+    S, Q = r["stored_per_pass"], r["queries_per_pass"]
 
-    # per-stage device time over a few passes spread over the schedule
-    stage = P.Session(ctx, ds, cfg(flags=A.SESSION_STAGE_TIMING))
-    stage.run(4)
-    st = stage.timings()
-    stage.close()
-    ms = {k: st[k] / 4 for k in ("ms_photon_trace", "ms_map_build", "ms_eye_trace", "ms_gather", "ms_accumulate")}
-
-    # roofline of the shade kernel (k_shade: gather + direct light, timed
-    # alone between HIP events on the context stream): algorithmic bytes per pass = the
-    # photon map read once (28 B/photon) + per query its record (48 B), its
-    # sorted index (4 B) and its result (16 B).
-    S = stored_per_pass
-    Q = t["gather_queries"] / max(1, args.steps)
-    gather_bytes = 28.0 * S + 68.0 * Q
-    achieved = gather_bytes / (ms["ms_gather"] / 1e3) / 1e9
+    # roofline of the dominant kernel, timed alone between HIP events on the
+    # context stream; its algorithmic bytes per pass are those of STAGE_KERNEL
+    # (k_shade: the photon map read once, 28 B per stored photon, plus per
+    # query its record 48 B, sorted index 4 B and result 16 B).
+    stage = max(STAGE_KERNEL, key=lambda k: ms[k])
+    kern, fbytes = STAGE_KERNEL[stage]
+    nbytes = fbytes(S, Q, pix)
+    achieved = nbytes / (ms[stage] / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_shade", "kernel_ms": round(ms["ms_gather"], 4),
-            "algorithmic_bytes": round(gather_bytes), "queries_per_pass": round(Q)}
-    traffic = profiled_traffic("k_shade")
+            "kernel": kern, "kernel_ms": round(ms[stage], 4), "algorithmic_bytes": round(nbytes),
+            "queries_per_pass": round(Q)}
+    traffic, src = profiled_traffic(kern, args.workload)
     if traffic is not None:
         roof["traffic"] = traffic
-        roof["traffic_source"] = "profiles/pmc_traffic_caustic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled)"
+        roof["traffic_source"] = src + " (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy probe)"
 
     out = {
         "metric": "Mphotons/s traced+gathered",
         "value": round(value, 3),
         "unit": "Mphotons/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(T * 1e3 / args.steps, 4),
+        "ms_per_step": round(T * 1e3 / steps, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "caustic: Cornell box (five 1x1 m quad walls) + diffuse/mirror/glass spheres, 1024x768, 2M photons/pass, "
-                               "max 16 bounces, r1=0.005 m, alpha=2/3, seed 1",
-                   "photons_per_pass": n_photon, "pixels": pix, "passes": args.steps,
-                   "accumulation": "f64"},
-        "gathered_px_per_s": round(pix * args.steps / T, 1),
-        "stored_photons_per_pass": round(stored_per_pass, 1),
+        "config": {"workload": DESCRIPTION[args.workload], "photons_per_pass": cfgd["n_photon"], "pixels": pix,
+                   "passes": steps, "accumulation": "f64"},
+        "gathered_px_per_s": round(pix * steps / T, 1),
+        "stored_photons_per_pass": round(S, 1),
         "stage_ms_per_pass": {k[3:]: round(v, 4) for k, v in ms.items()},
-        "image_mean": [round(float(x), 6) for x in mean.reshape(-1, 3).mean(0)],
+        "image_mean": r["image_mean"],
         "roofline": roof,
     }
 
@@ -129,9 +120,44 @@ def main():
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
 
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.cpu_sample_passes)
+        out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.workload, args.cpu_sample_passes)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    if args.workload == "caustic" and not args.no_3sigma and not args.no_cpu_baseline:
+        out["time_to_3sigma"] = time_to_3sigma(P, ctx)
     print(json.dumps(out), flush=True)
+
+
+def run_workload(P, ctx, ds, cfgd, steps, warmup):
+    """Warmup session, then `steps` passes timed on the device (hipGraph
+    replay) and by the wall clock, then 4 passes with per-stage HIP events."""
+    from ppmpa_amd import _abi as A
+
+    def cfg(flags=0):
+        return P.pass_cfg(seed=cfgd["seed"], n_photon=cfgd["n_photon"], radius0=cfgd["radius0"],
+                          alpha=cfgd["alpha"], max_depth=cfgd["max_depth"], first_pass=1, flags=flags)
+
+    if warmup > 0:
+        w = P.Session(ctx, ds, cfg())
+        w.run(warmup)
+        w.close()
+    s = P.Session(ctx, ds, cfg())
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.run(steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    t = s.timings()
+    mean, _, _ = s.image(False)
+    s.close()
+    T = max(wall, t["ms_total"] / 1e3)
+    stage = P.Session(ctx, ds, cfg(flags=A.SESSION_STAGE_TIMING))
+    stage.run(4)
+    st = stage.timings()
+    stage.close()
+    ms = {k: st[k] / 4 for k in ("ms_photon_trace", "ms_map_build", "ms_eye_trace", "ms_gather", "ms_accumulate")}
+    return {"T": T, "ms": ms, "stored_per_pass": t["photons_stored"] / max(1, steps),
+            "queries_per_pass": t["gather_queries"] / max(1, steps),
+            "image_mean": [round(float(x), 6) for x in mean.reshape(-1, 3).mean(0)]}
 
 
 def copy_probe(nbytes):
@@ -157,13 +183,14 @@ def copy_probe(nbytes):
     return nbytes
 
 
-def profiled_traffic(kernel):
+def profiled_traffic(kernel, workload):
     """HBM bytes per dispatch of `kernel` from the committed counter summary
-    (two rocprofv3 --pmc runs of this same workload), scaled by the factors
-    recorded there; None when absent."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_caustic.md")
+    of this workload (two rocprofv3 --pmc runs), scaled by the factors
+    recorded there; (None, None) when absent."""
+    rel = os.path.join("profiles", f"pmc_traffic_{workload}.md")
+    path = os.path.join(ROOT, rel)
     if not os.path.exists(path):
-        return None
+        return None, None
     sf, sw = 1.0, 1.0
     for line in open(path):
         if line.startswith("scale_fetch:"):
@@ -172,27 +199,119 @@ def profiled_traffic(kernel):
             sw = float(line.split(":")[1])
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) >= 5 and cells[0] == kernel:  # kernel | dispatches | FETCH KiB | WRITE KiB | MB
-            return round((float(cells[2]) * sf + float(cells[3]) * sw) * 1024)
-    return None
+            return round((float(cells[2]) * sf + float(cells[3]) * sw) * 1024), rel
+    return None, None
 
 
-def cpu_baseline(sc, cfgd, n_sample):
-    """The FP64 oracle (a restatement of the reference's CPU path) on this
-    host's cores, on a bounded sample: passes 1..n_sample (the widest radii,
-    the most gather work) of the same workload."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    return O
+
+
+def _threads():
+    return max(1, min(len(os.sched_getaffinity(0)), 16))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, cfgd, workload, n_sample):
+    """The FP64 oracle (a restatement of the reference's CPU path, which
+    tests every shape for every ray) on this host's cores, on a bounded
+    sample of the same workload."""
+    O = _oracle()
     import ppmpa_amd as P
-    threads = max(1, min(len(os.sched_getaffinity(0)), 16))
+    threads = _threads()
     osc = O.OScene(sc)
     c = P.pass_cfg(seed=cfgd["seed"], n_photon=cfgd["n_photon"], radius0=cfgd["radius0"], alpha=cfgd["alpha"],
                    max_depth=cfgd["max_depth"], first_pass=1)
+    if workload == "mesh":
+        # one pass over 2M pixels with 25 shadow rays per hit against 327,750
+        # shapes is out of reach; the sample is photon tracing alone
+        n = 400
+        t0 = time.perf_counter()
+        osc.trace_photons(cfgd["seed"], 1, n, True, cfgd["max_depth"])
+        T = time.perf_counter() - t0
+        return {"value": round(n / T / 1e6, 6), "unit": "Mphotons/s", "cores": threads, "kind": "port",
+                "sample": f"{n} photons of pass 1 traced (no map, no eye pass), FP64 oracle testing all "
+                          f"{sc.n_shapes} shapes per ray, {T:.1f} s; CPU {cpu_model()}"}
+    n_sample = min(n_sample, cfgd["passes"])
     t0 = time.perf_counter()
-    _, _, stored, ms_trace, ms_eye = osc.run_passes(c, n_sample, threads=threads)
+    _, _, _, ms_trace, ms_eye = osc.run_passes(c, n_sample, threads=threads)
     T = time.perf_counter() - t0
     return {"value": round(cfgd["n_photon"] * n_sample / T / 1e6, 4), "unit": "Mphotons/s", "cores": threads,
-            "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle, "
-                                      f"{T:.1f} s; trace+kd {ms_trace / 1e3:.1f} s, eye+gather {ms_eye / 1e3:.1f} s"}
+            "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle, {T:.1f} s; "
+                                      f"trace+kd {ms_trace / 1e3:.1f} s, eye+gather {ms_eye / 1e3:.1f} s; "
+                                      f"CPU {cpu_model()}"}
+
+
+def _blocks(img, b=4):
+    h, w, _ = img.shape
+    return img[:h - h % b, :w - w % b].reshape(h // b, b, w // b, b, 3).mean(axis=(1, 3))
+
+
+def time_to_3sigma(P, ctx):
+    """Equal error at equal pass count: the two_spheres workload at full size
+    (256x256, 200k photons/pass, 16 passes) on the GPU and in the FP64
+    oracle, on independent random streams (seeds 1 and 1001).  Their images
+    agree within the 3-sigma band of the per-pass spread (z on 4x4-pixel
+    block means, and per pixel), so both reach the same error after the same
+    passes; the times to get there are compared."""
+    import numpy as np
+    O = _oracle()
+    cfgd = P.CONFIGS["two_spheres"]
+    K = cfgd["passes"]
+    sc = cfgd["scene"]()
+    ds = P.DeviceScene(ctx, sc)
+
+    def cfg(seed, first=1):
+        return P.pass_cfg(seed=seed, n_photon=cfgd["n_photon"], radius0=cfgd["radius0"], alpha=cfgd["alpha"],
+                          max_depth=cfgd["max_depth"], first_pass=first)
+
+    s = P.Session(ctx, ds, cfg(1))
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.run(K)
+    ctx.sync()
+    gpu_s = time.perf_counter() - t0
+    s.close()
+    # per-pass GPU images (untimed) for the band
+    s = P.Session(ctx, ds, cfg(1))
+    imgs_g, prev = [], 0.0
+    for _ in range(K):
+        s.run(1)
+        m, _, n = s.image(False)
+        imgs_g.append(m * n - prev)
+        prev = m * n
+    s.close()
+    osc = O.OScene(sc)
+    threads = _threads()
+    imgs_o = []
+    t0 = time.perf_counter()
+    for k in range(K):
+        m1, _, _, _, _ = osc.run_passes(cfg(1001, k + 1), 1, threads=threads)
+        imgs_o.append(m1)
+    cpu_s = time.perf_counter() - t0
+    G = np.stack([_blocks(x) for x in imgs_g])
+    Ob = np.stack([_blocks(x) for x in imgs_o])
+    se = np.sqrt(G.var(0, ddof=1) / K + Ob.var(0, ddof=1) / K)
+    ok = se > 0
+    fb = float((np.abs(G.mean(0) - Ob.mean(0))[ok] / se[ok] <= 3).mean())
+    pg, po = np.stack(imgs_g), np.stack(imgs_o)
+    sp = np.sqrt(pg.var(0, ddof=1) / K + po.var(0, ddof=1) / K)
+    okp = sp > 0
+    fp = float((np.abs(pg.mean(0) - po.mean(0))[okp] / sp[okp] <= 3).mean())
+    return {"workload": DESCRIPTION["two_spheres"], "passes": K, "frac_blocks_within_3sigma": round(fb, 4),
+            "frac_pixels_within_3sigma": round(fp, 4), "gpu_s": round(gpu_s, 4), "cpu_s": round(cpu_s, 3),
+            "cpu_cores": threads, "speedup": round(cpu_s / gpu_s, 1)}
 
 
 if __name__ == "__main__":

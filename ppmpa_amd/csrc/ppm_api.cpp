@@ -357,16 +357,36 @@ namespace {
 // Shapes tested one by one and, when the scene asks for one (or has at least
 // BVH_MIN_PRIMS bounded shapes in automatic mode), the device-built BVH over
 // the bounded ones.  shapes[] is already on the device as sc->buf.shapes.
+// Box of a bounded shape (sphere, polygon, parallelogram) in double.
+void shape_box(const DevShape& s, double lo[3], double hi[3]) {
+  const double p[3] = {s.p.x, s.p.y, s.p.z};
+  if (s.kind == SHAPE_SPHERE) {
+    const double r = std::fabs((double)s.radius);
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = p[a] - r;
+      hi[a] = p[a] + r;
+    }
+    return;
+  }
+  const double d1[3] = {s.d1.x, s.d1.y, s.d1.z}, d2[3] = {s.d2.x, s.d2.y, s.d2.z};
+  const double q = s.kind == SHAPE_PARALLELOGRAM ? 1.0 : 0.0;  // the fourth corner p + d1 + d2
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = std::min({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
+    hi[a] = std::max({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
+  }
+}
+
+// Shapes tested one by one and, when the scene asks for one (or has at least
+// BVH_MIN_PRIMS bounded shapes in automatic mode), the device-built BVH over
+// the bounded shapes whose box sides are all within a quarter of the scene's
+// extent; planes and larger shapes (walls) are tested one by one.
+// shapes[] is already on the device as sc->buf.shapes.
 ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
   SceneBuf& b = sc->buf;
   std::vector<uint32_t> bounded;
-  std::vector<DevShape> lin;
   for (size_t i = 0; i < shapes.size(); ++i) {
     const uint32_t k = shapes[i].kind;
-    if (k == SHAPE_SPHERE || k == SHAPE_POLYGON || k == SHAPE_PARALLELOGRAM)
-      bounded.push_back((uint32_t)i);
-    else
-      lin.push_back(shapes[i]);
+    if (k == SHAPE_SPHERE || k == SHAPE_POLYGON || k == SHAPE_PARALLELOGRAM) bounded.push_back((uint32_t)i);
   }
   const bool use_bvh = flags == PPM_SCENE_BVH_ON || (flags == PPM_SCENE_BVH_AUTO && bounded.size() >= BVH_MIN_PRIMS);
   if (!use_bvh || bounded.empty()) {
@@ -374,33 +394,51 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
     b.n_lin = b.n_shapes;
     return PPM_OK;
   }
-  const uint64_t n = bounded.size();
-  const uint64_t leaves = (n + BVH_LEAF - 1) / BVH_LEAF;
-  uint32_t depth = 0;
-  while ((1ull << depth) < leaves) ++depth;
-  if (depth > BVH_MAX_DEPTH) return fail(c, PPM_E_RANGE, "too many shapes for the BVH (at most 16M)");
-  // centroid bounds (host, from the float records the device will see)
-  double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (uint32_t i : bounded) {
-    const DevShape& s = shapes[i];
-    const double p[3] = {s.p.x, s.p.y, s.p.z};
-    double lo[3], hi[3];
-    if (s.kind == SHAPE_SPHERE) {
-      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = p[a];
-    } else {
-      const double d1[3] = {s.d1.x, s.d1.y, s.d1.z}, d2[3] = {s.d2.x, s.d2.y, s.d2.z};
-      const double q = s.kind == SHAPE_PARALLELOGRAM ? 1.0 : 0.0;  // its fourth corner p + d1 + d2
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = std::min({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
-        hi[a] = std::max({p[a], p[a] + d1[a], p[a] + d2[a], p[a] + q * (d1[a] + d2[a])});
-      }
+  double smin[3] = {INFINITY, INFINITY, INFINITY}, smax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  std::vector<double> boxes(6 * bounded.size());
+  for (size_t j = 0; j < bounded.size(); ++j) {
+    double* lo = &boxes[6 * j];
+    double* hi = lo + 3;
+    shape_box(shapes[bounded[j]], lo, hi);
+    for (int a = 0; a < 3; ++a) {
+      smin[a] = std::min(smin[a], lo[a]);
+      smax[a] = std::max(smax[a], hi[a]);
     }
+  }
+  const double size = std::max({smax[0] - smin[0], smax[1] - smin[1], smax[2] - smin[2]});
+  std::vector<uint32_t> in_bvh;
+  std::vector<DevShape> lin;
+  double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+  std::vector<char> large(shapes.size(), 0);
+  for (size_t j = 0; j < bounded.size(); ++j) {
+    const double* lo = &boxes[6 * j];
+    const double* hi = lo + 3;
+    const double side = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
+    if (side > 0.25 * size && bounded.size() > 1) {
+      large[bounded[j]] = 1;
+      continue;
+    }
+    in_bvh.push_back(bounded[j]);
     for (int a = 0; a < 3; ++a) {
       const double cc = 0.5 * (lo[a] + hi[a]);
       cmin[a] = std::min(cmin[a], cc);
       cmax[a] = std::max(cmax[a], cc);
     }
   }
+  for (size_t i = 0; i < shapes.size(); ++i) {
+    const uint32_t k = shapes[i].kind;
+    if (large[i] || !(k == SHAPE_SPHERE || k == SHAPE_POLYGON || k == SHAPE_PARALLELOGRAM)) lin.push_back(shapes[i]);
+  }
+  if (in_bvh.empty()) {
+    b.lin = b.shapes;
+    b.n_lin = b.n_shapes;
+    return PPM_OK;
+  }
+  const uint64_t n = in_bvh.size();
+  const uint64_t leaves = (n + BVH_LEAF - 1) / BVH_LEAF;
+  uint32_t depth = 0;
+  while ((1ull << depth) < leaves) ++depth;
+  if (depth > BVH_MAX_DEPTH) return fail(c, PPM_E_RANGE, "too many shapes for the BVH (at most 16M)");
   float4 fmin = make_float4((float)cmin[0], (float)cmin[1], (float)cmin[2], 0.0f);
   float4 fscale;
   float* sc3[3] = {&fscale.x, &fscale.y, &fscale.z};
@@ -424,7 +462,7 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
       (e = dalloc(&pbox, n)) == hipSuccess && (e = dalloc(&b.bprims, n)) == hipSuccess &&
       (e = dalloc(&b.bvh, (2ull << depth) - 1)) == hipSuccess &&
       (e = dalloc(&b.lin, lin.size())) == hipSuccess &&
-      (e = hipMemcpy(ids, bounded.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = hipMemcpy(ids, in_bvh.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice)) == hipSuccess &&
       (e = hipMemcpy(np, &n32, sizeof(n32), hipMemcpyHostToDevice)) == hipSuccess &&
       (lin.empty() ||
        (e = hipMemcpy(b.lin, lin.data(), lin.size() * sizeof(DevShape), hipMemcpyHostToDevice)) == hipSuccess) &&
@@ -987,7 +1025,7 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
     hs.n_queries = 0;
     hs.overflow = 0;
     if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
-    if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth)) != hipSuccess) break;
+    if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth, c->n_cu)) != hipSuccess) break;
     if ((e = launch_gather(c->stream, sc->buf, m->m, st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=
         hipSuccess)
       break;
@@ -1100,7 +1138,7 @@ static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
   if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
   if ((e = launch_map_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
-  if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess) return e;
+  if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, s->ctx->n_cu)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
   if ((e = launch_gather(st, s->sbuf, s->map, s->st, s->eb, cfg.pfilter, cfg.use_classic, s->ctx->n_cu, s->sum,
                          s->sumsq, ev ? ev + 4 : nullptr)) != hipSuccess)

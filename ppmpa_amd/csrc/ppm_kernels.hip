@@ -11,6 +11,8 @@
 // leaf is staged once into LDS (coalesced loads) and tested by all 64 lanes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "ppm_dev.hpp"
 #include "ppm_launch.hpp"
 
@@ -739,11 +741,13 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 // ===========================================================================
 // Scene BVH, built on the device once per scene.  The reference tests every
 // object for every ray (calc_intersection, tracer.rs:306-335); here the
-// bounded shapes (spheres, polygons, parallelograms) sit in a heap-ordered
+// bounded shapes (spheres, polygons, parallelograms) smaller than a quarter
+// of the scene sit in a heap-ordered
 // complete binary tree over leaves of BVH_LEAF shapes in Morton order of
 // their centroids, the same implicit layout as the photon map: node i has
-// children 2i+1 and 2i+2 and needs no pointers.  Unbounded shapes (planes)
-// stay in the linear list.
+// children 2i+1 and 2i+2 and needs no pointers.  Planes and the large shapes
+// (walls, whose boxes would widen every node above them) stay in the linear
+// list.
 
 // Box of a bounded shape, padded by 1e-5 of its coordinate magnitude so that
 // the float slab test never rejects a ray that the shape's own float test
@@ -781,7 +785,7 @@ __device__ __forceinline__ Node empty_node() {
   return b;
 }
 
-// Per bounded shape i (scene index ids[i]): its padded box and the 30-bit
+// Per BVH shape i (scene index ids[i]): its padded box and the 30-bit
 // Morton key of its centroid in the centroid bounds.
 __global__ __launch_bounds__(256) void k_bvh_keys(const DevShape* shapes, const uint32_t* ids, uint32_t n,
                                                   float4 cmin, float4 cscale, uint32_t* keys, uint32_t* vals,
@@ -865,134 +869,176 @@ struct EyeNode {
   uint32_t meta;  // depth (5 bits) | medium (11 bits) << 5 | path (16 bits) << 16
 };
 
+// Persistent eye kernel.  Wave w owns the 8x8 tiles w, w + W, w + 2W, ...
+// (W waves; spread over the image, so that the costly pixels behind glass,
+// which cluster, are shared out) and keeps 64 trees in flight: each loop
+// step walks one node per lane, and a lane whose tree is done takes the next
+// pixel slot of the wave's tiles (ballot + mbcnt), so a deep tree holds one
+// lane, not a wave.  The pixel's queries go to its own
+// slots: the primary one to t, the k-th deeper one to q_primary*(1+k)+t,
+// further ones to the overflow region.
 template <bool BVH>
 __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
-                                             int maxd) {
+                                             int maxd, uint32_t n_waves) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t wave = t >> 6;  // = tile index
+  const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (w >= n_waves) return;
   const uint32_t pass = st->pass;
-  const uint32_t wi = t & 63u;
-  const int x = (int)((wave % (uint32_t)sc.cam.tiles_x) * 8u + (wi & 7u));
-  const int y = (int)((wave / (uint32_t)sc.cam.tiles_x) * 8u + (wi >> 3));
-  const bool valid = t < eb.q_primary && x < sc.cam.xreso && y < sc.cam.yreso;
-  if (t < eb.q_primary) eb.q[t].a.w = __uint_as_float(NO_PIXEL);
-  const uint32_t pixel = valid ? (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x : 0u;
+  // this wave's slots, numbered v = 64 k + lane-in-tile for its k-th tile
+  const uint32_t my_tiles = w < eb.n_tiles ? (eb.n_tiles - w + n_waves - 1) / n_waves : 0u;
+  const uint32_t s_end = my_tiles * 64u;
+  // draws of camera_ray on the pixel's root stream, which node 0 continues
+  const uint32_t cam_draws = (sc.cam.blur ? 2u : 0u) + (sc.cam.progressive && sc.cam.antialias ? 2u : 0u);
+  uint32_t next = 0;  // wave-uniform
+  bool alive = false, done = false;
+  uint32_t t = 0, pixel = 0, n_sec = 0;
   C3 acc{0.0f, 0.0f, 0.0f};
-  uint32_t n_sec = 0;
-  bool primary_q = false;
-  if (valid) {
-    Rng rng;
-    rng.init(seed, pixel, pass, TAG_EYE);
-    EyeNode cur;
-    camera_ray(sc.cam, x, y, rng, cur.o, cur.d);
-    const uint32_t cam_draws = rng.n;
-    cur.w = C3{1.0f, 1.0f, 1.0f};
-    cur.meta = (sc.n_mats << 5);
-    EyeNode stack[MAX_DEPTH_LIMIT];
-    int sp = 0;
-    bool have = true;
-    while (have) {
-      const uint32_t depth = cur.meta & 31u;
-      const uint32_t med = (cur.meta >> 5) & 2047u;
-      const uint32_t path = cur.meta >> 16;
-      bool next_set = false;
-      EyeNode next;
-      Isect is;
-      if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is, stk + threadIdx.x)) {
-        const DevMaterial& M = sc.mats[is.mat];
-        const DevMaterial& Mm = sc.mats[med];
-        Rng nr;
-        nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
-        if (depth == 0) nr.n = cam_draws, nr.blk = rng.blk;
-        acc = acc + cur.w * (c3(M.em) * SR_HALF_F);
-        float cos1;
-        V3 rdir = specular_reflection(is.n, cur.d, cos1);
-        const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
-        V3 tdir = mk(0.0f, 0.0f, 0.0f);
-        bool has_t = false;
-        float cos2 = 0.0f;
-        if (spec_part) {
-          // tracer.rs:150-161
-          rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
-          const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
-          const V3 hvec = normalize_or(rdir - cur.d, is.n);
-          has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+  EyeNode cur;
+  EyeNode stack[MAX_DEPTH_LIMIT];
+  int sp = 0;
+  while (true) {
+    const bool need = !alive && !done;
+    const uint64_t m = __ballot(need);
+    if (m) {
+      if (need) {
+        const uint32_t v = next + (uint32_t)__popcll(m & lanemask_lt());
+        if (v < s_end) {
+          t = (w + (v >> 6) * n_waves) * 64u + (v & 63u);
+          const uint32_t tile = t >> 6, wi = t & 63u;
+          const int x = (int)((tile % (uint32_t)sc.cam.tiles_x) * 8u + (wi & 7u));
+          const int y = (int)((tile / (uint32_t)sc.cam.tiles_x) * 8u + (wi >> 3));
+          eb.q[t].a.w = __uint_as_float(NO_PIXEL);
+          if (x < sc.cam.xreso && y < sc.cam.yreso) {
+            pixel = (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x;
+            Rng rng;
+            rng.init(seed, pixel, pass, TAG_EYE);
+            camera_ray(sc.cam, x, y, rng, cur.o, cur.d);
+            cur.w = C3{1.0f, 1.0f, 1.0f};
+            cur.meta = (sc.n_mats << 5);
+            acc = C3{0.0f, 0.0f, 0.0f};
+            n_sec = 0;
+            sp = 0;
+            alive = true;
+          } else {
+            eb.sec_count[t] = 0;  // a slot past the image edge: no pixel
+          }
+        } else {
+          done = true;
         }
-        const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
-        const BsdfW W = bsdf_weights(M, cos0);
-        const C3 wd = cur.w * W.wd;
-        if (nonzero(wd)) {
-          // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
-          // both are evaluated per query by k_shade
-          uint32_t slot = t;
-          if (depth == 0) primary_q = true;
-          if (depth > 0) {
-            // deeper query: the pixel's own slots, else the overflow region
-            if (n_sec < eb.sec_max) {
-              slot = eb.q_primary * (1u + n_sec) + t;
-              ++n_sec;
-            } else {
-              const uint32_t j = atomicAdd(&st->n_queries, 1u);
-              slot = j < eb.ovf_cap ? eb.q_primary * (1u + eb.sec_max) + j : 0xFFFFFFFFu;
-              if (j >= eb.ovf_cap) atomicOr(&st->overflow, 2u);
-            }
-          }
-          if (slot != 0xFFFFFFFFu) {
-            Query qq;
-            qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
-            qq.b = make_float4(is.n.x, is.n.y, is.n.z, 0.0f);
-            qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
-            eb.q[slot] = qq;
+      }
+      next += (uint32_t)__popcll(m);
+    }
+    if (!__any(alive || !done)) break;
+    if (!alive) continue;
+    const uint32_t depth = cur.meta & 31u;
+    const uint32_t med = (cur.meta >> 5) & 2047u;
+    const uint32_t path = cur.meta >> 16;
+    bool next_set = false;
+    EyeNode nxt;
+    Isect is;
+    if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is, stk + threadIdx.x)) {
+      const DevMaterial& M = sc.mats[is.mat];
+      const DevMaterial& Mm = sc.mats[med];
+      Rng nr;
+      nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
+      if (depth == 0)
+        for (uint32_t k = 0; k < cam_draws; ++k) (void)nr.next();
+      acc = acc + cur.w * (c3(M.em) * SR_HALF_F);
+      float cos1;
+      V3 rdir = specular_reflection(is.n, cur.d, cos1);
+      const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
+      V3 tdir = mk(0.0f, 0.0f, 0.0f);
+      bool has_t = false;
+      float cos2 = 0.0f;
+      if (spec_part) {
+        // tracer.rs:150-161
+        rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
+        const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
+        const V3 hvec = normalize_or(rdir - cur.d, is.n);
+        has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+      }
+      const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
+      const BsdfW W = bsdf_weights(M, cos0);
+      const C3 wd = cur.w * W.wd;
+      if (nonzero(wd)) {
+        // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
+        // both are evaluated per query by k_shade
+        uint32_t slot = t;
+        if (depth > 0) {
+          // deeper query: the pixel's own slots, else the overflow region
+          if (n_sec < eb.sec_max) {
+            slot = eb.q_primary * (1u + n_sec) + t;
+            ++n_sec;
+          } else {
+            const uint32_t j = atomicAdd(&st->n_queries, 1u);
+            slot = j < eb.ovf_cap ? eb.q_primary * (1u + eb.sec_max) + j : 0xFFFFFFFFu;
+            if (j >= eb.ovf_cap) atomicOr(&st->overflow, 2u);
           }
         }
-        if ((int)depth + 1 < maxd) {
-          const C3 ws = cur.w * W.ws;
-          const C3 wt = cur.w * W.wt;
-          const bool rc = surf_reflect(M, cos1) && nonzero(ws);
-          const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
-          if (tc) {
-            const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
-            const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
-            if (rc) {
-              if (sp < MAX_DEPTH_LIMIT) stack[sp++] = c;
-            } else {
-              next = c;
-              next_set = true;
-            }
-          }
+        if (slot != 0xFFFFFFFFu) {
+          Query qq;
+          qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
+          qq.b = make_float4(is.n.x, is.n.y, is.n.z, 0.0f);
+          qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
+          eb.q[slot] = qq;
+        }
+      }
+      if ((int)depth + 1 < maxd) {
+        const C3 ws = cur.w * W.ws;
+        const C3 wt = cur.w * W.wt;
+        const bool rc = surf_reflect(M, cos1) && nonzero(ws);
+        const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
+        if (tc) {
+          const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
+          const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
           if (rc) {
-            next = EyeNode{is.pos, rdir, ws, (depth + 1) | (med << 5) | (path << 16)};
+            if (sp < MAX_DEPTH_LIMIT) stack[sp++] = c;
+          } else {
+            nxt = c;
             next_set = true;
           }
         }
-      }
-      if (next_set) {
-        cur = next;
-      } else if (sp > 0) {
-        cur = stack[--sp];
-      } else {
-        have = false;
+        if (rc) {
+          nxt = EyeNode{is.pos, rdir, ws, (depth + 1) | (med << 5) | (path << 16)};
+          next_set = true;
+        }
       }
     }
-    eb.pix[pixel] = make_float4(acc.r, acc.g, acc.b, 0.0f);
+    if (next_set) {
+      cur = nxt;
+    } else if (sp > 0) {
+      cur = stack[--sp];
+    } else {
+      eb.pix[pixel] = make_float4(acc.r, acc.g, acc.b, 0.0f);
+      eb.sec_count[t] = n_sec;
+      alive = false;
+    }
   }
-  if (t < eb.q_primary) eb.sec_count[t] = n_sec;
-  // queries of this tile kept in place (primary + deeper), for the sort
-  uint32_t c = (primary_q ? 1u : 0u) + n_sec;
+}
+
+// Queries kept in place per tile (primary + deeper), for the sort's offsets.
+__global__ __launch_bounds__(256) void k_tile_count(EyeBuf eb) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t tile = t >> 6;
+  uint32_t c = 0;
+  if (t < eb.q_primary) c = (__float_as_uint(eb.q[t].a.w) != NO_PIXEL ? 1u : 0u) + eb.sec_count[t];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
-  if ((threadIdx.x & 63u) == 0 && wave < eb.n_tiles) eb.tile_count[wave] = c;
+  if ((threadIdx.x & 63u) == 0 && tile < eb.n_tiles) eb.tile_count[tile] = c;
 }
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
-                      int use_classic, int max_depth) {
-  const uint32_t blocks = (eb.q_primary + 255) / 256;
+                      int use_classic, int max_depth, int n_cu) {
+  uint32_t n_waves = std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u);
+  if (n_waves < 1) n_waves = 1;
+  const uint32_t blocks = (n_waves + 3) / 4;
   if (sc.bvh)
-    hipLaunchKernelGGL(k_eye<true>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic, max_depth);
+    hipLaunchKernelGGL(k_eye<true>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
+                       max_depth, n_waves);
   else
     hipLaunchKernelGGL(k_eye<false>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
-                       max_depth);
+                       max_depth, n_waves);
+  hipLaunchKernelGGL(k_tile_count, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb);
   return hipGetLastError();
 }
 

@@ -96,7 +96,7 @@ hipError_t launch_compact(hipStream_t s, const PhotonBuf& pb, const PassState* s
 hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax);
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
-                      int use_classic, int max_depth);
+                      int use_classic, int max_depth, int n_cu);
 
 // Gather + direct light of one pass over the Morton-sorted queries, then the
 // per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
@@ -112,9 +112,9 @@ hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const flo
                         const float4* qn, uint32_t nq, double r2, int refine64, int mode, int pfilter,
                         uint32_t* counts, const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out, int n_cu);
 
-// Scene BVH over the n bounded shapes shapes[ids[i]]: Morton keys of the
-// centroids (cmin, cscale = 1023 / extent), a radix sort (count *np on the
-// device), leaves of BVH_LEAF shapes copied into bprims, levels folded up.
+// Scene BVH over the n shapes shapes[ids[i]]: Morton keys of the centroids
+// (cmin, cscale = 1023 / extent), a radix sort (count *np on the device),
+// leaves of BVH_LEAF shapes copied into bprims, levels folded up.
 hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_t* ids, uint32_t n, float4 cmin,
                             float4 cscale, uint32_t depth, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np,
                             uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,

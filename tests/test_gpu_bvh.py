@@ -45,14 +45,17 @@ def test_bvh_layout():
     c = ctx()
     ds = P.DeviceScene(c, sc)
     n_lin, n_bvh, depth = ds.stats()
-    # every shape of the box scene is bounded (quads, spheres, triangles)
-    assert n_lin == 0 and n_bvh == sc.n_shapes
+    # the five 1 m walls (wider than a quarter of the scene) are tested one by
+    # one; the light quad, the spheres and the triangles are in the BVH
+    assert n_lin == 5 and n_bvh == sc.n_shapes - 5
     assert (1 << depth) >= -(-n_bvh // 4) > (1 << depth) // 2
     small = P.DeviceScene(c, P.caustic_scene(32, 24))
     assert small.stats() == (9, 0, 0)  # fewer than 64 bounded shapes: one by one
-    sc2 = P.caustic_scene(32, 24, walls="plane")
+    sc2 = P.mesh_scene(32, 24, subdiv=1, n_spheres=8, walls="plane")
     sc2.bvh = A.SCENE_BVH_ON
-    assert P.DeviceScene(c, sc2).stats() == (5, 4, 0)  # planes stay one by one
+    # planes stay one by one, and so does the 0.25 m light quad: the bounded
+    # shapes of this scene span 0.92 m, and it is wider than a quarter of that
+    assert P.DeviceScene(c, sc2).stats() == (6, 8 + 80, 5)
 
 
 @pytest.mark.parametrize("subdiv,walls", [(3, "quad"), (4, "plane")])
@@ -98,7 +101,7 @@ def test_bvh_full_mesh():
     assert sc.n_shapes == 327680 + 64 + 6
     on, off = _pair(sc)
     n_lin, n_bvh, depth = on.stats()
-    assert n_lin == 0 and n_bvh == sc.n_shapes and depth == 17
+    assert n_lin == 5 and n_bvh == sc.n_shapes - 5 and depth == 17
     o, d = _rays(4096, 5)
     t1, i1 = on.intersect(o, d)
     t0, i0 = off.intersect(o, d)
