@@ -32,6 +32,9 @@ DESCRIPTION = {
             "scene BVH",
     "two_spheres": "two_spheres: Cornell box + diffuse and mirror spheres, 256x256, 200k photons/pass, "
                    "max 8 bounces, r1=0.01 m, alpha=2/3, seed 1",
+    "batch": "batch: 64 Cornell-box rooms (random wall albedos, 2-512 spheres log-uniform), images 128^2-512^2, "
+             "50k-2M photons/pass log-uniform, 32-256 passes, max 8 bounces, seed 1; all rooms in one launch per "
+             "stage after autotuning, against the same rooms run one after another as plain sessions",
 }
 
 # stage -> (kernel, algorithmic bytes per pass from S stored photons, Q
@@ -52,6 +55,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-3sigma", action="store_true", help="skip the time-to-3-sigma comparison")
     ap.add_argument("--cpu-sample-passes", type=int, default=8)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch workload: skip the one-room-after-another comparison")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed region, fill one buffer of this many MiB and copy it device-to-device "
                          "(kernels of known traffic to scale profiler byte counters against; with HSA_ENABLE_SDMA=0 "
@@ -65,6 +70,9 @@ def main():
 
     import ppmpa_amd as P
 
+    if args.workload == "batch":
+        print(json.dumps(run_batch(P, args)), flush=True)
+        return
     cfgd = P.CONFIGS[args.workload]
     steps = args.steps if args.steps is not None else cfgd["passes"]
     sc = cfgd["scene"]()
@@ -125,6 +133,65 @@ def main():
     if args.workload == "caustic" and not args.no_3sigma and not args.no_cpu_baseline:
         out["time_to_3sigma"] = time_to_3sigma(P, ctx)
     print(json.dumps(out), flush=True)
+
+
+def run_batch(P, args):
+    """The batched workload: tune, run every room's schedule in the batch,
+    then the same rooms one after another as plain sessions (same passes and
+    seeds, so the same images and the same error)."""
+    ctx = P.Context(0)
+    items = P.batch_scenes(n=64, seed=1)
+    if args.steps is not None:  # cap the passes per room
+        items = [(sc, c, min(p, args.steps)) for sc, c, p in items]
+    dss = [P.DeviceScene(ctx, sc) for sc, _, _ in items]
+
+    def cfg(c):
+        return P.pass_cfg(seed=c["seed"], n_photon=c["n_photon"], radius0=c["radius0"], alpha=c["alpha"],
+                          max_depth=c["max_depth"])
+
+    cfgs = [cfg(c) for _, c, _ in items]
+    passes = [p for _, _, p in items]
+    b = P.Batch(ctx, dss, cfgs, passes)
+    t0 = time.perf_counter()
+    launch = b.tune(rounds=2)
+    tune_s = time.perf_counter() - t0
+    ctx.sync()
+    t0 = time.perf_counter()
+    b.run()
+    ctx.sync()
+    T = time.perf_counter() - t0
+    tb = b.timings()
+    b.close()
+    photons = sum(c["n_photon"] * p for (_, c, _), p in zip(items, passes))
+    pixels = sum(sc.camera.xreso * sc.camera.yreso * p for (sc, _, _), p in zip(items, passes))
+    # plain sessions, one room after another (graph replay per room); only
+    # the runs are timed, session set-up is not
+    T_seq = 0.0
+    for ds, cf, p in ([] if args.no_sequential else zip(dss, cfgs, passes)):
+        s = P.Session(ctx, ds, cf)
+        s.run(1)  # captures the session's pass graph
+        ctx.sync()
+        t0 = time.perf_counter()
+        s.run(p - 1)
+        ctx.sync()
+        T_seq += time.perf_counter() - t0
+        s.close()
+    T_seq *= sum(passes) / max(1, sum(passes) - len(passes))
+    if args.no_sequential:
+        T_seq = float("nan")
+    value = photons / T / 1e6
+    return {
+        "metric": "Mphotons/s traced+gathered", "value": round(value, 3), "unit": "Mphotons/s", "n_gpus": 1,
+        "steps": max(passes), "warmup": 0, "ms_per_step": round(T * 1e3 / max(passes), 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": DESCRIPTION["batch"], "scenes": len(items), "photons_total": photons,
+                   "pixel_passes_total": pixels, "rooms_with_bvh": sum(1 for d in dss if d.stats()[1] > 0),
+                   "accumulation": "f64"},
+        "gathered_px_per_s": round(pixels / T, 1),
+        "batch_s": round(T, 3), "batch_device_s": round(tb["ms_total"] / 1e3, 3),
+        "sequential_sessions_s": round(T_seq, 3), "speedup_vs_sequential": round(T_seq / T, 2),
+        "autotuned_launch": launch, "tune_s": round(tune_s, 2),
+    }
 
 
 def run_workload(P, ctx, ds, cfgd, steps, warmup):

@@ -328,6 +328,44 @@ ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
                            const ppm_pass_cfg* cfg, uint32_t n_pass,
                            double* mean_out, double* var_out, ppm_timings* t);
 
+/* ---- batches (many scenes, one launch per stage) ------------------------- */
+
+/* No counterpart in the reference, which renders one scene per process
+ * (ppmpa.rs:48-92, iterator.rb:90-117): n scenes, each with its own pass
+ * schedule (cfgs[i], n_passes[i] passes), run together.  Every stage of a
+ * pass is one launch over all scenes still running (each scene's blocks run
+ * the single-scene kernel body on the scene's own buffers), captured as one
+ * hipGraph per number of running scenes.  Each scene's image equals what its
+ * own session would produce. */
+typedef struct ppm_batch ppm_batch;
+
+/* Launch sizes of a batch (0 = the defaults 16 and 8). */
+typedef struct {
+  uint32_t eye_waves_per_cu;     /* persistent eye waves per CU per scene */
+  uint32_t shade_blocks_per_cu;  /* gather/shade blocks per CU per scene */
+  uint32_t reserved[2];
+} ppm_launch_cfg;
+
+ppm_status  ppm_batch_create(ppm_ctx* ctx, uint32_t n,
+                             const ppm_scene* const* scenes,
+                             const ppm_pass_cfg* cfgs, const uint32_t* n_passes,
+                             ppm_batch** out);
+void        ppm_batch_destroy(ppm_batch* b);
+/* Run passes until every scene has run its n_passes (max_rounds = 0), or at
+ * most max_rounds more rounds.  Synchronises. */
+ppm_status  ppm_batch_run(ppm_batch* b, uint32_t max_rounds);
+ppm_status  ppm_batch_set_launch(ppm_batch* b, const ppm_launch_cfg* lc);
+ppm_status  ppm_batch_get_launch(const ppm_batch* b, ppm_launch_cfg* lc);
+/* Autotuner: time `rounds` passes of the batch under each candidate launch
+ * configuration (eye waves 4/8/16/32 x shade blocks 2/4/8/16 per CU), keep
+ * the fastest, and reset every scene to before its first pass. */
+ppm_status  ppm_batch_tune(ppm_batch* b, uint32_t rounds, ppm_launch_cfg* best);
+/* Scene i's per-pixel mean and variance, as ppm_session_image. */
+ppm_status  ppm_batch_image(ppm_batch* b, uint32_t i, double* mean_out,
+                            double* var_out, uint32_t* passes_done);
+/* ms_total = device time of the batch's runs; counts summed over scenes. */
+ppm_status  ppm_batch_timings(ppm_batch* b, ppm_timings* t);
+
 #ifdef __cplusplus
 }
 #endif

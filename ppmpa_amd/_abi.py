@@ -138,6 +138,14 @@ class Timings(C.Structure):
     ]
 
 
+class LaunchCfg(C.Structure):
+    _fields_ = [
+        ("eye_waves_per_cu", C.c_uint32),
+        ("shade_blocks_per_cu", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
+    ]
+
+
 vp = C.c_void_p
 dp = C.POINTER(C.c_double)
 u8p = C.POINTER(C.c_uint8)
@@ -178,6 +186,14 @@ PROTOTYPES = {
     "ppm_session_radius2": (st, [vp, dp]),
     "ppm_session_timings": (st, [vp, C.POINTER(Timings)]),
     "ppm_run_passes": (st, [vp, vp, C.POINTER(PassCfg), C.c_uint32, dp, dp, C.POINTER(Timings)]),
+    "ppm_batch_create": (st, [vp, C.c_uint32, C.POINTER(vp), C.POINTER(PassCfg), u32p, C.POINTER(vp)]),
+    "ppm_batch_destroy": (None, [vp]),
+    "ppm_batch_run": (st, [vp, C.c_uint32]),
+    "ppm_batch_set_launch": (st, [vp, C.POINTER(LaunchCfg)]),
+    "ppm_batch_get_launch": (st, [vp, C.POINTER(LaunchCfg)]),
+    "ppm_batch_tune": (st, [vp, C.c_uint32, C.POINTER(LaunchCfg)]),
+    "ppm_batch_image": (st, [vp, C.c_uint32, dp, dp, u32p]),
+    "ppm_batch_timings": (st, [vp, C.POINTER(Timings)]),
 }
 
 _lib = None

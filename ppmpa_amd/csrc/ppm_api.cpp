@@ -1244,6 +1244,288 @@ ppm_status ppm_session_image(ppm_session* s, double* mean_out, double* var_out, 
   return PPM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Batches: many scenes with their own pass schedules, one launch per stage.
+
+}  // extern "C"
+
+struct ppm_batch {
+  ppm_ctx* ctx = nullptr;
+  std::vector<ppm_session*> sess;  // caller's order
+  std::vector<uint32_t> n_pass;    // caller's order
+  std::vector<uint32_t> order;     // caller indices by pass count, descending
+  BatchItem* items = nullptr;      // device, in `order`
+  uint32_t* tab = nullptr;         // device, BK_KINDS x (n + 1)
+  std::vector<uint32_t> htab;
+  bool bvh = false;
+  ppm_launch_cfg lc{};
+  std::vector<hipGraphExec_t> exec;  // per number of running scenes (index n - 1)
+  std::vector<hipGraph_t> graph;
+  hipEvent_t ev[2] = {};
+  double ms_total = 0.0;
+  uint32_t rounds_done = 0;
+};
+
+namespace {
+
+void batch_drop_graphs(ppm_batch* b) {
+  for (auto& e : b->exec)
+    if (e) (void)hipGraphExecDestroy(e), e = nullptr;
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphDestroy(g), g = nullptr;
+}
+
+// Per-scene block counts of every launch kind -> prefix table (host and
+// device) for the current launch configuration.
+hipError_t batch_layout(ppm_batch* b) {
+  const uint32_t n = (uint32_t)b->order.size();
+  const uint32_t n_cu = (uint32_t)b->ctx->n_cu;
+  const uint32_t ew = b->lc.eye_waves_per_cu ? b->lc.eye_waves_per_cu : 16u;
+  const uint32_t sb = b->lc.shade_blocks_per_cu ? b->lc.shade_blocks_per_cu : 8u;
+  std::vector<BatchItem> items(n);
+  b->htab.assign((size_t)BK_KINDS * (n + 1), 0u);
+  for (uint32_t j = 0; j < n; ++j) {
+    const ppm_session* s = b->sess[b->order[j]];
+    BatchItem& it = items[j];
+    std::memset(&it, 0, sizeof(it));
+    it.sc = s->sbuf;
+    it.st = s->st;
+    it.pb = s->pb;
+    it.m = s->map;
+    it.eb = s->eb;
+    it.sum = s->sum;
+    it.sumsq = s->sumsq;
+    it.seed = s->cfg.seed;
+    it.r2_first = s->cfg.radius0 * s->cfg.radius0;
+    it.alpha = s->cfg.alpha;
+    it.power = s->power;
+    it.mscale = morton_scale(s->sbuf.bmin, s->sbuf.bmax, 1023u);
+    it.first_pass = s->cfg.first_pass;
+    it.uc = s->cfg.use_classic;
+    it.maxd = s->cfg.max_depth;
+    it.pfilter = s->cfg.pfilter;
+    it.trace_per_wave = (uint32_t)(((uint64_t)s->sbuf.n_emit_total + s->pb.n_regions - 1) / s->pb.n_regions);
+    it.eye_waves = std::max(1u, std::min(s->eb.n_tiles, n_cu * ew));
+    // sort blocks from the expected key counts (about 2 stored photons per
+    // emitted one, 1.5 queries per pixel), 8192 keys per block: a stable
+    // radix sort gives the same order for any block count, and a batch's
+    // sorts together need no more blocks than fill the GPU
+    it.psort_blocks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(s->map.sort_blocks, (2ull * s->sbuf.n_emit_total + 8191) / 8192));
+    it.qsort_blocks = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(s->eb.q_sort_blocks, (3ull * s->eb.q_primary / 2 + 8191) / 8192));
+    uint32_t cnt[BK_KINDS];
+    cnt[BK_TRACE] = (s->pb.n_regions + 3) / 4;
+    cnt[BK_MORTON] = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1, (s->map.cap + 255) / 256));
+    cnt[BK_PSORT] = it.psort_blocks;
+    cnt[BK_SCAN] = 256;
+    cnt[BK_LEAVES] = std::min<uint32_t>(16384, (s->map.lpad_max + 63) / 64);
+    cnt[BK_EYE] = (it.eye_waves + 3) / 4;
+    cnt[BK_PIX] = (s->eb.q_primary + 255) / 256;
+    cnt[BK_QKEYS] = (s->eb.q_primary + s->eb.ovf_cap + 255) / 256;
+    cnt[BK_QSORT] = it.qsort_blocks;
+    cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, s->eb.q_primary / 128u));
+    cnt[BK_OVF] = (s->eb.ovf_cap + 255) / 256;
+    for (int k = 0; k < BK_KINDS; ++k) b->htab[(size_t)k * (n + 1) + j + 1] = b->htab[(size_t)k * (n + 1) + j] + cnt[k];
+  }
+  hipError_t e;
+  if ((e = hipMemcpy(b->items, items.data(), n * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(b->tab, b->htab.data(), b->htab.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess)
+    return e;
+  batch_drop_graphs(b);
+  b->exec.assign(n, nullptr);
+  b->graph.assign(n, nullptr);
+  return hipSuccess;
+}
+
+// One pass of the first `act` scenes (in `order`), through a captured graph.
+hipError_t batch_round(ppm_batch* b, uint32_t act) {
+  const uint32_t n = (uint32_t)b->order.size();
+  hipStream_t st = b->ctx->stream;
+  if (!b->exec[act - 1]) {
+    uint32_t blocks[BK_KINDS];
+    for (int k = 0; k < BK_KINDS; ++k) blocks[k] = b->htab[(size_t)k * (n + 1) + act];
+    hipStream_t cap;
+    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (e != hipSuccess) return e;
+    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      const hipError_t e2 = launch_batch_pass(cap, b->items, b->tab, blocks, act, n, b->bvh);
+      e = hipStreamEndCapture(cap, &b->graph[act - 1]);
+      if (e2 != hipSuccess) e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[act - 1], b->graph[act - 1], nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess) return e;
+  }
+  return hipGraphLaunch(b->exec[act - 1], st);
+}
+
+// Every scene back to before its first pass.
+hipError_t batch_reset(ppm_batch* b) {
+  const PassState h = host_state();
+  for (ppm_session* s : b->sess) {
+    const uint64_t n3 = 3ull * s->eb.n_pix;
+    hipError_t e;
+    if ((e = hipMemsetAsync(s->sum, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(s->sumsq, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(s->st, &h, sizeof(h), hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess) return e;
+    s->passes_done = 0;
+  }
+  b->rounds_done = 0;
+  return hipStreamSynchronize(b->ctx->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+void ppm_batch_destroy(ppm_batch* b) {
+  if (!b) return;
+  if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  batch_drop_graphs(b);
+  for (auto& e : b->ev)
+    if (e) (void)hipEventDestroy(e);
+  dfree(b->items);
+  dfree(b->tab);
+  for (ppm_session* s : b->sess) ppm_session_destroy(s);
+  delete b;
+}
+
+ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scenes, const ppm_pass_cfg* cfgs,
+                            const uint32_t* n_passes, ppm_batch** out) {
+  if (!c || !scenes || !cfgs || !n_passes || !out || n == 0) return fail(c, PPM_E_INVALID, "null argument or empty batch");
+  *out = nullptr;
+  if (n > 4096) return fail(c, PPM_E_RANGE, "at most 4096 scenes per batch");
+  ppm_batch* b = new ppm_batch();
+  b->ctx = c;
+  b->n_pass.assign(n_passes, n_passes + n);
+  for (uint32_t i = 0; i < n; ++i) {
+    if (!scenes[i]) {
+      ppm_batch_destroy(b);
+      return fail(c, PPM_E_INVALID, "null scene in batch");
+    }
+    ppm_pass_cfg cfg = cfgs[i];
+    cfg.flags = 0;
+    ppm_session* s = nullptr;
+    if (ppm_status st = ppm_session_create(c, scenes[i], &cfg, &s)) {
+      ppm_batch_destroy(b);
+      return st;
+    }
+    b->sess.push_back(s);
+    if (scenes[i]->buf.bvh) b->bvh = true;
+  }
+  b->order.resize(n);
+  for (uint32_t i = 0; i < n; ++i) b->order[i] = i;
+  std::stable_sort(b->order.begin(), b->order.end(),
+                   [&](uint32_t a, uint32_t d) { return b->n_pass[a] > b->n_pass[d]; });
+  hipError_t e;
+  if ((e = dalloc(&b->items, n)) != hipSuccess || (e = dalloc(&b->tab, (uint64_t)BK_KINDS * (n + 1))) != hipSuccess ||
+      (e = hipEventCreate(&b->ev[0])) != hipSuccess || (e = hipEventCreate(&b->ev[1])) != hipSuccess ||
+      (e = batch_layout(b)) != hipSuccess) {
+    ppm_batch_destroy(b);
+    return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  }
+  *out = b;
+  return PPM_OK;
+}
+
+ppm_status ppm_batch_set_launch(ppm_batch* b, const ppm_launch_cfg* lc) {
+  if (!b || !lc) return PPM_E_INVALID;
+  if (lc->eye_waves_per_cu > 64 || lc->shade_blocks_per_cu > 64)
+    return fail(b->ctx, PPM_E_RANGE, "at most 64 eye waves / shade blocks per CU");
+  b->lc = *lc;
+  HIP_TRY(b->ctx, batch_layout(b));
+  return PPM_OK;
+}
+
+ppm_status ppm_batch_get_launch(const ppm_batch* b, ppm_launch_cfg* lc) {
+  if (!b || !lc) return PPM_E_INVALID;
+  *lc = b->lc;
+  if (!lc->eye_waves_per_cu) lc->eye_waves_per_cu = 16;
+  if (!lc->shade_blocks_per_cu) lc->shade_blocks_per_cu = 8;
+  return PPM_OK;
+}
+
+// Passes until every scene has run n_passes[i] of them (or `max_rounds`
+// more rounds, when it is not 0).
+ppm_status ppm_batch_run(ppm_batch* b, uint32_t max_rounds) {
+  if (!b) return PPM_E_INVALID;
+  ppm_ctx* c = b->ctx;
+  const uint32_t n = (uint32_t)b->order.size();
+  const uint32_t total = b->n_pass[b->order[0]];
+  uint32_t end = total;
+  if (max_rounds && b->rounds_done + max_rounds < end) end = b->rounds_done + max_rounds;
+  HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
+  for (uint32_t r = b->rounds_done; r < end; ++r) {
+    uint32_t act = n;
+    while (act > 0 && b->n_pass[b->order[act - 1]] <= r) --act;
+    if (act == 0) break;
+    HIP_TRY(c, batch_round(b, act));
+    for (uint32_t j = 0; j < act; ++j) b->sess[b->order[j]]->passes_done += 1;
+  }
+  HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
+  HIP_TRY(c, hipEventSynchronize(b->ev[1]));
+  float ms = 0.0f;
+  HIP_TRY(c, hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+  b->ms_total += ms;
+  b->rounds_done = end;
+  return PPM_OK;
+}
+
+ppm_status ppm_batch_tune(ppm_batch* b, uint32_t rounds, ppm_launch_cfg* best) {
+  if (!b || rounds == 0) return PPM_E_INVALID;
+  ppm_ctx* c = b->ctx;
+  static const uint32_t eye_c[] = {4, 8, 16, 32};
+  static const uint32_t shade_c[] = {2, 4, 8, 16};
+  ppm_launch_cfg pick = b->lc;
+  double best_ms = INFINITY;
+  for (uint32_t ew : eye_c)
+    for (uint32_t sb : shade_c) {
+      ppm_launch_cfg lc{};
+      lc.eye_waves_per_cu = ew;
+      lc.shade_blocks_per_cu = sb;
+      b->lc = lc;
+      HIP_TRY(c, batch_layout(b));
+      HIP_TRY(c, batch_reset(b));
+      // the first round captures the graph; the next `rounds` are timed
+      if (ppm_status st = ppm_batch_run(b, 1)) return st;
+      const double t0 = b->ms_total;
+      if (ppm_status st = ppm_batch_run(b, rounds)) return st;
+      const double ms = b->ms_total - t0;
+      if (ms < best_ms) {
+        best_ms = ms;
+        pick = lc;
+      }
+    }
+  b->lc = pick;
+  HIP_TRY(c, batch_layout(b));
+  HIP_TRY(c, batch_reset(b));
+  b->ms_total = 0.0;
+  if (best) ppm_batch_get_launch(b, best);
+  return PPM_OK;
+}
+
+ppm_status ppm_batch_image(ppm_batch* b, uint32_t i, double* mean_out, double* var_out, uint32_t* passes_done) {
+  if (!b || i >= b->sess.size()) return PPM_E_INVALID;
+  return ppm_session_image(b->sess[i], mean_out, var_out, passes_done);
+}
+
+ppm_status ppm_batch_timings(ppm_batch* b, ppm_timings* t) {
+  if (!b || !t) return PPM_E_INVALID;
+  std::memset(t, 0, sizeof(*t));
+  for (ppm_session* s : b->sess) {
+    ppm_timings u;
+    if (ppm_status st = ppm_session_timings(s, &u)) return st;
+    t->photons_emitted += u.photons_emitted;
+    t->photons_stored += u.photons_stored;
+    t->gather_queries += u.gather_queries;
+  }
+  t->ms_total = b->ms_total;
+  t->passes = b->rounds_done;
+  return PPM_OK;
+}
+
 ppm_status ppm_run_passes(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, uint32_t n_pass, double* mean_out,
                           double* var_out, ppm_timings* t) {
   ppm_session* s = nullptr;

@@ -226,6 +226,7 @@ __device__ __forceinline__ bool slab(const Node& b, V3 o, V3 inv, float tmax, fl
 // block, so the lanes of a wave hit distinct banks).  A node is entered while its box starts no later than
 // the best hit, so equal distances still reach the lower shape index.
 __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h, uint32_t* stk) {
+  if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
   const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, sp = 0;

@@ -18,10 +18,17 @@
 
 namespace ppm {
 
+// Block index and grid size as a kernel body sees them: a plain launch passes
+// blockIdx.x / gridDim.x, a batched launch the block's place within its
+// scene's share of the grid.
+struct Blk {
+  uint32_t b, n;
+};
+
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << lane_id()) - 1ull; }
 
-static SceneView make_view(const SceneBuf& b) {
+__host__ __device__ inline SceneView make_view(const SceneBuf& b) {
   SceneView v;
   v.shapes = b.shapes;
   v.mats = b.mats;
@@ -46,8 +53,8 @@ static SceneView make_view(const SceneBuf& b) {
 // Pass state (util/iterator.rb:33-38): r_{p}^2 = r_{p-1}^2 (p - 1 + alpha) / p,
 // in double on the device so that a captured pass loop needs no host input.
 
-__global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power, Blk blk) {
+  if (threadIdx.x != 0 || blk.b != 0) return;
   PassState s = *st;
   if (s.pass == 0) {
     s.pass = first_pass;
@@ -70,10 +77,18 @@ __global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first
   *st = s;
 }
 
-__global__ void k_pass_end(PassState* st) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power) {
+  k_pass_begin_body(st, first_pass, r2_first, alpha, power, Blk{blockIdx.x, gridDim.x});
+}
+
+__device__ __forceinline__ void k_pass_end_body(PassState* st, Blk blk) {
+  if (threadIdx.x != 0 || blk.b != 0) return;
   st->total_stored += st->n_stored;
   st->total_queries += st->n_qsort + min(st->n_queries, 0xFFFFFFFFu);
+}
+
+__global__ void k_pass_end(PassState* st) {
+  k_pass_end_body(st, Blk{blockIdx.x, gridDim.x});
 }
 
 hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
@@ -226,11 +241,11 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
 // consecutive slots of the region (coalesced 32-byte writes).  No global
 // atomics: the record order depends only on the data, so it is reproducible.
 template <bool BVH>
-__global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
+__device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
-                                                       uint64_t seed, int uc, int maxd) {
+                                                       uint64_t seed, int uc, int maxd, Blk blk) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (w >= n_waves) return;
   const uint32_t pass = st->pass;
   const uint32_t lane = lane_id();
@@ -286,10 +301,17 @@ __global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* 
   }
 }
 
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
+                                                       uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
+                                                       uint64_t seed, int uc, int maxd) {
+  k_trace_photons_body<BVH>(sc, st, rec, count, n_waves, per_wave, region_cap, seed, uc, maxd, Blk{blockIdx.x, gridDim.x});
+}
+
 // Exclusive scan of up to 65536 counts in one block; the total goes to
 // *total (the pass's stored-photon or sorted-query count).
-__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uint32_t* offset, uint32_t n,
-                                                      uint32_t* total, const uint32_t* extra, uint32_t extra_cap) {
+__device__ __forceinline__ void k_scan_counts_body(const uint32_t* count, uint32_t* offset, uint32_t n,
+                                                   uint32_t* total, const uint32_t* extra, uint32_t extra_cap) {
   __shared__ uint32_t wsum[16];
   const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
   const uint32_t per = (n + 1023) / 1024;
@@ -325,6 +347,10 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uin
       run += c;
     }
   }
+}
+__global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uint32_t* offset, uint32_t n,
+                                                      uint32_t* total, const uint32_t* extra, uint32_t extra_cap) {
+  k_scan_counts_body(count, offset, n, total, extra, extra_cap);
 }
 
 uint32_t trace_waves(uint64_t n_photon, int n_cu) {
@@ -422,7 +448,7 @@ __device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, flo
          expand_bits10((uint32_t)fz);
 }
 
-static float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax) {
+float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax) {
   float4 scale;
   scale.x = (float)((double)(qmax + 1) / fmax((double)bmax.x - (double)bmin.x, 1e-30));
   scale.y = (float)((double)(qmax + 1) / fmax((double)bmax.y - (double)bmin.y, 1e-30));
@@ -433,19 +459,26 @@ static float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax) {
 
 
 // Keys of the stored photons (morton_key) and their dense -> region index.
-__global__ __launch_bounds__(256) void k_morton(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
+__device__ __forceinline__ void k_morton_body(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
                                                 uint32_t region_cap, const PassState* st, uint32_t* keys,
                                                 uint32_t* vals, float4 bmin, float4 bscale, uint32_t qmax,
-                                                uint32_t* nonfinite) {
+                                                uint32_t* nonfinite, Blk blk) {
   const uint32_t n = st->n_stored;
   const float fq = (float)qmax;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
     const uint64_t src = region_index(offset, n_regions, region_cap, i);
     const float4 p = ph[src].pm;
     if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(nonfinite, 1u);
     keys[i] = morton_key(p.x, p.y, p.z, bmin, bscale, fq);
     vals[i] = (uint32_t)src;
   }
+}
+
+__global__ __launch_bounds__(256) void k_morton(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
+                                                uint32_t region_cap, const PassState* st, uint32_t* keys,
+                                                uint32_t* vals, float4 bmin, float4 bscale, uint32_t qmax,
+                                                uint32_t* nonfinite) {
+  k_morton_body(ph, offset, n_regions, region_cap, st, keys, vals, bmin, bscale, qmax, nonfinite, Blk{blockIdx.x, gridDim.x});
 }
 
 // --- LSD radix sort, 8-bit digits, count on the device ---------------------
@@ -461,10 +494,10 @@ __device__ __forceinline__ void rs_range(uint32_t n, uint32_t G, uint32_t b, uin
   end = ee < n ? (uint32_t)ee : n;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, const uint32_t* np,
-                                                         uint32_t* ghist, uint32_t shift) {
+__device__ __forceinline__ void k_rs_hist_body(const uint32_t* keys, const uint32_t* np,
+                                                         uint32_t* ghist, uint32_t shift, Blk blk) {
   __shared__ uint32_t h[256];
-  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t G = blk.n, b = blk.b;
   h[threadIdx.x] = 0;
   __syncthreads();
   uint32_t beg, end;
@@ -474,10 +507,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, co
   ghist[threadIdx.x * G + b] = h[threadIdx.x];
 }
 
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(const uint32_t* keys, const uint32_t* np,
+                                                         uint32_t* ghist, uint32_t shift) {
+  k_rs_hist_body(keys, np, ghist, shift, Blk{blockIdx.x, gridDim.x});
+}
+
 // One block per digit: exclusive scan of that digit's per-block counts.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* ghist, uint32_t* totals, uint32_t G) {
+__device__ __forceinline__ void k_rs_scan_body(uint32_t* ghist, uint32_t* totals, uint32_t G, Blk blk) {
   __shared__ uint32_t wsum[16];
-  const uint32_t d = blockIdx.x;
+  const uint32_t d = blk.b;
   uint32_t* row = ghist + (uint64_t)d * G;
   const uint32_t t = threadIdx.x;
   const uint32_t per = (G + 1023) / 1024;  // entries per thread
@@ -516,17 +554,21 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* ghist, uint32_t* tot
   }
 }
 
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* ghist, uint32_t* totals, uint32_t G) {
+  k_rs_scan_body(ghist, totals, G, Blk{blockIdx.x, gridDim.x});
+}
+
 // Stable scatter.  Each round moves 256 keys: lanes with equal digits are
 // matched with 8 ballots, ranked by mbcnt, and placed after the same digit's
 // keys of earlier waves and earlier rounds.
-__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+__device__ __forceinline__ void k_rs_scatter_body(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                                             uint32_t* vout, const uint32_t* np,
                                                             const uint32_t* ghist, const uint32_t* totals,
-                                                            uint32_t shift) {
+                                                            uint32_t shift, Blk blk) {
   __shared__ uint32_t run_off[256];
   __shared__ uint32_t wcnt[RS_THREADS / 64][256];
   __shared__ uint32_t dig[256];
-  const uint32_t G = gridDim.x, b = blockIdx.x, t = threadIdx.x;
+  const uint32_t G = blk.n, b = blk.b, t = threadIdx.x;
   const uint32_t lane = t & 63u, w = t >> 6;
   // exclusive scan of the digit totals
   dig[t] = totals[t];
@@ -587,6 +629,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, 
   }
 }
 
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                            uint32_t* vout, const uint32_t* np,
+                                                            const uint32_t* ghist, const uint32_t* totals,
+                                                            uint32_t shift) {
+  k_rs_scatter_body(kin, vin, kout, vout, np, ghist, totals, shift, Blk{blockIdx.x, gridDim.x});
+}
+
 // Leaves + tree.  The tree is a complete binary tree in heap order over
 // lpad = 2^depth leaves of 64 consecutive photons (empty leaves have empty
 // boxes).  k_build_leaves: one block per 64 leaves; each wave gathers 16
@@ -622,9 +671,9 @@ __device__ __forceinline__ Node node_union(const Node& a, const Node& c) {
 constexpr uint32_t SUB_LEAVES = 64;  // leaves per block of k_build_leaves
 constexpr uint32_t SUB_LEVELS = 6;   // log2(SUB_LEAVES)
 
-__global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const double* in64, const uint32_t* order,
+__device__ __forceinline__ void k_build_leaves_body(const PhotonRec* in, const double* in64, const uint32_t* order,
                                                       PhotonRec* out, double* out64, uint32_t* src_idx, Node* nodes,
-                                                      PassState* st) {
+                                                      PassState* st, Blk blk) {
   __shared__ Node box[SUB_LEAVES];
   const uint32_t n = st->n_stored;
   uint32_t L, depth;
@@ -632,11 +681,11 @@ __global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const
   const uint32_t lpad = 1u << depth;
   const uint32_t groups = (lpad + SUB_LEAVES - 1) / SUB_LEAVES;
   const uint32_t lane = lane_id(), wib = threadIdx.x >> 6;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk.b == 0 && threadIdx.x == 0) {
     st->n_leaves = L;
     st->depth = depth;
   }
-  for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
+  for (uint32_t g = blk.b; g < groups; g += blk.n) {
     for (uint32_t k = 0; k < SUB_LEAVES / 4; ++k) {
       const uint32_t jl = wib * (SUB_LEAVES / 4) + k;  // leaf within the group
       const uint32_t j = g * SUB_LEAVES + jl;
@@ -685,7 +734,13 @@ __global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const
   }
 }
 
-__global__ __launch_bounds__(1024) void k_build_top(Node* nodes, const PassState* st) {
+__global__ __launch_bounds__(256) void k_build_leaves(const PhotonRec* in, const double* in64, const uint32_t* order,
+                                                      PhotonRec* out, double* out64, uint32_t* src_idx, Node* nodes,
+                                                      PassState* st) {
+  k_build_leaves_body(in, in64, order, out, out64, src_idx, nodes, st, Blk{blockIdx.x, gridDim.x});
+}
+
+__device__ __forceinline__ void k_build_top_body(Node* nodes, const PassState* st) {
   uint32_t L, depth;
   tree_shape(st->n_stored, L, depth);
   if (depth <= SUB_LEVELS) return;
@@ -699,6 +754,7 @@ __global__ __launch_bounds__(1024) void k_build_top(Node* nodes, const PassState
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(1024) void k_build_top(Node* nodes, const PassState* st) { k_build_top_body(nodes, st); }
 
 // LSD radix sort of (key, value) pairs with the count on the device; the
 // result lands in keys[*cur] / vals[*cur].
@@ -878,10 +934,10 @@ struct EyeNode {
 // slots: the primary one to t, the k-th deeper one to q_primary*(1+k)+t,
 // further ones to the overflow region.
 template <bool BVH>
-__global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
-                                             int maxd, uint32_t n_waves) {
+__device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
+                                             int maxd, uint32_t n_waves, Blk blk) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  const uint32_t w = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (w >= n_waves) return;
   const uint32_t pass = st->pass;
   // this wave's slots, numbered v = 64 k + lane-in-tile for its k-th tile
@@ -1016,15 +1072,25 @@ __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf
   }
 }
 
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
+                                             int maxd, uint32_t n_waves) {
+  k_eye_body<BVH>(sc, st, eb, seed, uc, maxd, n_waves, Blk{blockIdx.x, gridDim.x});
+}
+
 // Queries kept in place per tile (primary + deeper), for the sort's offsets.
-__global__ __launch_bounds__(256) void k_tile_count(EyeBuf eb) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_tile_count_body(EyeBuf eb, Blk blk) {
+  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   const uint32_t tile = t >> 6;
   uint32_t c = 0;
   if (t < eb.q_primary) c = (__float_as_uint(eb.q[t].a.w) != NO_PIXEL ? 1u : 0u) + eb.sec_count[t];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63u) == 0 && tile < eb.n_tiles) eb.tile_count[tile] = c;
+}
+
+__global__ __launch_bounds__(256) void k_tile_count(EyeBuf eb) {
+  k_tile_count_body(eb, Blk{blockIdx.x, gridDim.x});
 }
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
@@ -1125,9 +1191,9 @@ __device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const Photo
 // prefix of the earlier lanes' counts.  Threads past the tiles key the
 // overflow queries, which follow all in-place ones.  Key = Morton code of
 // the position.
-__global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
-                                                    float fq, uint32_t* keys, uint32_t* vals) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
+                                                    float fq, uint32_t* keys, uint32_t* vals, Blk blk) {
+  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   const uint32_t g = t >> 6, lane = lane_id();
   const uint64_t ovf_base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
   if (g >= eb.n_tiles) {
@@ -1166,6 +1232,11 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
   }
 }
 
+__global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
+                                                    float fq, uint32_t* keys, uint32_t* vals) {
+  k_query_keys_body(eb, st, bmin, bscale, fq, keys, vals, Blk{blockIdx.x, gridDim.x});
+}
+
 // Shading of the queries in Morton order.  A wave takes 64 consecutive
 // sorted queries (close together in space), one per lane.  The wave
 // traverses the tree for the union of its queries' balls; each overlapped
@@ -1175,9 +1246,9 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
 // BSDF factor and path throughput.
 template <bool BVH>
-__global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, const PhotonRec* ph,
+__device__ __forceinline__ void k_shade_body(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
-                                               int uc) {
+                                               int uc, Blk blk) {
   __shared__ LeafLds lds[4];
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
@@ -1185,9 +1256,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, 
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
   const uint32_t groups = (nq + 63) / 64;
-  const uint32_t waves = gridDim.x * 4;
+  const uint32_t waves = blk.n * 4;
   const uint32_t first_leaf = (1u << depth) - 1u;
-  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
+  for (uint32_t g = blk.b * 4 + wib; g < groups; g += waves) {
     const uint32_t i = g * 64 + lane;
     const bool active = i < nq;
     uint32_t slot = 0;
@@ -1248,9 +1319,16 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, 
   }
 }
 
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, const PhotonRec* ph,
+                                               const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
+                                               int uc) {
+  k_shade_body<BVH>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+}
+
 // Overflow results into their pixels (float atomics; few per pass).
-__global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState* st) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_overflow_add_body(EyeBuf eb, const PassState* st, Blk blk) {
+  const uint32_t i = blk.b * blockDim.x + threadIdx.x;
   if (i >= min(st->n_queries, eb.ovf_cap)) return;
   const uint64_t slot = (uint64_t)eb.q_primary * (1u + eb.sec_max) + i;
   const uint32_t p = __float_as_uint(eb.q[slot].a.w);
@@ -1260,11 +1338,15 @@ __global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState
   atomicAdd(&eb.pix[p].z, r.z);
 }
 
+__global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState* st) {
+  k_overflow_add_body(eb, st, Blk{blockIdx.x, gridDim.x});
+}
+
 // Per-pixel pass radiance = pix (emission + overflow results) + res of the pixel's primary and deeper queries, in a fixed
 // order; accumulated into the FP64 sums (util/averager2.rb:58-60) when sum
 // is given, else stored back into pix.
-__global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, double* sum, double* sumsq) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void k_accumulate_body(EyeBuf eb, DevCamera cam, double* sum, double* sumsq, Blk blk) {
+  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   if (t >= eb.q_primary) return;
   const uint32_t g = t >> 6, wi = t & 63u;
   const int x = (int)((g % (uint32_t)cam.tiles_x) * 8u + (wi & 7u));
@@ -1298,6 +1380,10 @@ __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, do
   } else {
     eb.pix[p] = c;
   }
+}
+
+__global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, double* sum, double* sumsq) {
+  k_accumulate_body(eb, cam, sum, sumsq, Blk{blockIdx.x, gridDim.x});
 }
 
 hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
@@ -1454,6 +1540,203 @@ hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, 
   else
     hipLaunchKernelGGL(k_intersect<false>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
                        obj_out);
+  return hipGetLastError();
+}
+
+}  // namespace ppm
+
+// ===========================================================================
+// Batched passes: many scenes, each with its own buffers and schedule, in one
+// launch per stage.  A stage's grid is the concatenation of every scene's
+// own grid for that stage: scene s owns blocks [first[s], first[s+1]) and
+// its blocks run the same kernel body as a single-scene launch, with their
+// block index taken within that range, so a scene computes exactly what its
+// own session would.
+namespace ppm {
+
+// Scene owning block b: the last s with first[s] <= b (uniform per block,
+// so the search runs on scalar loads).
+__device__ __forceinline__ uint32_t batch_find(const uint32_t* first, uint32_t n, uint32_t b) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ldc(first, mid) <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// The scene's item, read at a uniform address through the constant space.
+__device__ __forceinline__ BatchItem load_item(const BatchItem* items, uint32_t s) {
+  return ldc(items, s);
+}
+
+#define BATCH_PROLOGUE(kind)                                        \
+  const uint32_t* first = tab + (size_t)(kind) * (n_all + 1u);      \
+  const uint32_t s = batch_find(first, n, blockIdx.x);              \
+  const uint32_t b0 = ldc(first, s);                                \
+  const Blk blk{blockIdx.x - b0, ldc(first, s + 1u) - b0};          \
+  const BatchItem it = load_item(items, s);
+
+__global__ void kb_pass_begin(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_pass_begin_body(it.st, it.first_pass, it.r2_first, it.alpha, it.power, Blk{0u, 1u});
+}
+
+__global__ void kb_pass_end(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_pass_end_body(it.st, Blk{0u, 1u});
+}
+
+template <bool BVH>
+__global__ __launch_bounds__(256) void kb_trace(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                uint32_t n_all) {
+  BATCH_PROLOGUE(BK_TRACE)
+  k_trace_photons_body<BVH>(make_view(it.sc), it.st, it.pb.rec, it.pb.count, it.pb.n_regions, it.trace_per_wave,
+                            it.pb.region_cap, it.seed, it.uc, it.maxd, blk);
+}
+
+// one block per scene: the photon counts of the tracing waves, then (second
+// use) the per-tile query counts
+__global__ __launch_bounds__(1024) void kb_scan_photons(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_scan_counts_body(it.pb.count, it.pb.offset, it.pb.n_regions, &it.st->n_stored, nullptr, 0u);
+}
+
+__global__ __launch_bounds__(1024) void kb_scan_tiles(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_scan_counts_body(it.eb.tile_count, it.eb.tile_off, it.eb.n_tiles, &it.st->n_qsort, &it.st->n_queries,
+                     it.eb.ovf_cap);
+}
+
+__global__ __launch_bounds__(256) void kb_morton(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                 uint32_t n_all) {
+  BATCH_PROLOGUE(BK_MORTON)
+  k_morton_body(it.pb.rec, it.pb.offset, it.pb.n_regions, it.pb.region_cap, it.st, it.m.keys[0], it.m.vals[0],
+                it.sc.bmin, it.mscale, 1023u, &it.st->nonfinite, blk);
+}
+
+// Radix-sort digit pass `p` (shift 8p) of the photon keys (which = 0) or the
+// query keys (which = 1).
+__global__ __launch_bounds__(RS_THREADS) void kb_rs_hist(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                         uint32_t n_all, int which, uint32_t p) {
+  BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
+  const uint32_t* keys = which ? it.eb.qkeys[p & 1u] : it.m.keys[p & 1u];
+  k_rs_hist_body(keys, which ? &it.st->n_qsort : &it.st->n_stored, which ? it.eb.qghist : it.m.ghist, 8u * p, blk);
+}
+
+__global__ __launch_bounds__(1024) void kb_rs_scan(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                   uint32_t n_all, int which) {
+  BATCH_PROLOGUE(BK_SCAN)
+  k_rs_scan_body(which ? it.eb.qghist : it.m.ghist, which ? it.eb.qtotals : it.m.totals,
+                 which ? it.qsort_blocks : it.psort_blocks, blk);
+}
+
+__global__ __launch_bounds__(RS_THREADS) void kb_rs_scatter(const BatchItem* items, const uint32_t* tab,
+                                                            uint32_t n, uint32_t n_all, int which, uint32_t p) {
+  BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
+  const uint32_t i0 = p & 1u, i1 = i0 ^ 1u;
+  if (which)
+    k_rs_scatter_body(it.eb.qkeys[i0], it.eb.qvals[i0], it.eb.qkeys[i1], it.eb.qvals[i1], &it.st->n_qsort,
+                      it.eb.qghist, it.eb.qtotals, 8u * p, blk);
+  else
+    k_rs_scatter_body(it.m.keys[i0], it.m.vals[i0], it.m.keys[i1], it.m.vals[i1], &it.st->n_stored, it.m.ghist,
+                      it.m.totals, 8u * p, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_build_leaves(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                       uint32_t n_all) {
+  BATCH_PROLOGUE(BK_LEAVES)
+  k_build_leaves_body(it.pb.rec, it.pb.rec64, it.m.vals[0], it.m.sorted, it.m.sorted64, it.m.src_idx, it.m.nodes,
+                      it.st, blk);
+}
+
+__global__ __launch_bounds__(1024) void kb_build_top(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_build_top_body(it.m.nodes, it.st);
+}
+
+template <bool BVH>
+__global__ __launch_bounds__(256) void kb_eye(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                              uint32_t n_all) {
+  BATCH_PROLOGUE(BK_EYE)
+  k_eye_body<BVH>(make_view(it.sc), it.st, it.eb, it.seed, it.uc, it.maxd, it.eye_waves, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_tile_count(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_PIX)
+  k_tile_count_body(it.eb, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_QKEYS)
+  k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, 1023.0f, it.eb.qkeys[0], it.eb.qvals[0], blk);
+}
+
+template <bool BVH>
+__global__ __launch_bounds__(256) void kb_shade(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                uint32_t n_all) {
+  BATCH_PROLOGUE(BK_SHADE)
+  k_shade_body<BVH>(make_view(it.sc), it.m.nodes, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_overflow_add(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                       uint32_t n_all) {
+  BATCH_PROLOGUE(BK_OVF)
+  k_overflow_add_body(it.eb, it.st, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_accumulate(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_PIX)
+  k_accumulate_body(it.eb, it.sc.cam, it.sum, it.sumsq, blk);
+}
+
+#undef BATCH_PROLOGUE
+
+hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32_t* tab, const uint32_t* blocks,
+                             uint32_t n, uint32_t n_all, bool bvh) {
+  // blocks[k] = total blocks of kind k over the first n scenes
+  const dim3 T256(256), T1024(1024), T64(64), TRS(RS_THREADS);
+  hipLaunchKernelGGL(kb_pass_begin, dim3(n), T64, 0, s, items, n);
+  if (bvh)
+    hipLaunchKernelGGL(kb_trace<true>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
+  else
+    hipLaunchKernelGGL(kb_trace<false>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_scan_photons, dim3(n), T1024, 0, s, items, n);
+  hipLaunchKernelGGL(kb_morton, dim3(blocks[BK_MORTON]), T256, 0, s, items, tab, n, n_all);
+  for (uint32_t p = 0; p < 4; ++p) {
+    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
+    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s, items, tab, n, n_all, 0);
+    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
+  }
+  hipLaunchKernelGGL(kb_build_leaves, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_build_top, dim3(n), T1024, 0, s, items, n);
+  if (bvh)
+    hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
+  else
+    hipLaunchKernelGGL(kb_eye<false>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s, items, n);
+  hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s, items, tab, n, n_all);
+  for (uint32_t p = 0; p < 4; ++p) {
+    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_QSORT]), TRS, 0, s, items, tab, n, n_all, 1, p);
+    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s, items, tab, n, n_all, 1);
+    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_QSORT]), TRS, 0, s, items, tab, n, n_all, 1, p);
+  }
+  if (bvh)
+    hipLaunchKernelGGL(kb_shade<true>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
+  else
+    hipLaunchKernelGGL(kb_shade<false>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_overflow_add, dim3(blocks[BK_OVF]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_accumulate, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_pass_end, dim3(n), T64, 0, s, items, n);
   return hipGetLastError();
 }
 

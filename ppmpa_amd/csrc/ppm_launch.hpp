@@ -120,6 +120,50 @@ hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_
                             uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,
                             Node* nodes);
 
+// One scene of a batch: everything its pass kernels read (device memory).
+struct BatchItem {
+  SceneBuf sc;
+  PassState* st;
+  PhotonBuf pb;
+  MapBuf m;
+  EyeBuf eb;
+  double* sum;
+  double* sumsq;
+  uint64_t seed;
+  double r2_first, alpha, power;
+  float4 mscale;             // Morton scale of the scene bounds (1023 cells per axis)
+  uint32_t first_pass;
+  int32_t uc, maxd, pfilter;
+  uint32_t trace_per_wave;   // photons per tracing wave
+  uint32_t eye_waves;        // persistent eye waves
+  uint32_t psort_blocks;     // blocks of the photon-key sort (<= m.sort_blocks)
+  uint32_t qsort_blocks;     // blocks of the query-key sort (<= eb.q_sort_blocks)
+};
+
+// Launch kinds of a batched pass; tab[k * (n_all + 1) + s] is the first block
+// of scene s in a kind-k launch (tab[k * (n_all + 1) + n_all] the total).
+enum BatchKind : int {
+  BK_TRACE = 0,  // photon tracing waves / 4
+  BK_MORTON,     // photon Morton keys
+  BK_PSORT,      // photon sort blocks (hist, scatter)
+  BK_SCAN,       // 256 digit blocks of a sort's scan
+  BK_LEAVES,     // map leaf groups
+  BK_EYE,        // eye waves / 4
+  BK_PIX,        // primary slots / 256 (tile counts, accumulation)
+  BK_QKEYS,      // (primary + overflow slots) / 256
+  BK_QSORT,      // query sort blocks
+  BK_SHADE,      // shade blocks
+  BK_OVF,        // overflow slots / 256
+  BK_KINDS
+};
+
+float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax);
+
+// One pass of the first n scenes of a batch (scenes sorted by pass count,
+// so the scenes still running are a prefix); blocks[k] = tab[k*(n_all+1)+n].
+hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32_t* tab, const uint32_t* blocks,
+                             uint32_t n, uint32_t n_all, bool bvh);
+
 // Test hook: nearest hit of n rays (unit directions).
 hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
                             float* t_out, int32_t* obj_out);

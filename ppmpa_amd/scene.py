@@ -440,6 +440,62 @@ def mesh_scene(xreso=1920, yreso=1080, subdiv=7, n_spheres=64, seed=1, walls="qu
     return sc
 
 
+def batch_scenes(n=64, seed=1, max_side=512, min_side=128):
+    """The batched workload: n Cornell-box-like rooms (1 x 1 x 1 m, quad
+    walls of random RGB albedo U(0.1, 0.9)^3 each, the 100 W ceiling light),
+    each holding k spheres, k log-uniform in [2, 512], of radius
+    U(0.5, 1) * min(0.15, 0.25 k^(-1/3)) at non-overlapping places (at most
+    200 tries per sphere), each diffuse (0.75), mirror or glass (IOR 1.5) with
+    equal probability.  Per room: a square image of side U{min_side..max_side},
+    photons per pass log-uniform in [50k, 2M], passes U{32..256}, r_1 =
+    0.01 m * sqrt(200k / photons), alpha = 2/3, max depth 8, seed = seed * 1000
+    + room.  Draws come from a Philox stream keyed by `seed`.
+
+    Returns [(scene, cfg dict, passes)]."""
+    rng = np.random.Generator(np.random.Philox(key=int(seed) + 7))
+    out = []
+    for i in range(n):
+        sc = Scene()
+        walls = [lambertian(tuple(rng.uniform(0.1, 0.9, 3))) for _ in range(5)]
+        wm = [sc.add_material(m) for m in walls]
+        sc.shapes += [
+            parallelogram((0, 0, 0), (0, 0, 1), (1, 0, 0), wm[0]),
+            parallelogram((0, 1, 0), (1, 1, 0), (0, 1, 1), wm[1]),
+            parallelogram((0, 0, 1), (0, 1, 1), (1, 0, 1), wm[2]),
+            parallelogram((0, 0, 0), (0, 1, 0), (0, 0, 1), wm[3]),
+            parallelogram((1, 0, 0), (1, 0, 1), (1, 1, 0), wm[4]),
+        ]
+        sc.bounds_min, sc.bounds_max = BOX_BOUNDS
+        s = 0.25
+        x0 = 0.5 - s / 2
+        p0 = (x0, 0.999, x0)
+        em = 2.0 * 100.0 / (s * s)
+        lm = sc.add_material(simple((0, 0, 0), (0.8, 0.8, 0.8), 0.0, 0.0, 0.0, emittance=(em, em, em)))
+        sc.shapes.append(parallelogram(p0, (x0 + s, 0.999, x0), (x0, 0.999, x0 + s), lm, nvec=(0, -1, 0)))
+        sc.lights.append(parallelogram_light((1, 1, 1), 100.0, p0, (0, -1, 0), (s, 0, 0), (0, 0, s)))
+        mats = (sc.add_material(lambertian(0.75)), sc.add_material(mirror()), sc.add_material(glass(1.5)))
+        k = int(round(np.exp(rng.uniform(np.log(2), np.log(512)))))
+        rmax = min(0.15, 0.25 * k ** (-1.0 / 3.0))
+        placed = []
+        for _ in range(k):
+            for _try in range(200):
+                r = rng.uniform(0.5, 1.0) * rmax
+                c = rng.uniform(r + 0.005, 1.0 - r - 0.005, 3)
+                c[1] = min(c[1], 0.999 - r - 0.01)
+                if all(np.linalg.norm(c - q) > r + t for q, t in placed):
+                    placed.append((c, r))
+                    sc.shapes.append(sphere(c, r, mats[int(rng.integers(0, 3))]))
+                    break
+        side = int(rng.integers(min_side, max_side + 1))
+        sc.camera = box_camera(side, side)
+        n_photon = int(round(np.exp(rng.uniform(np.log(50_000), np.log(2_000_000)))))
+        passes = int(rng.integers(32, 257))
+        cfg = dict(n_photon=n_photon, radius0=0.01 * np.sqrt(200_000 / n_photon), alpha=2.0 / 3.0, max_depth=8,
+                   seed=int(seed) * 1000 + i)
+        out.append((sc, cfg, passes))
+    return out
+
+
 def reference_room() -> Scene:
     """A room in the style of the reference's built-in scene (scene.rs:20-447):
     4 x 4 x 11 m walls of Plain shapes, a 1.33 m ceiling parallelogram light of

@@ -320,3 +320,65 @@ def run_passes(ctx, dscene, cfg, n_pass):
         return mean, var, s.timings()
     finally:
         s.close()
+
+
+class Batch:
+    """Many scenes, each with its own pass schedule, run together: one launch
+    per pass stage over every scene still running (ppm_batch_*).  Each
+    scene's image equals what its own Session would produce."""
+
+    def __init__(self, ctx, dscenes, cfgs, n_passes):
+        self.ctx, self.dscenes = ctx, list(dscenes)
+        n = len(self.dscenes)
+        hs = (C.c_void_p * n)(*[d.h.value for d in self.dscenes])
+        cf = (A.PassCfg * n)(*cfgs)
+        npass = (C.c_uint32 * n)(*[int(k) for k in n_passes])
+        h = C.c_void_p()
+        ctx.check(ctx.lib.ppm_batch_create(ctx.h, n, hs, cf, npass, C.byref(h)))
+        self.h = h
+        self.n_passes = [int(k) for k in n_passes]
+
+    def run(self, max_rounds=0):
+        self.ctx.check(self.ctx.lib.ppm_batch_run(self.h, int(max_rounds)))
+
+    def launch(self):
+        lc = A.LaunchCfg()
+        self.ctx.check(self.ctx.lib.ppm_batch_get_launch(self.h, C.byref(lc)))
+        return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu}
+
+    def set_launch(self, eye_waves_per_cu=0, shade_blocks_per_cu=0):
+        lc = A.LaunchCfg()
+        lc.eye_waves_per_cu, lc.shade_blocks_per_cu = int(eye_waves_per_cu), int(shade_blocks_per_cu)
+        self.ctx.check(self.ctx.lib.ppm_batch_set_launch(self.h, C.byref(lc)))
+
+    def tune(self, rounds=2):
+        """Autotune the launch sizes on `rounds` timed passes per candidate;
+        every scene is reset to before its first pass afterwards."""
+        lc = A.LaunchCfg()
+        self.ctx.check(self.ctx.lib.ppm_batch_tune(self.h, int(rounds), C.byref(lc)))
+        return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu}
+
+    def image(self, i, with_var=True):
+        cam = self.dscenes[i].scene.camera
+        mean = np.zeros((cam.yreso, cam.xreso, 3))
+        var = np.zeros_like(mean) if with_var else None
+        n = C.c_uint32()
+        self.ctx.check(self.ctx.lib.ppm_batch_image(self.h, int(i), _dp(mean), _dp(var) if with_var else None,
+                                                    C.byref(n)))
+        return mean, var, n.value
+
+    def timings(self):
+        t = A.Timings()
+        self.ctx.check(self.ctx.lib.ppm_batch_timings(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in A.Timings._fields_ if k != "reserved"}
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.ppm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,96 @@
+"""GPU: batched passes over many scenes.  A batch runs every pass stage as one
+launch over all its scenes; each scene's blocks run the single-scene kernel
+bodies on that scene's buffers, so every scene's image must equal the image
+of its own session with the same schedule, bit for bit except in pixels
+whose eye tree overflowed its 2 * max_depth query slots (their extra queries
+are added with float atomics, in no fixed order, in a session as well)."""
+import numpy as np
+import pytest
+
+from gpu_common import ctx, say
+
+import ppmpa_amd as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch(n, seed):
+    out = []
+    for sc, cfg, passes in P.batch_scenes(n=n, seed=seed, max_side=48, min_side=24):
+        cfg = dict(cfg, n_photon=max(2000, cfg["n_photon"] // 40))
+        out.append((sc, cfg, max(2, passes // 32)))
+    return out
+
+
+def _same(a, b):
+    return np.allclose(a, b, rtol=1e-6, atol=0.0) and (a == b).mean() >= 0.99
+
+
+def _cfg(c):
+    return P.pass_cfg(seed=c["seed"], n_photon=c["n_photon"], radius0=c["radius0"], alpha=c["alpha"],
+                      max_depth=c["max_depth"])
+
+
+def _plain_images(c, items, dss):
+    imgs = []
+    for (sc, cfg, passes), ds in zip(items, dss):
+        s = P.Session(c, ds, _cfg(cfg))
+        s.run(passes)
+        m, v, k = s.image()
+        s.close()
+        assert k == passes
+        imgs.append((m, v))
+    return imgs
+
+
+@pytest.mark.parametrize("seed", [3, 11])
+def test_batch_equals_sessions(seed):
+    say("batch == sessions", seed)
+    c = ctx()
+    items = _small_batch(10, seed)
+    dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
+    kinds = {ds.stats()[1] > 0 for ds in dss}
+    plain = _plain_images(c, items, dss)
+    b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    b.run()
+    for i, (sc, cfg, passes) in enumerate(items):
+        m, v, k = b.image(i)
+        assert k == passes
+        assert _same(m, plain[i][0]), i
+        assert np.allclose(v, plain[i][1], rtol=1e-5, atol=1e-12), i
+    t = b.timings()
+    assert t["photons_emitted"] > 0 and t["ms_total"] > 0
+    b.close()
+    print("scenes with a BVH:", kinds)
+
+
+def test_batch_mixed_bvh_and_tuning():
+    """A batch holding scenes with and without a BVH; the autotuner's choice
+    of launch sizes changes the speed, not the images."""
+    say("batch tune")
+    c = ctx()
+    items = _small_batch(12, 5)
+    # make sure both layouts are present: one room with many spheres
+    big = P.batch_scenes(n=40, seed=5, max_side=40, min_side=32)
+    many = [x for x in big if len(x[0].shapes) - 6 >= 80][0]
+    items.append((many[0], dict(many[1], n_photon=20000), 3))
+    dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
+    assert any(ds.stats()[1] > 0 for ds in dss) and any(ds.stats()[1] == 0 for ds in dss)
+    plain = _plain_images(c, items, dss)
+    b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    best = b.tune(rounds=1)
+    assert best["eye_waves_per_cu"] in (4, 8, 16, 32) and best["shade_blocks_per_cu"] in (2, 4, 8, 16)
+    b.run()
+    for i, (_, _, passes) in enumerate(items):
+        m, _, k = b.image(i, with_var=False)
+        assert k == passes and _same(m, plain[i][0]), i
+    # partial runs: rounds continue where they stopped
+    b2 = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    b2.set_launch(eye_waves_per_cu=4, shade_blocks_per_cu=2)
+    b2.run(max_rounds=1)
+    b2.run()
+    for i in range(len(items)):
+        m, _, _ = b2.image(i, with_var=False)
+        assert _same(m, plain[i][0]), i
+    b.close()
+    b2.close()
