@@ -323,6 +323,25 @@ ppm_status  ppm_session_image(ppm_session* s, double* mean_out,
 ppm_status  ppm_session_radius2(ppm_session* s, double* r2_next);
 ppm_status  ppm_session_timings(ppm_session* s, ppm_timings* t);
 
+/* ---- differentiable form (no counterpart in the reference) ---------------- */
+
+/* Set the reflectance c1 of materials 0..n-1 (n x 3 doubles; NULL keeps the
+ * scene's), the survival probabilities q of their Simple diffuse bounce (n x
+ * 3 in (0, 1]; NULL: q = c1), and scale every light's flux and every
+ * emittance by light_scale, all relative to the scene as created.  A
+ * diffuse bounce survives with probability q and weights the photon by c1/q
+ * (tracer.rs:83-92 is the case q = c1, where every weight is 1).  Sessions
+ * created afterwards see the new values. */
+ppm_status  ppm_scene_set_params(ppm_ctx* ctx, ppm_scene* scene, uint32_t n,
+                                 const double* albedo, const double* survival,
+                                 double light_scale);
+/* Backward of the session's passes run so far: for grad_image = dL/d(mean
+ * image) (yreso x xreso x 3), replays the passes from their seeds and
+ * returns dL/dc1 of every material (n_materials x 3 doubles; non-zero for
+ * Simple materials) and, when grad_light is not NULL, dL/d(light_scale). */
+ppm_status  ppm_session_backward(ppm_session* s, const double* grad_image,
+                                 double* grad_albedo, double* grad_light);
+
 /* create + run(cfg->n_pass) + image + timings + destroy. */
 ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
                            const ppm_pass_cfg* cfg, uint32_t n_pass,

@@ -230,6 +230,7 @@ static V generate_random_dir(Rng& rng) {
 // ---------------------------------------------------------------- scene
 struct Material {
   C em, ior, c1, c2;
+  C q;  // survival probability of a diffuse bounce (= c1, the reference's rule, unless set)
   int surface;
   Flt diff, metal, rough, density_pow;
 };
@@ -431,10 +432,19 @@ static int s_next_direction(const Material& m, Flt eta, V n, V v, int wl, Rng& r
 }
 
 // ---------------------------------------------------------------- photons
+// w: the photon's weight, the product of c1/q over its diffuse survivals
+// (1 when q = c1); ev: the materials of those survivals, in order.
+constexpr int MAX_EV = 16;
+struct PathW {
+  Flt w = 1.0;
+  int n = 0;
+  uint16_t ev[MAX_EV] = {};
+};
 struct Photon {
   int wl;
   V pos, dir;
   uint32_t meta;
+  PathW pw;
 };
 
 // Light::generate_photon (light.rs:67-91)
@@ -457,8 +467,11 @@ static void generate_photon(const Light& L, Rng& rng, int& wl, V& o, V& d) {
 // trace_photon (tracer.rs:31-81), reflect_diff (:83-92), reflect_spec
 // (:94-109), reflect_trans (:111-125).  Children are pushed before the
 // parent's own record, as in the reference.
+// With q != c1 a diffuse bounce survives with probability q and multiplies
+// the photon's weight by c1/q (the same expectation; with q = c1 it is the
+// reference's rule and every weight stays 1).
 static void trace_photon(const Scene& sc, bool uc, int maxd, int m0, int l, int wl, V o, V d, uint32_t pid,
-                         Rng& rng, std::vector<Photon>& pcs) {
+                         Rng& rng, std::vector<Photon>& pcs, const PathW& pw) {
   if (l >= maxd) return;
   Isect is;
   if (!calc_intersection(sc, o, d, is)) return;
@@ -466,9 +479,13 @@ static void trace_photon(const Scene& sc, bool uc, int maxd, int m0, int l, int 
   if (M.surface == PPM_SURF_SIMPLE) {
     if (russian_roulette1(s_roughness(M), rng) == 0) {
       // reflect_diff
-      if (russian_roulette1(wlv(M.c1, wl), rng) == 0) {
+      if (russian_roulette1(wlv(M.q, wl), rng) == 0) {
         V dr = diffuse_reflection(is.n, rng);
-        trace_photon(sc, uc, maxd, m0, l + 1, wl, is.pos, dr, pid, rng, pcs);
+        PathW p2 = pw;
+        const Flt a = wlv(M.c1, wl), qq = wlv(M.q, wl);
+        if (a != qq) p2.w = pw.w * (a / qq);
+        if (p2.n < MAX_EV) p2.ev[p2.n++] = (uint16_t)is.mat;
+        trace_photon(sc, uc, maxd, m0, l + 1, wl, is.pos, dr, pid, rng, pcs, p2);
       }
     } else {
       // reflect_spec
@@ -476,7 +493,7 @@ static void trace_photon(const Scene& sc, bool uc, int maxd, int m0, int l, int 
       V rdir = specular_reflection(is.n, d, cos1);
       Flt f = schlick(wlv(M.c2, wl), cos1);
       if (russian_roulette1(f, rng) == 0) {
-        trace_photon(sc, uc, maxd, m0, l + 1, wl, is.pos, rdir, pid, rng, pcs);
+        trace_photon(sc, uc, maxd, m0, l + 1, wl, is.pos, rdir, pid, rng, pcs, pw);
       } else if (wlv(M.ior, wl) != 0.0) {
         // reflect_trans
         Flt eta = relative_ior_wavelength(sc.mats[m0].ior, M.ior, wl);
@@ -484,7 +501,7 @@ static void trace_photon(const Scene& sc, bool uc, int maxd, int m0, int l, int 
         Flt c2;
         if (specular_refraction(is.n, d, eta, tdir, c2)) {
           int m02 = dot(tdir, is.n) < 0.0 ? is.mat : sc.air;
-          trace_photon(sc, uc, maxd, m02, l + 1, wl, is.pos, tdir, pid, rng, pcs);
+          trace_photon(sc, uc, maxd, m02, l + 1, wl, is.pos, tdir, pid, rng, pcs, pw);
         }
       }
     }
@@ -492,11 +509,11 @@ static void trace_photon(const Scene& sc, bool uc, int maxd, int m0, int l, int 
     Flt eta = relative_ior_wavelength(sc.mats[m0].ior, M.ior, wl);
     V nd;
     int r = s_next_direction(M, eta, is.n, d, wl, rng, nd);
-    if (r != 0) trace_photon(sc, uc, maxd, r == 1 ? m0 : is.mat, l + 1, wl, is.pos, nd, pid, rng, pcs);
+    if (r != 0) trace_photon(sc, uc, maxd, r == 1 ? m0 : is.mat, l + 1, wl, is.pos, nd, pid, rng, pcs, pw);
   }
   if ((!uc || l > 0) && s_store(M)) {
     uint32_t meta = (uint32_t)(wl & 3) | ((uint32_t)(l & 31) << 2) | (pid << 7);
-    pcs.push_back(Photon{wl, is.pos, d, meta});
+    pcs.push_back(Photon{wl, is.pos, d, meta, pw});
   }
 }
 
@@ -631,7 +648,7 @@ static C estimate_radiance(const Kd& kd, Flt r2, int pf, V pos, V n, std::vector
     const Photon& p = kd.ph[e.second];
     Flt pw = wt * kd.power;
     Flt cos0 = dot(n, p.dir);
-    Flt pw2 = cos0 < 0.0 ? pw * -cos0 : 0.0;
+    Flt pw2 = cos0 < 0.0 ? pw * -cos0 * p.pw.w : 0.0;
     if (p.wl == 0) rad.r = rad.r + pw2;
     else if (p.wl == 1) rad.g = rad.g + pw2;
     else rad.b = rad.b + pw2;
@@ -707,13 +724,47 @@ struct EyeCtx {
   uint32_t cam_draws;
   const Rng* root;
   std::vector<std::pair<Flt, int>>* buf;
+  const C* g = nullptr;        // dL/d(this pass's pixel), when the gradient is wanted
+  double* grad = nullptr;      // dL/d(c1): 3 per material
 };
+
+// Gradient of one eye node's contribution wpath * wd * di to the pixel.
+// Eye side: wd is proportional to c1 of the node's (Simple) material, so
+// d/dc1 = contribution / c1.  Photon side: each photon in the ball carries
+// the weight prod c1/q over its diffuse survivals, so its share of the
+// contribution differentiates to share / c1 of every survival's material.
+static void node_grad(const EyeCtx& ec, const Isect& is, const Material& mate, C wpath, C wd, C di, Flt r2) {
+  const Scene& sc = *ec.sc;
+  const C& g = *ec.g;
+  double* grad = ec.grad;
+  const C contrib{wpath.r * wd.r * di.r, wpath.g * wd.g * di.g, wpath.b * wd.b * di.b};
+  if (mate.surface == PPM_SURF_SIMPLE) {
+    if (mate.c1.r != 0.0) grad[3 * is.mat + 0] += g.r * contrib.r / mate.c1.r;
+    if (mate.c1.g != 0.0) grad[3 * is.mat + 1] += g.g * contrib.g / mate.c1.g;
+    if (mate.c1.b != 0.0) grad[3 * is.mat + 2] += g.b * contrib.b / mate.c1.b;
+  }
+  const Kd& kd = *ec.kd;
+  for (auto& e : *ec.buf) {
+    const Photon& p = kd.ph[e.second];
+    if (p.pw.n == 0) continue;
+    Flt wt = ec.pf == PPM_FILTER_CONE ? filter_cone(e.first, r2) : (ec.pf == PPM_FILTER_GAUSS ? filter_gauss(e.first, r2) : 1.0);
+    Flt cos0 = dot(is.n, p.dir);
+    if (!(cos0 < 0.0)) continue;
+    const int c = p.wl;
+    const Flt share = wt * kd.power * -cos0 * p.pw.w * (ONE_PI / r2) * wlv(wpath, c) * wlv(wd, c) * wlv(g, c);
+    for (int k = 0; k < p.pw.n; ++k) {
+      const int m = p.pw.ev[k];
+      const Flt a = wlv(sc.mats[m].c1, c);
+      if (a != 0.0) grad[3 * m + c] += share / a;
+    }
+  }
+}
 
 // trace_ray (tracer.rs:129-177).  Node (l, path) draws its random numbers
 // from its own stream; children whose BSDF weight is zero in every channel
 // are not traced (they would be multiplied by zero), nor is di when the
 // diffuse weight is zero.
-static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d) {
+static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d, C wpath = C{1, 1, 1}) {
   const Scene& sc = *ec.sc;
   if (l >= ec.maxd) return C{0, 0, 0};
   Isect is;
@@ -741,13 +792,14 @@ static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d) {
     if (ec.uc)
       for (const Light& lt : sc.lights) di = di + radiance_from_light(sc, is.pos, is.n, lt);
     di = di + estimate_radiance(*ec.kd, ec.r2, ec.pf, is.pos, is.n, *ec.buf);
+    if (ec.grad) node_grad(ec, is, mate, wpath, wd, di, ec.r2);
   }
   C zero{0, 0, 0};
   C f = reflection_index(mate.c2, cos);
   C si{0, 0, 0}, ti{0, 0, 0};
   C ws = mate.surface == PPM_SURF_SIMPLE ? f * (1.0 - mate.diff) : (mate.surface == PPM_SURF_TS ? f : zero);
   if (s_reflect(mate, cos1) && (ws.r != 0.0 || ws.g != 0.0 || ws.b != 0.0))
-    si = trace_ray(ec, m0, l + 1, path, is.pos, rdir);
+    si = trace_ray(ec, m0, l + 1, path, is.pos, rdir, wpath * ws);
   C wt = zero;
   if (mate.surface == PPM_SURF_SIMPLE) {
     Flt k = (1.0 - mate.diff) * (1.0 - mate.metal);
@@ -757,7 +809,7 @@ static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d) {
   }
   if (has_t && s_refract(mate, cos1) && (wt.r != 0.0 || wt.g != 0.0 || wt.b != 0.0)) {
     int m02 = is.in ? is.mat : sc.air;
-    ti = trace_ray(ec, m02, l + 1, path | (1u << l), is.pos, tdir);
+    ti = trace_ray(ec, m02, l + 1, path | (1u << l), is.pos, tdir, wpath * wt);
   }
   return mate.em * SR_HALF + s_bsdf(mate, cos, di, si, ti);
 }
@@ -780,12 +832,13 @@ static void generate_ray(const Camera& cam, int x, int y, Rng& rng, V& o, V& d) 
 }
 
 static C eye_pixel(const Scene& sc, const Kd& kd, uint64_t seed, uint32_t pass, Flt r2, bool uc, int maxd, int pf,
-                   int x, int y, std::vector<std::pair<Flt, int>>& buf) {
+                   int x, int y, std::vector<std::pair<Flt, int>>& buf, const C* g = nullptr,
+                   double* grad = nullptr) {
   uint32_t pixel = (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x;
   Rng root(seed, pixel, pass, TAG_EYE);
   V o, d;
   generate_ray(sc.cam, x, y, root, o, d);
-  EyeCtx ec{&sc, &kd, r2, uc, maxd, pf, seed, pixel, pass, root.n, &root, &buf};
+  EyeCtx ec{&sc, &kd, r2, uc, maxd, pf, seed, pixel, pass, root.n, &root, &buf, g, grad};
   return trace_ray(ec, sc.air, 0, 0u, o, d);
 }
 
@@ -822,7 +875,7 @@ static void trace_all_photons(const Scene& sc, uint64_t seed, uint32_t pass, uin
     int wl;
     V o, d;
     generate_photon(sc.lights[li], rng, wl, o, d);
-    trace_photon(sc, uc, maxd, sc.air, 0, wl, o, d, (uint32_t)i, rng, per[(size_t)i]);
+    trace_photon(sc, uc, maxd, sc.air, 0, wl, o, d, (uint32_t)i, rng, per[(size_t)i], PathW{});
   }
   out.clear();
   for (auto& v : per) out.insert(out.end(), v.begin(), v.end());
@@ -930,12 +983,13 @@ orc_scene* orc_scene_create(const ppm_scene_desc* d) {
     mm.metal = m.metalness;
     mm.rough = m.roughness;
     mm.density_pow = 1.0 / (std::pow(10.0, 5.0 * (1.0 - std::sqrt(m.roughness))) + 1.0);  // surface.rs:52
+    mm.q = mm.c1;
     sc.mats.push_back(mm);
   }
   Material air;  // M_AIR (scene.rs:13-18)
   air.em = C{0, 0, 0};
   air.ior = C{1, 1, 1};
-  air.c1 = air.c2 = C{0, 0, 0};
+  air.c1 = air.c2 = air.q = C{0, 0, 0};
   air.surface = PPM_SURF_NOTHING;
   air.diff = air.metal = air.rough = air.density_pow = 0.0;
   sc.air = (int)sc.mats.size();
@@ -1032,7 +1086,7 @@ orc_map* orc_map_build(uint64_t n, const double* pos, const double* dir, const u
   m->kd.power = power;
   m->kd.ph.resize(n);
   for (uint64_t i = 0; i < n; ++i)
-    m->kd.ph[i] = Photon{wl[i], vd(pos + 3 * i), vd(dir + 3 * i), 0u};
+    m->kd.ph[i] = Photon{wl[i], vd(pos + 3 * i), vd(dir + 3 * i), 0u, PathW{}};
   kd_build(m->kd);
   return m;
 }
@@ -1138,6 +1192,68 @@ int64_t orc_run_passes(const orc_scene* s, const ppm_pass_cfg* cfg, uint32_t n_p
   }
   if (ms_trace) *ms_trace = t_trace;
   if (ms_eye) *ms_eye = t_eye;
+  return stored;
+}
+
+// Material parameters for the differentiable form: c1 (n_mat x 3, NULL:
+// keep), the survival probabilities q of the diffuse bounce (NULL: q = c1),
+// and a factor on every light's flux and every emittance.
+void orc_scene_set_params(orc_scene* s, int n_mat, const double* c1, const double* q, double light_scale) {
+  Scene& sc = s->sc;
+  for (int i = 0; i < n_mat && i < sc.air; ++i) {
+    Material& m = sc.mats[i];
+    if (c1) m.c1 = cd(c1 + 3 * i);
+    m.q = q ? cd(q + 3 * i) : m.c1;
+    m.em = m.em * light_scale;
+  }
+  for (Light& L : sc.lights) L.flux *= light_scale;
+}
+
+// The passes of orc_run_passes with, for dL/d(mean image) g_img, the
+// gradient dL/dc1 of the same estimator on the same random numbers
+// (grad: 3 per material; pass k's image gets g_img / n_pass).
+int64_t orc_run_passes_grad(const orc_scene* s, const ppm_pass_cfg* cfg, uint32_t n_pass, const double* g_img,
+                            double* grad, double* mean, int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  const Scene& sc = s->sc;
+  const int W = sc.cam.xreso, H = sc.cam.yreso;
+  const uint64_t P = (uint64_t)W * H;
+  const size_t NG = 3 * sc.mats.size();
+  std::vector<double> sum(3 * P, 0.0), total(NG, 0.0);
+  std::vector<double> r2s(n_pass);
+  uint32_t first = cfg->first_pass ? cfg->first_pass : 1;
+  orc_radius_schedule(cfg->radius0, cfg->alpha, first, n_pass, r2s.data());
+  int64_t stored = 0;
+  for (uint32_t i = 0; i < n_pass; ++i) {
+    uint32_t pass = first + i;
+    orc_map m;
+    Flt pw;
+    trace_all_photons(sc, cfg->seed, pass, cfg->n_photon, cfg->use_classic != 0, cfg->max_depth, m.kd.ph, pw);
+    m.kd.power = pw;
+    kd_build(m.kd);
+    stored += (int64_t)m.kd.ph.size();
+#pragma omp parallel
+    {
+      std::vector<std::pair<Flt, int>> buf;
+      std::vector<double> gl(NG, 0.0);
+#pragma omp for schedule(dynamic, 16)
+      for (int64_t p = 0; p < (int64_t)P; ++p) {
+        int y = (int)(p / W), x = (int)(p % W);
+        const C g = C{g_img[3 * p], g_img[3 * p + 1], g_img[3 * p + 2]} * (1.0 / n_pass);
+        C c = eye_pixel(sc, m.kd, cfg->seed, pass, r2s[i], cfg->use_classic != 0, cfg->max_depth, cfg->pfilter, x,
+                        y, buf, &g, gl.data());
+        sum[3 * p] += c.r; sum[3 * p + 1] += c.g; sum[3 * p + 2] += c.b;
+      }
+#pragma omp critical
+      for (size_t k = 0; k < NG; ++k) total[k] += gl[k];
+    }
+  }
+  const size_t nm = NG - 3;  // without M_AIR
+  for (size_t k = 0; k < nm; ++k) grad[k] = total[k];
+  if (mean)
+    for (uint64_t k = 0; k < 3 * P; ++k) mean[k] = sum[k] / (double)n_pass;
   return stored;
 }
 

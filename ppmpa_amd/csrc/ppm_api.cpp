@@ -33,6 +33,11 @@ struct ppm_scene {
   std::vector<DevLight> lights;  // host copy (photon counts are filled per call)
   double total_flux = 0.0;
   double bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
+  // as created, for ppm_scene_set_params
+  std::vector<DevMaterial> mats0;
+  std::vector<float> flux0;
+  double total_flux0 = 0.0;
+  double light_scale = 1.0;
 };
 
 struct ppm_photons {
@@ -549,6 +554,7 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
     o.ior = f4(m.ior);
     o.c1 = f4(m.color1);
     o.c2 = f4(m.color2);
+    o.q = o.c1;
   }
   ppm_scene* sc = new ppm_scene();
   sc->ctx = c;
@@ -614,6 +620,9 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
     ppm_scene_destroy(sc);
     return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
   }
+  sc->mats0 = mats;
+  for (const DevLight& l : sc->lights) sc->flux0.push_back(l.flux);
+  sc->total_flux0 = sc->total_flux;
   const ppm_status bs = scene_layout(c, sc, shapes, d->flags);
   if (bs != PPM_OK) {
     ppm_scene_destroy(sc);
@@ -634,6 +643,42 @@ void ppm_scene_destroy(ppm_scene* sc) {
   dfree(sc->buf.bvh);
   dfree(sc->buf.bprims);
   delete sc;
+}
+
+ppm_status ppm_scene_set_params(ppm_ctx* c, ppm_scene* sc, uint32_t n, const double* albedo, const double* survival,
+                                double light_scale) {
+  if (!c || !sc) return fail(c, PPM_E_INVALID, "null argument");
+  const uint32_t nm = (uint32_t)sc->mats0.size() - 1;  // without M_AIR
+  if (n > nm) return fail(c, PPM_E_RANGE, "more parameters than materials");
+  if (!(light_scale >= 0.0) || !std::isfinite(light_scale)) return fail(c, PPM_E_INVALID, "bad light scale");
+  std::vector<DevMaterial> mats = sc->mats0;
+  for (uint32_t i = 0; i < nm; ++i) {
+    DevMaterial& m = mats[i];
+    if (i < n && albedo) {
+      const double* a = albedo + 3 * i;
+      if (!finite3(a)) return fail(c, PPM_E_NONFINITE, "non-finite albedo");
+      m.c1 = f4(a);
+    } else {
+      m.c1 = sc->mats0[i].c1;
+    }
+    m.q = m.c1;
+    if (i < n && survival) {
+      const double* q = survival + 3 * i;
+      for (int k = 0; k < 3; ++k)
+        if (!(q[k] > 0.0 && q[k] <= 1.0)) return fail(c, PPM_E_INVALID, "survival probabilities must lie in (0, 1]");
+      m.q = f4(q);
+    }
+    m.em = make_float4(sc->mats0[i].em.x * (float)light_scale, sc->mats0[i].em.y * (float)light_scale,
+                       sc->mats0[i].em.z * (float)light_scale, 0.0f);
+  }
+  for (size_t i = 0; i < sc->lights.size(); ++i) sc->lights[i].flux = (float)(sc->flux0[i] * light_scale);
+  sc->total_flux = sc->total_flux0 * light_scale;
+  sc->light_scale = light_scale;
+  HIP_TRY(c, hipMemcpy(sc->buf.mats, mats.data(), sizeof(DevMaterial) * mats.size(), hipMemcpyHostToDevice));
+  if (!sc->lights.empty())
+    HIP_TRY(c, hipMemcpy(sc->buf.lights, sc->lights.data(), sizeof(DevLight) * sc->lights.size(),
+                         hipMemcpyHostToDevice));
+  return PPM_OK;
 }
 
 ppm_status ppm_scene_stats(const ppm_scene* sc, uint32_t* n_linear, uint32_t* n_bvh, uint32_t* bvh_depth) {
@@ -777,7 +822,7 @@ ppm_status ppm_photons_upload(ppm_ctx* c, ppm_photons* p, uint64_t n, const doub
     if (!norm3(dd, dn)) return fail(c, PPM_E_INVALID, "zero direction at photon " + std::to_string(i));
     recs[i].pm = make_float4((float)pp[0], (float)pp[1], (float)pp[2],
                              __builtin_bit_cast(float, pack_meta(wl[i], 0, (uint32_t)i)));
-    recs[i].dm = make_float4((float)dn[0], (float)dn[1], (float)dn[2], 0.0f);
+    recs[i].dm = make_float4((float)dn[0], (float)dn[1], (float)dn[2], 1.0f);
     for (int k = 0; k < 3; ++k) {
       lo[k] = std::min(lo[k], pp[k]);
       hi[k] = std::max(hi[k], pp[k]);
@@ -1210,6 +1255,77 @@ ppm_status ppm_session_timings(ppm_session* s, ppm_timings* t) {
   t->gather_queries = h.total_queries + (uint64_t)s->passes_done * 0;
   t->photons_emitted = (uint64_t)s->sbuf.n_emit_total * s->passes_done;
   t->passes = s->passes_done;
+  return PPM_OK;
+}
+
+ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double* grad_albedo, double* grad_light) {
+  if (!s || !grad_image || !grad_albedo) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  const uint32_t K = s->passes_done;
+  if (K == 0) return fail(c, PPM_E_INVALID, "backward before any pass");
+  const ppm_pass_cfg& cfg = s->cfg;
+  const ppm_scene* sc = s->scene;
+  const uint32_t n_pix = s->eb.n_pix;
+  const uint32_t nm = s->sbuf.n_mats;
+  // dL/d(pass image) = dL/d(mean image) / K
+  std::vector<float4> g(n_pix);
+  for (uint32_t p = 0; p < n_pix; ++p) {
+    const double* v = grad_image + 3ull * p;
+    if (!finite3(v)) return fail(c, PPM_E_NONFINITE, "non-finite image gradient");
+    g[p] = make_float4((float)(v[0] / K), (float)(v[1] / K), (float)(v[2] / K), 0.0f);
+  }
+  float4* gimg = nullptr;
+  float *adj_sorted = nullptr, *adj_slot = nullptr;
+  double* grad = nullptr;
+  PassState* st = nullptr;
+  const float4 bmin = make_float4((float)sc->bmin[0], (float)sc->bmin[1], (float)sc->bmin[2], 0.f);
+  const float4 bmax = make_float4((float)sc->bmax[0], (float)sc->bmax[1], (float)sc->bmax[2], 0.f);
+  hipStream_t q = c->stream;
+  hipError_t e;
+  if ((e = dalloc(&gimg, n_pix)) == hipSuccess && (e = dalloc(&adj_sorted, s->map.cap)) == hipSuccess &&
+      (e = dalloc(&adj_slot, s->pb.cap)) == hipSuccess && (e = dalloc(&grad, 3ull * nm)) == hipSuccess &&
+      (e = dalloc(&st, 1)) == hipSuccess &&
+      (e = hipMemcpy(gimg, g.data(), sizeof(float4) * n_pix, hipMemcpyHostToDevice)) == hipSuccess &&
+      (e = hipMemset(grad, 0, sizeof(double) * 3 * nm)) == hipSuccess) {
+    const PassState h = host_state();
+    e = hipMemcpy(st, &h, sizeof(h), hipMemcpyHostToDevice);
+    // replay the session's passes from their seeds (the passes' only
+    // checkpoint), each followed by its adjoint
+    for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
+      if ((e = launch_pass_begin(q, st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
+        break;
+      if ((e = launch_trace_photons(q, s->sbuf, st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
+        break;
+      if ((e = launch_map_build(q, s->map, st, s->pb, bmin, bmax)) != hipSuccess) break;
+      if ((e = launch_eye(q, s->sbuf, st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) != hipSuccess)
+        break;
+      if ((e = launch_gather(q, s->sbuf, s->map, st, s->eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
+                             nullptr)) != hipSuccess)
+        break;
+      if ((e = launch_adjoint(q, s->sbuf, s->map, st, s->pb, s->eb, cfg.pfilter, c->n_cu, cfg.seed,
+                              cfg.use_classic, cfg.max_depth, gimg, adj_sorted, adj_slot, grad)) != hipSuccess)
+        break;
+      e = launch_pass_end(q, st);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(q);
+  }
+  std::vector<double> hg(3ull * nm, 0.0);
+  if (e == hipSuccess) e = hipMemcpy(hg.data(), grad, sizeof(double) * hg.size(), hipMemcpyDeviceToHost);
+  dfree(gimg);
+  dfree(adj_sorted);
+  dfree(adj_slot);
+  dfree(grad);
+  dfree(st);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  for (size_t i = 0; i < hg.size(); ++i) grad_albedo[i] = hg[i];
+  if (grad_light) {
+    // the image is proportional to the light scale: dL/ds = sum g * mean / s
+    std::vector<double> sum(3ull * n_pix);
+    HIP_TRY(c, hipMemcpy(sum.data(), s->sum, sizeof(double) * sum.size(), hipMemcpyDeviceToHost));
+    double acc = 0.0;
+    for (size_t i = 0; i < sum.size(); ++i) acc += grad_image[i] * (sum[i] / K);
+    *grad_light = sc->light_scale > 0.0 ? acc / sc->light_scale : 0.0;
+  }
   return PPM_OK;
 }
 

@@ -107,6 +107,7 @@ hipError_t launch_pass_end(hipStream_t s, PassState* st) {
 struct PhotonState {
   V3 o, d;
   uint32_t wl, med, depth, pid;
+  float w;  // weight: the product of c1/q over the diffuse survivals
 };
 
 // Light::generate_photon for photon `pid` of the pass.
@@ -148,14 +149,16 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
   p.med = sc.n_mats;  // M_AIR
   p.depth = 0;
   p.pid = pid;
+  p.w = 1.0f;
 }
 
 // One step of trace_photon.  Returns whether the photon lives on; `store`
 // reports a photon-map record (position, INCOMING direction, channel).
 template <bool BVH>
 __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int maxd, PhotonState& p, Rng& rng,
-                                              bool& store, V3& spos, uint32_t* stk) {
+                                              bool& store, V3& spos, uint32_t* stk, int& ev_mat) {
   store = false;
+  ev_mat = -1;
   if ((int)p.depth >= maxd) return false;  // tracer.rs:32
   Isect is;
   if (!intersect<BVH>(sc, p.o, p.d, is, stk)) return false;  // tracer.rs:36-39
@@ -168,9 +171,13 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
     // (tracer.rs:51-53, surface.rs:358-367); check_under: 0 iff u <= p.
     float u = rng.next();
     if (u <= M.diff) {
-      // reflect_diff (tracer.rs:83-92)
+      // reflect_diff (tracer.rs:83-92): survive with probability q (the
+      // reference's c1 unless set otherwise) and weight c1/q
       float u2 = rng.next();
-      if (!(u2 <= chan(M.c1, p.wl))) return false;
+      const float qq = chan(M.q, p.wl), a = chan(M.c1, p.wl);
+      if (!(u2 <= qq)) return false;
+      if (a != qq) p.w *= a / qq;
+      ev_mat = (int)is.mat;
       p.d = diffuse_reflection(is.n, rng);
       p.o = is.pos;
       p.depth += 1;
@@ -240,13 +247,49 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
 // + mbcnt, a wave-uniform counter in registers), and stored records go to
 // consecutive slots of the region (coalesced 32-byte writes).  No global
 // atomics: the record order depends only on the data, so it is reproducible.
-template <bool BVH>
+// Adjoint replay (ADJ): the same walk, storing nothing; at each record slot
+// the record's adjoint weight adj[slot] is spread over the lane's diffuse
+// survivals so far (their materials kept in LDS): d(weight)/d(c1) =
+// weight / c1 per survival, so each adds adj / c1 of its material and the
+// record's channel.
+struct AdjTrace {
+  const float* adj;  // per record slot (trace layout)
+  double* grad;      // 3 per material
+};
+constexpr uint32_t ADJ_LDS_MATS = 128;  // materials with per-block LDS partial sums
+constexpr uint32_t ADJ_MAX_EV = MAX_DEPTH_LIMIT;
+
+__device__ __forceinline__ void adj_add(float* part, double* grad, uint32_t m, uint32_t c, float v) {
+  if (m < ADJ_LDS_MATS)
+    atomicAdd(&part[3 * m + c], v);
+  else
+    atomicAdd(&grad[3 * m + c], (double)v);
+}
+
+__device__ __forceinline__ void adj_flush(float* part, double* grad, uint32_t n_mats) {
+  __syncthreads();
+  const uint32_t n = 3u * min(n_mats, ADJ_LDS_MATS);
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+    if (part[i] != 0.0f) atomicAdd(&grad[i], (double)part[i]);
+}
+
+template <bool BVH, bool ADJ = false>
 __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
-                                                       uint64_t seed, int uc, int maxd, Blk blk) {
+                                                       uint64_t seed, int uc, int maxd, Blk blk,
+                                                       AdjTrace at = AdjTrace{nullptr, nullptr}) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
+  __shared__ float part[ADJ ? 3 * ADJ_LDS_MATS : 1];
+  __shared__ uint16_t evl[ADJ ? ADJ_MAX_EV * 256 : 1];
   const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
-  if (w >= n_waves) return;
+  if (ADJ) {
+    for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
+    __syncthreads();
+  }
+  if (w >= n_waves) {
+    if (ADJ) adj_flush(part, at.grad, sc.n_mats);
+    return;
+  }
   const uint32_t pass = st->pass;
   const uint32_t lane = lane_id();
   const uint64_t lo = (uint64_t)w * per_wave;
@@ -259,6 +302,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
   PhotonState p;
   Rng rng;
   p.pid = 0;
+  uint32_t n_ev = 0;
   while (true) {
     const bool need = !alive && !done;
     const uint64_t m = __ballot(need);
@@ -268,6 +312,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
         if (pid < p_end) {
           rng.init(seed, pid, pass, TAG_PHOTON);
           emit_photon(sc, pid, rng, p);
+          n_ev = 0;
           alive = true;
         } else {
           done = true;
@@ -279,21 +324,39 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
     bool store = false;
     V3 spos = mk(0.0f, 0.0f, 0.0f);
     const V3 sdir = p.d;
+    const float sw = p.w;
+    const uint32_t swl = p.wl;
     const uint32_t smeta = pack_meta(p.wl, p.depth, p.pid);
-    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, stk + threadIdx.x);
+    int ev_mat = -1;
+    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, stk + threadIdx.x, ev_mat);
     const uint64_t sm = __ballot(store);
     if (sm) {
       if (store) {
         const uint32_t slot = cnt + (uint32_t)__popcll(sm & lanemask_lt());
         if (slot < region_cap) {
-          PhotonRec r;
-          r.pm = make_float4(spos.x, spos.y, spos.z, __uint_as_float(smeta));
-          r.dm = make_float4(sdir.x, sdir.y, sdir.z, 0.0f);
-          region[slot] = r;
+          if (ADJ) {
+            const float A = at.adj[(uint64_t)w * region_cap + slot];
+            if (A != 0.0f)
+              for (uint32_t k = 0; k < n_ev; ++k) {
+                const uint32_t mi = evl[k * 256u + threadIdx.x];
+                const float a = chan(sc.mats[mi].c1, swl);
+                if (a != 0.0f) adj_add(part, at.grad, mi, swl, A / a);
+              }
+          } else {
+            PhotonRec r;
+            r.pm = make_float4(spos.x, spos.y, spos.z, __uint_as_float(smeta));
+            r.dm = make_float4(sdir.x, sdir.y, sdir.z, sw);
+            region[slot] = r;
+          }
         }
       }
       cnt += (uint32_t)__popcll(sm);
     }
+    if (ADJ && ev_mat >= 0 && n_ev < ADJ_MAX_EV) evl[(n_ev++) * 256u + threadIdx.x] = (uint16_t)ev_mat;
+  }
+  if (ADJ) {
+    adj_flush(part, at.grad, sc.n_mats);
+    return;
   }
   if (lane == 0) {
     if (cnt > region_cap) atomicOr(&st->overflow, 1u);
@@ -1034,7 +1097,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
         if (slot != 0xFFFFFFFFu) {
           Query qq;
           qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
-          qq.b = make_float4(is.n.x, is.n.y, is.n.z, 0.0f);
+          qq.b = make_float4(is.n.x, is.n.y, is.n.z, __uint_as_float(is.mat));
           qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
           eb.q[slot] = qq;
         }
@@ -1176,7 +1239,7 @@ __device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const Photo
     const float d2 = dist2_nc(qx, qy, qz, pm.x, pm.y, pm.z);
     if (d2 <= r2) {
       const float c = nx * dm.x + ny * dm.y + nz * dm.z;
-      const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) : 0.0f;
+      const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
       const uint32_t wl = __float_as_uint(pm.w) & 3u;
       a.r += wl == 0 ? v : 0.0f;
       a.g += wl == 1 ? v : 0.0f;
@@ -1297,7 +1360,7 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const Node* nodes, co
             if (d2 <= r2) {
               const float4 dm = lds[wib].dm[j];
               const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
-              const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) : 0.0f;
+              const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
               const uint32_t wl = __float_as_uint(pm.w) & 3u;
               ar += wl == 0 ? v : 0.0f;
               ag += wl == 1 ? v : 0.0f;
@@ -1324,6 +1387,144 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, 
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc) {
   k_shade_body<BVH>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+}
+
+// ===========================================================================
+// Adjoint of one pass with respect to the Simple materials' c1, for
+// gimg = dL/d(this pass's image).  Per sorted query (pixel p, material m,
+// weight w = path throughput * c1/pi * diffuseness, result res = w * di):
+//   eye side:    dL/dc1[m] += gimg[p] * res / c1[m]  (res is linear in c1[m]);
+//   photon side: each photon j in the ball adds to its adjoint weight
+//                adj[j] += gimg[p][wl_j] * w[wl_j] * scale * max(0,-n.dir_j)
+//                          * filter * weight_j,
+// the photon's share of dL; the replay of the photon walks then turns
+// adj[j] into dL/dc1 over the diffuse survivals before record j.
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const Node* nodes, const PhotonRec* ph,
+                                                    const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
+                                                    const float4* gimg, float* adj, double* grad) {
+  __shared__ LeafLds lds[4];
+  __shared__ float part[3 * ADJ_LDS_MATS];
+  for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
+  __syncthreads();
+  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
+  const uint32_t n = st->n_stored, depth = st->depth;
+  const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
+  const uint32_t nq = st->n_qsort;
+  const uint32_t groups = (nq + 63) / 64;
+  const uint32_t waves = gridDim.x * 4;
+  const uint32_t first_leaf = (1u << depth) - 1u;
+  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
+    const uint32_t i = g * 64 + lane;
+    const bool active = i < nq;
+    Query qq{};
+    float4 gp = make_float4(0.f, 0.f, 0.f, 0.f), res = gp;
+    if (active) {
+      const uint32_t slot = order[i];
+      qq = eb.q[slot];
+      res = eb.res[slot];
+      gp = gimg[__float_as_uint(qq.a.w)];
+      const uint32_t m = __float_as_uint(qq.b.w);
+      const DevMaterial& M = sc.mats[m];
+      if (M.surface == SURF_SIMPLE) {
+        if (M.c1.x != 0.0f) adj_add(part, grad, m, 0, gp.x * res.x / M.c1.x);
+        if (M.c1.y != 0.0f) adj_add(part, grad, m, 1, gp.y * res.y / M.c1.y);
+        if (M.c1.z != 0.0f) adj_add(part, grad, m, 2, gp.z * res.z / M.c1.z);
+      }
+    }
+    // per channel: gimg * w * scale, the factor on each photon's term
+    const float fr = gp.x * qq.w.x * scale, fg = gp.y * qq.w.y * scale, fb = gp.z * qq.w.z * scale;
+    const bool want = active && (fr != 0.0f || fg != 0.0f || fb != 0.0f);
+    const float qx = qq.a.x, qy = qq.a.y, qz = qq.a.z;
+    const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+    uint32_t node = 0;
+    while (n > 0) {
+      node = __builtin_amdgcn_readfirstlane(node);
+      const Node nd = ldc(nodes, node);
+      const bool ov = want && box_dist2_nc(qx, qy, qz, nd.lo, nd.hi) <= r2;
+      if (__ballot(ov)) {
+        if (node < first_leaf) {
+          node = 2 * node + 1;
+          continue;
+        }
+        const uint32_t base = (node - first_leaf) * LEAF;
+        const uint32_t cnt = base < n ? min((uint32_t)LEAF, n - base) : 0u;
+        if (lane < cnt) {
+          const PhotonRec r = ph[base + lane];
+          lds[wib].pm[lane] = r.pm;
+          lds[wib].dm[lane] = r.dm;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ov) {
+          for (uint32_t j = 0; j < cnt; ++j) {
+            const float4 pm = lds[wib].pm[j];
+            const float d2 = dist2_nc(qx, qy, qz, pm.x, pm.y, pm.z);
+            if (d2 <= r2) {
+              const float4 dm = lds[wib].dm[j];
+              const float c = nx * dm.x + ny * dm.y + nz * dm.z;
+              if (c < 0.0f) {
+                const uint32_t wl = __float_as_uint(pm.w) & 3u;
+                const float f = wl == 0 ? fr : (wl == 1 ? fg : fb);
+                const float a = f * (-c * filter_weight(pf, d2 * inv_r2) * dm.w);
+                if (a != 0.0f) atomicAdd(&adj[base + j], a);
+              }
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      while (node != 0 && (node & 1u) == 0) node = (node - 1) >> 1;
+      if (node == 0) break;
+      node += 1;
+    }
+  }
+  adj_flush(part, grad, sc.n_mats);
+}
+
+// Photon adjoint weights from map order to trace-slot order.
+__global__ __launch_bounds__(256) void k_adj_permute(const uint32_t* src_idx, const float* adj_sorted,
+                                                     float* adj_slot, const PassState* st) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < st->n_stored) adj_slot[src_idx[e]] = adj_sorted[e];
+}
+
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_adj_trace(SceneView sc, PassState* st, uint32_t n_waves, uint32_t per_wave,
+                                                   uint32_t region_cap, uint64_t seed, int uc, int maxd,
+                                                   const float* adj_slot, double* grad) {
+  k_trace_photons_body<BVH, true>(sc, st, nullptr, nullptr, n_waves, per_wave, region_cap, seed, uc, maxd,
+                                  Blk{blockIdx.x, gridDim.x}, AdjTrace{adj_slot, grad});
+}
+
+hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const PhotonBuf& pb,
+                          const EyeBuf& eb, int pfilter, int n_cu, uint64_t seed, int use_classic, int max_depth,
+                          const float4* gimg, float* adj_sorted, float* adj_slot, double* grad) {
+  hipError_t e;
+  if ((e = hipMemsetAsync(adj_sorted, 0, sizeof(float) * m.cap, s)) != hipSuccess) return e;
+  if ((e = hipMemsetAsync(adj_slot, 0, sizeof(float) * pb.cap, s)) != hipSuccess) return e;
+  const SceneView v = make_view(sc);
+  const uint32_t blocks = (uint32_t)n_cu * 8u;
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_adj_gather<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb,
+                       eb.qvals[0], pfilter, gimg, adj_sorted, grad);
+  else
+    hipLaunchKernelGGL(k_adj_gather<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb,
+                       eb.qvals[0], pfilter, gimg, adj_sorted, grad);
+  hipLaunchKernelGGL(k_adj_permute, dim3((uint32_t)((m.cap + 255) / 256)), dim3(256), 0, s, m.src_idx, adj_sorted,
+                     adj_slot, (const PassState*)st);
+  const uint32_t n_waves = pb.n_regions;
+  const uint32_t per_wave = (uint32_t)((sc.n_emit_total + (uint64_t)n_waves - 1) / n_waves);
+  if (sc.bvh)
+    hipLaunchKernelGGL(k_adj_trace<true>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves, per_wave,
+                       pb.region_cap, seed, use_classic, max_depth, (const float*)adj_slot, grad);
+  else
+    hipLaunchKernelGGL(k_adj_trace<false>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves, per_wave,
+                       pb.region_cap, seed, use_classic, max_depth, (const float*)adj_slot, grad);
+  return hipGetLastError();
 }
 
 // Overflow results into their pixels (float atomics; few per pass).
@@ -1486,7 +1687,7 @@ __global__ __launch_bounds__(256) void k_query(const Node* nodes, const PhotonRe
                    ++cnt;
                    if (mode == 2) {
                      const float c = nn.x * dm.x + nn.y * dm.y + nn.z * dm.z;
-                     const float v = c < 0.0f ? -c * filter_weight(pf, t) : 0.0f;
+                     const float v = c < 0.0f ? -c * filter_weight(pf, t) * dm.w : 0.0f;
                      const uint32_t wl = __float_as_uint(pm.w) & 3u;
                      ar += wl == 0 ? v : 0.0f;
                      ag += wl == 1 ? v : 0.0f;

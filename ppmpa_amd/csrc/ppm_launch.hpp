@@ -159,6 +159,14 @@ enum BatchKind : int {
 
 float4 morton_scale(float4 bmin, float4 bmax, uint32_t qmax);
 
+// Adjoint of the pass just run on these buffers (eye queries, their results,
+// the map), for gimg = dL/d(the pass's image) per pixel: adds dL/dc1 of the
+// Simple materials into grad (3 per material).  Not graph-capturable (it
+// clears its scratch with hipMemsetAsync).
+hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const PhotonBuf& pb,
+                          const EyeBuf& eb, int pfilter, int n_cu, uint64_t seed, int use_classic, int max_depth,
+                          const float4* gimg, float* adj_sorted, float* adj_slot, double* grad);
+
 // One pass of the first n scenes of a batch (scenes sorted by pass count,
 // so the scenes still running are a prefix); blocks[k] = tab[k*(n_all+1)+n].
 hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32_t* tab, const uint32_t* blocks,

@@ -81,6 +81,7 @@ struct alignas(16) DevMaterial {
   float4 ior;
   float4 c1;  // reflectance / albedo_diff
   float4 c2;  // specular_refl / albedo_spec
+  float4 q;   // survival probability of a Simple diffuse bounce (= c1 unless set)
 };
 
 struct alignas(16) DevLight {
@@ -105,8 +106,9 @@ struct alignas(16) DevCamera {
 };
 
 // --- photon record (32 B in HBM; 28 B of payload) ---------------------------
-// pm = (x, y, z, meta), dm = (dx, dy, dz, 0); meta bits 0-1 channel,
-// 2-6 depth, 7-31 photon index.
+// pm = (x, y, z, meta), dm = (dx, dy, dz, w); meta bits 0-1 channel,
+// 2-6 depth, 7-31 photon index; w = the photon's weight, the product of
+// c1/q over its diffuse survivals (exactly 1 when q = c1).
 struct alignas(16) PhotonRec {
   float4 pm;
   float4 dm;
@@ -122,7 +124,7 @@ struct alignas(16) Node {
 };
 
 // --- gather query (48 B) -----------------------------------------------------
-// a = (x, y, z, pixel), b = (nx, ny, nz, 0), w = (wr, wg, wb, 0)
+// a = (x, y, z, pixel), b = (nx, ny, nz, material), w = (wr, wg, wb, 0)
 struct alignas(16) Query {
   float4 a;
   float4 b;

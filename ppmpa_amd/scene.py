@@ -496,6 +496,62 @@ def batch_scenes(n=64, seed=1, max_side=512, min_side=128):
     return out
 
 
+N_INVERSE_MATERIALS = 32
+
+
+def inverse_albedos(seed):
+    """32 RGB albedos drawn from U(0.1, 0.9) by a Philox stream keyed by seed."""
+    rng = np.random.Generator(np.random.Philox(key=int(seed) + 11))
+    return rng.uniform(0.1, 0.9, (N_INVERSE_MATERIALS, 3))
+
+
+def inverse_scene(albedo=None, xreso=512, yreso=512, seed=1, walls="quad") -> Scene:
+    """The differentiable workload's scene: the caustic scene's box, mirror and
+    glass spheres, with 32 Lambertian materials as the unknowns: materials
+    0-2 are the walls (floor, ceiling and back share 0; left 1; right 2) and
+    3-31 are 29 spheres of radius 0.05 at seeded non-overlapping places.
+    albedo: 32 x 3 (default U(0.1, 0.9) from `seed`).  Materials 0-31 come
+    first, so a gradient over the scene's materials starts with them."""
+    alb = inverse_albedos(seed) if albedo is None else np.asarray(albedo, np.float64).reshape(N_INVERSE_MATERIALS, 3)
+    sc = Scene()
+    mats = [sc.add_material(lambertian(tuple(alb[i]))) for i in range(N_INVERSE_MATERIALS)]
+    if walls == "plane":
+        sc.shapes += [plain((0, 1, 0), 0.0, mats[0]), plain((0, -1, 0), 1.0, mats[0]), plain((0, 0, -1), 1.0, mats[0]),
+                      plain((1, 0, 0), 0.0, mats[1]), plain((-1, 0, 0), 1.0, mats[2])]
+        sc.bounds_min, sc.bounds_max = TUNNEL_BOUNDS
+    else:
+        sc.shapes += [
+            parallelogram((0, 0, 0), (0, 0, 1), (1, 0, 0), mats[0]),
+            parallelogram((0, 1, 0), (1, 1, 0), (0, 1, 1), mats[0]),
+            parallelogram((0, 0, 1), (0, 1, 1), (1, 0, 1), mats[0]),
+            parallelogram((0, 0, 0), (0, 1, 0), (0, 0, 1), mats[1]),
+            parallelogram((1, 0, 0), (1, 0, 1), (1, 1, 0), mats[2]),
+        ]
+        sc.bounds_min, sc.bounds_max = BOX_BOUNDS
+    s = 0.25
+    x0 = 0.5 - s / 2
+    p0 = (x0, 0.999, x0)
+    em = 2.0 * 100.0 / (s * s)
+    lm = sc.add_material(simple((0, 0, 0), (0.8, 0.8, 0.8), 0.0, 0.0, 0.0, emittance=(em, em, em)))
+    sc.shapes.append(parallelogram(p0, (x0 + s, 0.999, x0), (x0, 0.999, x0 + s), lm, nvec=(0, -1, 0)))
+    sc.lights.append(parallelogram_light((1, 1, 1), 100.0, p0, (0, -1, 0), (s, 0, 0), (0, 0, s)))
+    mm = sc.add_material(mirror())
+    gm = sc.add_material(glass(1.5))
+    big = [(np.array([0.75, 0.2, 0.75]), 0.2), (np.array([0.5, 0.2, 0.3]), 0.2)]
+    sc.shapes += [sphere(big[0][0], 0.2, mm), sphere(big[1][0], 0.2, gm)]
+    rng = np.random.Generator(np.random.Philox(key=int(seed) + 13))
+    placed = []
+    while len(placed) < N_INVERSE_MATERIALS - 3:
+        c = rng.uniform(0.06, 0.94, 3)
+        c[1] = rng.uniform(0.05, 0.8)
+        if all(np.linalg.norm(c - q) > 0.05 + r + 0.01 for q, r in big + [(p, 0.05) for p in placed]):
+            placed.append(c)
+    for i, c in enumerate(placed):
+        sc.shapes.append(sphere(c, 0.05, mats[3 + i]))
+    sc.camera = box_camera(xreso, yreso)
+    return sc
+
+
 def reference_room() -> Scene:
     """A room in the style of the reference's built-in scene (scene.rs:20-447):
     4 x 4 x 11 m walls of Plain shapes, a 1.33 m ceiling parallelogram light of
@@ -533,4 +589,6 @@ CONFIGS = {
                     max_depth=16, seed=1),
     "mesh": dict(scene=mesh_scene, n_photon=8_000_000, passes=256, radius0=0.005, alpha=2.0 / 3.0, max_depth=16,
                  seed=1),
+    "inverse": dict(scene=inverse_scene, n_photon=1_000_000, passes=64, radius0=0.005, alpha=2.0 / 3.0, max_depth=16,
+                    seed=1),
 }

@@ -78,6 +78,19 @@ class DeviceScene:
     def pixels(self):
         return self.scene.camera.xreso * self.scene.camera.yreso
 
+    def set_params(self, albedo=None, survival=None, light_scale=1.0):
+        """Reflectance c1 of materials 0..n-1 (n x 3), the survival
+        probabilities q of their diffuse bounce (None: q = c1) and a factor on
+        every light flux and emittance, relative to the scene as created
+        (ppm_scene_set_params).  Sessions created afterwards see them."""
+        a = None if albedo is None else np.ascontiguousarray(albedo, np.float64).reshape(-1, 3)
+        q = None if survival is None else np.ascontiguousarray(survival, np.float64).reshape(-1, 3)
+        n = len(a) if a is not None else (len(q) if q is not None else 0)
+        if a is not None and q is not None and len(q) != len(a):
+            raise ValueError("albedo and survival need the same number of materials")
+        self.ctx.check(self.ctx.lib.ppm_scene_set_params(self.ctx.h, self.h, n, _dp(a) if a is not None else None,
+                                                         _dp(q) if q is not None else None, float(light_scale)))
+
     def stats(self):
         """(shapes tested one by one, shapes in the BVH, BVH leaf level)."""
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
@@ -294,6 +307,17 @@ class Session:
         r = C.c_double()
         self.ctx.check(self.ctx.lib.ppm_session_radius2(self.h, C.byref(r)))
         return r.value
+
+    def backward(self, grad_image):
+        """dL/dc1 of every material (n_materials x 3) and dL/d(light_scale)
+        for grad_image = dL/d(mean image), by replaying the passes run so far
+        (ppm_session_backward)."""
+        cam = self.dscene.scene.camera
+        g = np.ascontiguousarray(grad_image, np.float64).reshape(cam.yreso, cam.xreso, 3)
+        ga = np.zeros((len(self.dscene.scene.materials), 3))
+        gl = C.c_double()
+        self.ctx.check(self.ctx.lib.ppm_session_backward(self.h, _dp(g), _dp(ga), C.byref(gl)))
+        return ga, gl.value
 
     def timings(self):
         t = A.Timings()

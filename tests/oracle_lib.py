@@ -51,6 +51,8 @@ _PROTOS = {
     "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, dp]),
     "orc_run_passes": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, C.c_int, dp, dp]),
     "orc_max_threads": (C.c_int, []),
+    "orc_scene_set_params": (None, [vp, C.c_int, dp, dp, C.c_double]),
+    "orc_run_passes_grad": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, dp, C.c_int]),
 }
 
 _lib = None
@@ -122,6 +124,25 @@ class OScene:
         out = np.zeros((cam.yreso, cam.xreso, 3))
         lib().orc_trace_eye(self.h, omap.h, seed, pass_, r2, int(uc), maxd, pf, _d(out))
         return out
+
+    def set_params(self, c1=None, q=None, light_scale=1.0):
+        """c1 (n_mat x 3) and survival probabilities q of the diffuse bounce
+        (None: q = c1); light_scale multiplies every flux and emittance."""
+        n = len(self.scene.materials)
+        c1a = None if c1 is None else np.ascontiguousarray(c1, np.float64).reshape(n, 3)
+        qa = None if q is None else np.ascontiguousarray(q, np.float64).reshape(n, 3)
+        lib().orc_scene_set_params(self.h, n, _d(c1a) if c1a is not None else None,
+                                   _d(qa) if qa is not None else None, float(light_scale))
+
+    def run_passes_grad(self, cfg, n_pass, g_img, threads=0):
+        """dL/dc1 (n_mat x 3) of the passes' estimator for dL/d(mean image)
+        g_img, and the mean image."""
+        cam = self.scene.camera
+        g = np.ascontiguousarray(g_img, np.float64).reshape(cam.yreso, cam.xreso, 3)
+        grad = np.zeros((len(self.scene.materials), 3))
+        mean = np.zeros((cam.yreso, cam.xreso, 3))
+        lib().orc_run_passes_grad(self.h, C.byref(cfg), n_pass, _d(g), _d(grad), _d(mean), threads)
+        return grad, mean
 
     def run_passes(self, cfg, n_pass, threads=0):
         cam = self.scene.camera

@@ -1,0 +1,128 @@
+"""GPU: the adjoint of the PPM-PA passes with respect to the Lambertian
+albedos and the light power, through ppm_session_backward.
+
+Checks:
+- with q = c1 set explicitly, the forward image is the reference estimator's
+  (bit-identical to a scene never touched by set_params);
+- central finite differences (h = 1e-3) on common random numbers with the
+  survival probabilities q held at the base albedos, where the image is a
+  smooth function of the albedos: within 1e-3 relative.  The loss is the
+  inverse problem's, L = sum (I - T)^2 with T rendered from another albedo
+  set on the same seed, so pixels that see the light (radiance ~500, the
+  same in I and T) carry no weight; with a flat target their float32
+  rounding (ulp ~3e-5), times I - T ~ 1e3, would swamp the difference;
+- the light-scale gradient (the image is linear in it) against a finite
+  difference;
+- the FP64 oracle's gradient of the same estimator on the same streams.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_common import ctx, say
+
+import ppmpa_amd as P
+
+pytestmark = pytest.mark.gpu
+
+RES = 40
+N_PASS = 2
+
+
+def _scene():
+    alb = P.inverse_albedos(3)
+    return P.inverse_scene(alb, RES, RES, seed=3), alb
+
+
+def _cfg():
+    return P.pass_cfg(seed=9, n_photon=20000, radius0=0.02, alpha=2 / 3, max_depth=8)
+
+
+def _render(c, ds, alb, q, scale=1.0):
+    ds.set_params(albedo=alb, survival=q, light_scale=scale)
+    s = P.Session(c, ds, _cfg())
+    s.run(N_PASS)
+    m, _, _ = s.image(with_var=False)
+    return s, m
+
+
+def test_set_params_keeps_the_reference_estimator():
+    say("adjoint: set_params identity")
+    c = ctx()
+    sc, alb = _scene()
+    ds0 = P.DeviceScene(c, sc)
+    s0 = P.Session(c, ds0, _cfg())
+    s0.run(N_PASS)
+    m0, _, _ = s0.image(with_var=False)
+    s0.close()
+    ds = P.DeviceScene(c, sc)
+    s, m1 = _render(c, ds, alb, alb)
+    s.close()
+    assert np.array_equal(m0, m1)
+
+
+def test_adjoint_matches_central_differences():
+    say("adjoint vs FD")
+    c = ctx()
+    sc, alb = _scene()
+    ds = P.DeviceScene(c, sc)
+    hidden = P.inverse_albedos(7)
+    st, target = _render(c, ds, hidden, hidden)
+    st.close()
+    s, mean = _render(c, ds, alb, alb)
+    g_img = 2.0 * (mean - target)
+    grad, gl = s.backward(g_img)
+    s.close()
+    h = 1e-3
+    checked = 0
+    worst = 0.0
+    def fd(m, ch, step):
+        up, dn = alb.copy(), alb.copy()
+        up[m, ch] += step
+        dn[m, ch] -= step
+        s1, ip = _render(c, ds, up, alb)
+        s1.close()
+        s2, im = _render(c, ds, dn, alb)
+        s2.close()
+        # L+ - L- summed per pixel: no cancellation between large totals
+        return float(((ip - im) * (ip + im - 2 * target)).sum()) / (2 * step)
+
+    for (m, ch) in [(0, 0), (0, 1), (1, 0), (2, 2), (3, 0), (9, 1), (17, 2), (28, 0), (31, 1)]:
+        f1 = fd(m, ch, h)
+        if abs(f1) < 1e-3 * np.abs(grad[:32]).max():
+            continue
+        rel = abs(f1 - grad[m, ch]) / abs(f1)
+        worst = max(worst, rel)
+        print(f"param ({m},{ch}) albedo {alb[m, ch]:.3f}: fd {f1:.7g} adjoint {grad[m, ch]:.7g} rel {rel:.2e}")
+        assert rel <= 1e-3, (m, ch, f1, grad[m, ch])
+        checked += 1
+    assert checked >= 5
+    # light scale: the image is linear in it
+    hl = 1e-3
+    s1, ip = _render(c, ds, alb, alb, 1.0 + hl)
+    s1.close()
+    s2, im = _render(c, ds, alb, alb, 1.0 - hl)
+    s2.close()
+    fdl = float(((ip - im) * (ip + im - 2 * target)).sum()) / (2 * hl)
+    assert abs(fdl - gl) <= 1e-3 * abs(fdl), (fdl, gl)
+    print("worst albedo rel", worst, "light", fdl, gl)
+
+
+def test_adjoint_against_fp64_oracle():
+    say("adjoint vs oracle")
+    c = ctx()
+    sc, alb = _scene()
+    ds = P.DeviceScene(c, sc)
+    target = np.full((RES, RES, 3), 0.5)
+    s, mean = _render(c, ds, alb, alb)
+    g_img = 2.0 * (mean - target)
+    grad, _ = s.backward(g_img)
+    s.close()
+    osc = O.OScene(sc)
+    ograd, omean = osc.run_passes_grad(_cfg(), N_PASS, g_img)
+    a, b = grad[:32].ravel(), ograd[:32].ravel()
+    rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print("gpu vs oracle gradient: rel L2", rel, "image rel", np.abs(mean - omean).sum() / np.abs(omean).sum())
+    assert rel < 2e-2
+    big = np.abs(b) > 0.05 * np.abs(b).max()
+    assert np.all(np.sign(a[big]) == np.sign(b[big]))
