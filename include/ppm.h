@@ -335,6 +335,10 @@ ppm_status  ppm_session_timings(ppm_session* s, ppm_timings* t);
 ppm_status  ppm_scene_set_params(ppm_ctx* ctx, ppm_scene* scene, uint32_t n,
                                  const double* albedo, const double* survival,
                                  double light_scale);
+/* Rewind a session to before its first pass (sums cleared), taking up the
+ * scene's current parameters (ppm_scene_set_params), so that a parameter
+ * loop reuses the session's buffers.  No counterpart in the reference. */
+ppm_status  ppm_session_reset(ppm_session* s);
 /* Backward of the session's passes run so far: for grad_image = dL/d(mean
  * image) (yreso x xreso x 3), replays the passes from their seeds and
  * returns dL/dc1 of every material (n_materials x 3 doubles; non-zero for

@@ -188,6 +188,7 @@ PROTOTYPES = {
     "ppm_run_passes": (st, [vp, vp, C.POINTER(PassCfg), C.c_uint32, dp, dp, C.POINTER(Timings)]),
     "ppm_scene_set_params": (st, [vp, vp, C.c_uint32, dp, dp, C.c_double]),
     "ppm_session_backward": (st, [vp, dp, dp, dp]),
+    "ppm_session_reset": (st, [vp]),
     "ppm_batch_create": (st, [vp, C.c_uint32, C.POINTER(vp), C.POINTER(PassCfg), u32p, C.POINTER(vp)]),
     "ppm_batch_destroy": (None, [vp]),
     "ppm_batch_run": (st, [vp, C.c_uint32]),

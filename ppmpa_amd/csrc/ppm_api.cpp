@@ -1258,6 +1258,34 @@ ppm_status ppm_session_timings(ppm_session* s, ppm_timings* t) {
   return PPM_OK;
 }
 
+ppm_status ppm_session_reset(ppm_session* s) {
+  if (!s) return PPM_E_INVALID;
+  ppm_ctx* c = s->ctx;
+  const ppm_scene* sc = s->scene;
+  // the scene's lights as they are now (ppm_scene_set_params may have
+  // scaled them): same photon counts, new photon power
+  std::vector<DevLight> L = sc->lights;
+  double power = 0.0;
+  const uint64_t total = assign_photons(L, sc->total_flux, s->cfg.n_photon, &power);
+  if (total != s->sbuf.n_emit_total) return fail(c, PPM_E_INVALID, "the scene's photon counts changed; create a new session");
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(s->lights, L.data(), sizeof(DevLight) * L.size(), hipMemcpyHostToDevice));
+  s->power = power;
+  const uint64_t n3 = 3ull * s->eb.n_pix;
+  HIP_TRY(c, hipMemset(s->sum, 0, sizeof(double) * n3));
+  HIP_TRY(c, hipMemset(s->sumsq, 0, sizeof(double) * n3));
+  const PassState h = host_state();
+  HIP_TRY(c, hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
+  // the captured pass has the photon power as an argument: capture again
+  if (s->exec) (void)hipGraphExecDestroy(s->exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->exec = nullptr;
+  s->graph = nullptr;
+  s->passes_done = 0;
+  s->t = ppm_timings{};
+  return PPM_OK;
+}
+
 ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double* grad_albedo, double* grad_light) {
   if (!s || !grad_image || !grad_albedo) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;

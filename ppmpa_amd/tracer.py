@@ -308,6 +308,11 @@ class Session:
         self.ctx.check(self.ctx.lib.ppm_session_radius2(self.h, C.byref(r)))
         return r.value
 
+    def reset(self):
+        """Back to before the first pass, with the scene's current parameters
+        (ppm_session_reset)."""
+        self.ctx.check(self.ctx.lib.ppm_session_reset(self.h))
+
     def backward(self, grad_image):
         """dL/dc1 of every material (n_materials x 3) and dL/d(light_scale)
         for grad_image = dL/d(mean image), by replaying the passes run so far

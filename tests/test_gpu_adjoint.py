@@ -126,3 +126,46 @@ def test_adjoint_against_fp64_oracle():
     assert rel < 2e-2
     big = np.abs(b) > 0.05 * np.abs(b).max()
     assert np.all(np.sign(a[big]) == np.sign(b[big]))
+
+
+def test_autograd_adam_recovers_albedos():
+    """The torch.autograd.Function pair: gradients equal the session's
+    backward, and a few Adam steps from a wrong guess lower the loss and move
+    the albedos towards the hidden ones."""
+    import torch
+
+    from ppmpa_amd.autograd import Renderer
+    say("adjoint: autograd + adam")
+    c = ctx()
+    hidden = P.inverse_albedos(7)
+    sc = P.inverse_scene(hidden, RES, RES, seed=3)
+    ds = P.DeviceScene(c, sc)
+    r = Renderer(c, ds, _cfg(), N_PASS)
+    with torch.no_grad():
+        target = r(torch.tensor(hidden), torch.tensor(1.0, dtype=torch.float64))
+    x = torch.full((32, 3), 0.5, dtype=torch.float64, requires_grad=True)
+    light = torch.tensor(1.0, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([x], lr=0.03)
+    losses, errs = [], []
+    for step in range(12):
+        opt.zero_grad()
+        img = r(x, light)
+        loss = ((img - target) ** 2).mean()
+        loss.backward()
+        if step == 0:
+            # autograd's gradient is the session backward of dL/dI
+            g_img = (2.0 * (img - target) / img.numel()).detach().numpy()
+            s, _ = _render(c, ds, x.detach().numpy(), np.clip(x.detach().numpy(), 1e-6, 1.0))
+            ga, gl = s.backward(g_img)
+            s.close()
+            assert np.allclose(x.grad.numpy(), ga[:32], rtol=1e-6, atol=1e-12)
+            assert abs(light.grad.item() - gl) <= 1e-6 * abs(gl)
+        opt.step()
+        with torch.no_grad():
+            x.clamp_(0.02, 0.98)
+        losses.append(loss.item())
+        errs.append(float(np.sqrt(((x.detach().numpy() - hidden) ** 2).mean())))
+    r.close()
+    print("losses", [f"{v:.4g}" for v in losses], "albedo rms error", [f"{v:.3f}" for v in errs])
+    assert losses[-1] < 0.5 * losses[0]
+    assert errs[-1] < errs[0]
