@@ -12,7 +12,7 @@
  *  - Every call returns a ppm_status (0 = ok, < 0 = error); nothing aborts.
  *    ppm_last_error() returns a text description of the last failure on a
  *    context.  (The reference panics instead: photonmap.rs:26,71 unwrap on
- *    non-finite kd coordinates; algebra.rs:220, camera.rs:167 on zero-length
+ *    non-finite kd coordinates; algebra.rs:220, camera.rs:74 on zero-length
  *    normalisation; photonmap.rs:62-67 on malformed map text.)
  *  - The caller owns host buffers.  Device buffers are owned by the object
  *    that allocated them and are freed by the matching *_destroy.
@@ -119,7 +119,7 @@ typedef struct {
 } ppm_light_desc;
 
 /* Camera (camera.rs:23-50): the derived screen vectors of read_camera
- * (camera.rs:244-256); ppm_camera_setup() computes them from the
+ * (camera.rs:151-164); ppm_camera_setup() computes them from the
  * user-facing parameters. */
 typedef struct {
   int32_t xreso, yreso;
@@ -193,8 +193,8 @@ const char* ppm_status_string(ppm_status s);
 /* Block until all work queued on the context's stream is done. */
 ppm_status  ppm_sync(ppm_ctx* ctx);
 
-/* camera.rs:201-299 read_camera: fill the derived screen vectors of `cam`
- * from the camera parameters (focal_len in metres, as camera.rs:235). */
+/* camera.rs:108-206 read_camera: fill the derived screen vectors of `cam`
+ * from the camera parameters (focal_len in metres, as camera.rs:142). */
 ppm_status  ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso,
                              const double eye[3], const double target[3],
                              const double upper[3], double focus,
@@ -203,7 +203,7 @@ ppm_status  ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso,
 /* ---- scene -------------------------------------------------------------- */
 
 /* Upload shapes, materials, lights and camera (scene.rs:20 read_scene,
- * camera.rs:201 read_camera). */
+ * camera.rs:108 read_camera). */
 ppm_status  ppm_scene_create(ppm_ctx* ctx, const ppm_scene_desc* desc,
                              ppm_scene** out);
 void        ppm_scene_destroy(ppm_scene* scene);
@@ -297,7 +297,7 @@ ppm_status  ppm_gather(ppm_ctx* ctx, const ppm_map* map, const double* q_xyz,
 /* ---- eye pass (H7) ------------------------------------------------------ */
 
 /* One pass of per-pixel trace_ray (tracer.rs:129-177) over the camera's
- * screen (camera.rs:151-168, ppmpa.rs:576-577): rgb_out is yreso x xreso x 3
+ * screen (camera.rs:58-75, ppmpa.rs:81-82): rgb_out is yreso x xreso x 3
  * doubles, row-major (y, then x) as the reference prints it. */
 ppm_status  ppm_trace_eye(ppm_ctx* ctx, const ppm_scene* scene,
                           const ppm_map* map, uint64_t seed, uint32_t pass,

@@ -814,7 +814,7 @@ static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d, C w
   return mate.em * SR_HALF + s_bsdf(mate, cos, di, si, ti);
 }
 
-// Camera::generate_ray (camera.rs:151-168)
+// Camera::generate_ray (camera.rs:58-75)
 static void generate_ray(const Camera& cam, int x, int y, Rng& rng, V& o, V& d) {
   V blur{0, 0, 0};
   if (cam.blur) {
@@ -846,13 +846,13 @@ static C eye_pixel(const Scene& sc, const Kd& kd, uint64_t seed, uint32_t pass, 
 static void assign(const Scene& sc, uint64_t n_photon, std::vector<uint64_t>& first, std::vector<uint64_t>& cnt,
                    Flt& power, uint64_t& total) {
   Flt tf = 0;
-  for (auto& l : sc.lights) tf += l.flux;  // ppmpa.rs:525
+  for (auto& l : sc.lights) tf += l.flux;  // ppmpa.rs:30
   power = tf / (Flt)n_photon;
   total = 0;
   first.clear();
   cnt.clear();
   for (auto& l : sc.lights) {
-    Flt nl = std::round(l.flux / power);  // ppmpa.rs:565-567
+    Flt nl = std::round(l.flux / power);  // ppmpa.rs:70-72
     uint64_t c = nl > 0 ? (uint64_t)nl : 0;
     first.push_back(total);
     cnt.push_back(c);
@@ -1117,7 +1117,7 @@ void orc_gather(const orc_map* m, uint64_t nq, const double* q, const double* nr
   }
 }
 
-// One eye pass over the camera's screen (ppmpa.rs:576-577): row-major image.
+// One eye pass over the camera's screen (ppmpa.rs:81-82): row-major image.
 void orc_trace_eye(const orc_scene* s, const orc_map* m, uint64_t seed, uint32_t pass, double r2, int uc, int maxd,
                    int pf, double* rgb) {
   const Scene& sc = s->sc;

@@ -333,7 +333,7 @@ ppm_status ppm_camera_setup(ppm_camera_desc* cam, int32_t xreso, int32_t yreso, 
                             double f_number) {
   if (!cam || !eye || !target || !upper || xreso <= 0 || yreso <= 0 || focal_len == 0.0 || f_number == 0.0)
     return PPM_E_INVALID;
-  // camera.rs:244-256
+  // camera.rs:151-164
   double d[3] = {target[0] - eye[0], target[1] - eye[1], target[2] - eye[2]};
   double ez[3], ex[3], ey[3], t[3];
   if (!norm3(d, ez)) return PPM_E_INVALID;

@@ -546,7 +546,7 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n, uint32_t* stk) {
 }
 
 // ---------------------------------------------------------------------------
-// Camera::generate_ray (camera.rs:151-168) for pixel (y, x); draws come from
+// Camera::generate_ray (camera.rs:58-75) for pixel (y, x); draws come from
 // the pixel's root stream: blur offsets first, then the AA jitter.
 __device__ __forceinline__ void camera_ray(const DevCamera& cam, int x, int y, Rng& rng, V3& o, V3& d) {
   V3 blur = mk(0.0f, 0.0f, 0.0f);

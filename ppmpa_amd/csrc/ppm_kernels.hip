@@ -1,7 +1,7 @@
 // ppm_kernels.hip — gfx950 kernels of one PPM-PA pass:
 //   photon emission + bouncing   (light.rs:67-91, tracer.rs:31-125)
 //   photon map build             (photonmap.rs:23-29 + crate kdtree)
-//   eye-ray tree + direct light  (tracer.rs:129-177, 263-290, camera.rs:151-168)
+//   eye-ray tree + direct light  (tracer.rs:129-177, 263-290, camera.rs:58-75)
 //   radius query + gather        (tracer.rs:179-216, optics.rs:224-233)
 //   FP64 per-pixel accumulation  (util/averager2.rb:49-62)
 //

@@ -1,14 +1,14 @@
 """Photon-map text format of the reference's `pm | rt` pipeline.
 
-Writer: pm.rs:463-464 (line 1 = number of photons emitted, line 2 = photon
-power) and pm.rs:481-493 (one "<Red|Green|Blue> px py pz dx dy dz" line per
+Writer: pm.rs:44-45 (line 1 = number of photons emitted, line 2 = photon
+power) and pm.rs:62-75 (one "<Red|Green|Blue> px py pz dx dy dz" line per
 stored photon, Rust `{}` f64 formatting).  Reader: read_map
 (photonmap.rs:31-74): line 1 ignored, line 2 parsed as the power (1.0 when it
 does not parse), then one photon per line; an unknown channel name reads as
 Red, a malformed number is an error (the reference panics).
 
-Also the per-pass radiance image (.ppmf): pnm_header (camera.rs:170-183)
-plus one `{:e} {:e} {:e}` line per pixel (camera.rs:305-307).
+Also the per-pass radiance image (.ppmf): pnm_header (camera.rs:77-90)
+plus one `{:e} {:e} {:e}` line per pixel (camera.rs:212-214).
 """
 from dataclasses import dataclass
 
@@ -102,7 +102,7 @@ def _is_file(s):
 
 
 def format_map(n_emitted, power, pos, dir, wl) -> str:
-    """pm's stdout (pm.rs:463-464,481-493)."""
+    """pm's stdout (pm.rs:44-45,62-75)."""
     out = [str(int(n_emitted)), rust_f64(power)]
     for p, d, w in zip(np.asarray(pos), np.asarray(dir), np.asarray(wl)):
         out.append(" ".join([WL_NAMES[int(w)]] + [rust_f64(v) for v in p] + [rust_f64(v) for v in d]))
@@ -115,8 +115,9 @@ def write_map(path, n_emitted, power, pos, dir, wl):
 
 
 def format_ppmf(img, max_radiance=0.01, shutter_speed=0.004, f_number=4.0, iso=100.0) -> str:
-    """A per-pass radiance image as ppmpa/rt print it (camera.rs:170-183,
-    305-307): 5 header lines then one '{:e} {:e} {:e}' line per pixel."""
+    """A per-pass radiance image as ppmpa/rt print it in progressive mode:
+    pnm_header (camera.rs:77-90) then one radiance_to_string line per pixel
+    (camera.rs:212-214, '{:e} {:e} {:e}'; rt.rs:47-55)."""
     h, w, _ = img.shape
     ss = f"1/{rust_f64(1.0 / shutter_speed)}" if shutter_speed < 1.0 else rust_f64(shutter_speed)
     head = ["P3", f"## max radiance = {rust_f64(max_radiance)}",

@@ -3,7 +3,7 @@ terms, and the synthetic scenes of the benchmark configurations.
 
 Reference types: Material (material.rs:11-16), Surface (surface.rs:16-66),
 Shape (geometry.rs:66-115), Object (object.rs:10-13), Light (light.rs:16-39),
-Camera (camera.rs:23-50, read_camera camera.rs:201-299), M_AIR
+Camera (camera.rs:23-50, read_camera camera.rs:108-206), M_AIR
 (scene.rs:13-18).  A Scene converts to the ppm_scene_desc of include/ppm.h.
 """
 import ctypes as C
@@ -156,7 +156,7 @@ class Camera:
     target: Vec
     upper: Vec = (0.0, 1.0, 0.0)
     focus: float = 7.0
-    focal_len: float = 0.05    # metres (camera.rs:235 divides mm by 1000)
+    focal_len: float = 0.05    # metres (camera.rs:142 divides mm by 1000)
     f_number: float = 4.0
     progressive: bool = True
     antialias: bool = True
@@ -164,7 +164,7 @@ class Camera:
     pfilter: int = A.FILTER_NONE
 
     def screen(self):
-        """Derived vectors of read_camera (camera.rs:244-256)."""
+        """Derived vectors of read_camera (camera.rs:151-164)."""
         eye = np.asarray(self.eye, float)
         ez = np.asarray(self.target, float) - eye
         ez = ez / np.linalg.norm(ez)
@@ -199,7 +199,7 @@ def pinhole(xreso, yreso, eye, target, fov_deg=60.0, upper=(0.0, 1.0, 0.0), anti
 
 
 def read_camera() -> Camera:
-    """read_camera's built-in defaults (camera.rs:201-221); the file is ignored
+    """read_camera's built-in defaults (camera.rs:109-128); the file is ignored
     by the reference as well."""
     return Camera(256, 256, (1.0, 2.0, -4.5), (0.0, 1.0, 0.0), (0.0, 1.0, 0.0), focus=7.0, focal_len=0.05,
                   f_number=4.0, progressive=True, antialias=True, blur=True)

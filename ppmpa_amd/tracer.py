@@ -9,7 +9,7 @@ gfx950 C ABI (libppm.so).
   read_map (photonmap.rs:31-74)                  read_map()  (text parsing in photonmap_io)
   KdTree::within (tracer.rs:180)                 PhotonMap.within()
   estimate_radiance (tracer.rs:179-195)          PhotonMap.estimate_radiance()
-  trace_ray per pixel (ppmpa.rs:576-577)         trace_eye()
+  trace_ray per pixel (ppmpa.rs:81-82)         trace_eye()
   util/iterator.rb + util/averager2.rb           Session / run_passes()
 
 Every call goes to the HIP library; there is no CPU fallback.  Errors raise
@@ -174,7 +174,7 @@ class PhotonSet:
 
 
 def photon_power(scene, n_photon):
-    """ppmpa.rs:525 / pm.rs:459-460: Σ flux / nphoton."""
+    """ppmpa.rs:30 / pm.rs:40-41: Σ flux / nphoton."""
     return sum(l.flux for l in scene.lights) / float(n_photon)
 
 
@@ -265,7 +265,7 @@ def read_map(ctx, path_or_text):
 
 
 def trace_eye(ctx, dscene, pmap, seed, pass_, r2, use_classic=True, max_depth=10, pfilter=A.FILTER_NONE):
-    """One eye pass: trace_ray for every pixel (ppmpa.rs:576-577); (H, W, 3)."""
+    """One eye pass: trace_ray for every pixel (ppmpa.rs:81-82); (H, W, 3)."""
     cam = dscene.scene.camera
     out = np.zeros((cam.yreso, cam.xreso, 3))
     ctx.check(ctx.lib.ppm_trace_eye(ctx.h, dscene.h, pmap.h, int(seed), int(pass_), float(r2), int(use_classic),

@@ -3,7 +3,7 @@
 The reference publishes no vectors and cannot be built here (Rust, no
 cargo), so the photon set is produced by the FP64 oracle's restatement of
 trace_photon and written in the reference's own pm text format
-(pm.rs:463-493), which read_map parses (photonmap.rs:31-74).  Expected
+(pm.rs:44-45,62-75), which read_map parses (photonmap.rs:31-74).  Expected
 outputs: the FP64 ball-query index sets (brute force, the reference's
 predicate) and the FP64 radiance estimates (estimate_radiance, all three
 filters).

@@ -54,7 +54,7 @@ def test_struct_layout_matches_header(tmp_path):
 
 
 def test_camera_setup_matches_read_camera(lib):
-    """ppm_camera_setup (C) == Camera.screen() (camera.rs:244-256)."""
+    """ppm_camera_setup (C) == Camera.screen() (camera.rs:151-164)."""
     cam = P.read_camera()
     d = A.CameraDesc()
     eye, tgt, up = (C.c_double * 3)(*cam.eye), (C.c_double * 3)(*cam.target), (C.c_double * 3)(*cam.upper)

@@ -1,0 +1,131 @@
+"""GPU: the command lines (pm | rt, ppmpa, iterate) on a scene and camera
+read from files (tests/golden/box.scene, box.scr): each prints what the
+in-process API computes for the same inputs, and the rendered images match
+the FP64 oracle on common random numbers."""
+import io
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_common import ctx, eye_agreement, say
+
+import ppmpa_amd as P
+from ppmpa_amd import cli
+from ppmpa_amd import image_io as I
+from ppmpa_amd import loaders as L
+from ppmpa_amd import photonmap_io as M
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+SCENE = os.path.join(GOLD, "box.scene")
+SCR = os.path.join(GOLD, "box.scr")
+
+
+def _scene():
+    cam, cfg = L.read_camera_file(SCR)
+    return L.read_scene(SCENE, camera=cam), cam, cfg
+
+
+def test_pm_rt_pipeline(capsys, monkeypatch):
+    say("cli pm | rt")
+    capsys.readouterr()  # keep the case line out of the captured tool output
+    assert cli.pm_main([SCENE, "8000", "--seed", "3", "--pass", "2"]) == 0
+    text = capsys.readouterr().out
+    mp = M.parse_map(text)
+    assert mp.n_emitted == 8000 and mp.power == 20.0 / 8000
+    # the same photons as the API's pass (one line per stored photon)
+    sc, cam, cfg = _scene()
+    c = ctx()
+    ds = P.DeviceScene(c, sc)
+    ph = P.trace_photons(c, ds, 3, 2, 8000, use_classic=True, max_depth=cli.MAX_TRACE)
+    pos, dir, wl, _ = ph.download()
+    api = M.format_map(8000, ph.power, pos, dir, wl).split("\n")[2:-1]
+    assert sorted(api) == sorted(text.split("\n")[2:-1])
+    # rt reads the map from stdin
+    monkeypatch.setattr("sys.stdin", io.StringIO(text))
+    assert cli.rt_main([SCENE, SCR, "0.05", "--seed", "5"]) == 0
+    out = capsys.readouterr().out
+    img, maxrad = M.parse_ppmf(out)
+    assert maxrad == 2.0 and img.shape == (30, 40, 3)
+    assert out.split("\n")[3] == "40 30"
+    pmap, _ = P.read_map(c, text)
+    want = P.trace_eye(c, ds, pmap, 5, 1, 0.05 ** 2, use_classic=True, max_depth=cli.MAX_TRACE,
+                       pfilter=cam.pfilter)
+    assert np.allclose(img, want, rtol=1e-6, atol=0) and (img == want).mean() >= 0.99
+    # and the FP64 oracle on the same photons and streams
+    osc = O.OScene(sc)
+    om = O.OMap(mp.pos, mp.dir, mp.wl, mp.power)
+    ref = osc.trace_eye(om, 5, 1, 0.05 ** 2, True, cli.MAX_TRACE, cam.pfilter)
+    assert ref.max() > 0
+    frac = eye_agreement(img, ref)
+    print("rt vs oracle: agreeing pixels", frac)
+    assert frac > 0.98, frac
+
+
+def test_rt_rgb_output(capsys, monkeypatch, tmp_path):
+    """A non-progressive camera prints RGB lines (rt.rs:47-55)."""
+    say("cli rt rgb")
+    capsys.readouterr()  # keep the case line out of the captured tool output
+    scr = tmp_path / "rgb.scr"
+    scr.write_text(open(SCR).read().replace("progressive   : yes", "progressive   : no"))
+    assert cli.pm_main([SCENE, "4000"]) == 0
+    text = capsys.readouterr().out
+    monkeypatch.setattr("sys.stdin", io.StringIO(text))
+    assert cli.rt_main([SCENE, str(scr), "0.05"]) == 0
+    out = capsys.readouterr().out.split("\n")
+    assert out[0] == "P3" and out[3] == "40 30" and out[4] == "255"
+    rgb = np.array([[int(v) for v in l.split()] for l in out[5:5 + 1200]])
+    assert rgb.shape == (1200, 3) and rgb.min() >= 0 and rgb.max() <= 255 and rgb.max() > 0
+
+
+def test_ppmpa_cli(capsys):
+    say("cli ppmpa")
+    capsys.readouterr()  # keep the case line out of the captured tool output
+    assert cli.ppmpa_main(["10000", "0.05", SCR, SCENE, "--seed", "7"]) == 0
+    img, _ = M.parse_ppmf(capsys.readouterr().out)
+    sc, cam, cfg = _scene()
+    c = ctx()
+    ds = P.DeviceScene(c, sc)
+    ph = P.trace_photons(c, ds, 7, 1, 10000, use_classic=True, max_depth=cli.MAX_TRACE)
+    pmap = P.build_photonmap(c, ph, P.photon_power(sc, 10000), ds)
+    want = P.trace_eye(c, ds, pmap, 7, 1, 0.05 ** 2, use_classic=True, max_depth=cli.MAX_TRACE,
+                       pfilter=cam.pfilter)
+    assert np.allclose(img, want, rtol=1e-6, atol=0) and (img == want).mean() >= 0.99
+    # -nc: photons stored from depth 0 and no direct-light term
+    assert cli.ppmpa_main(["-nc", "10000", "0.05", SCR, SCENE, "--seed", "7"]) == 0
+    img_nc, _ = M.parse_ppmf(capsys.readouterr().out)
+    assert img_nc.shape == img.shape and not np.array_equal(img_nc, img)
+    assert abs(img_nc.mean() - img.mean()) < 0.5 * img.mean()
+
+
+def test_iterate_cli(capsys, tmp_path):
+    """iterate: a PPM-PA run averaged into OpenEXR / PPM as averager2.rb
+    writes them; the mean equals the session's and the oracle's."""
+    say("cli iterate")
+    capsys.readouterr()  # keep the case line out of the captured tool output
+    exr = tmp_path / "out.exr"
+    assert cli.iterate_main(["-exr", str(exr), "10000", "3", "0.05", SCR, SCENE, "--seed", "2"]) == 0
+    capsys.readouterr()
+    got, _ = I.read_exr(exr.read_bytes())
+    sc, cam, cfg = _scene()
+    c = ctx()
+    ds = P.DeviceScene(c, sc)
+    pc = P.pass_cfg(seed=2, n_photon=10000, radius0=0.05, alpha=cli.ITER_ALPHA, max_depth=cli.MAX_TRACE,
+                    use_classic=True, pfilter=cam.pfilter)
+    mean, _, t = P.run_passes(c, ds, pc, 3)
+    assert np.allclose(got, (mean / 2.0).astype(np.float32), rtol=1e-5, atol=0)
+    ppm = tmp_path / "out.ppm"
+    assert cli.iterate_main(["-ppm", str(ppm), "10000", "3", "0.05", SCR, SCENE, "--seed", "2"]) == 0
+    capsys.readouterr()
+    lines = ppm.read_text().split("\n")
+    want = I.format_average_ppm(mean, 2.0).split("\n")
+    assert lines[:4] == want[:4] == ["P3", "## max radiance = 2.0", "40 30", "255"]
+    same = np.mean([a == b for a, b in zip(lines[4:], want[4:])])
+    assert same >= 0.99
+    omean, _, _, _, _ = O.OScene(sc).run_passes(pc, 3)
+    frac = eye_agreement(mean, omean)
+    print("iterate vs oracle: agreeing pixels", frac)
+    assert frac > 0.97, frac

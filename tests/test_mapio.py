@@ -1,5 +1,5 @@
 """The pm / rt photon-map text format and the per-pass image text
-(pm.rs:463-493, photonmap.rs:31-74, camera.rs:170-183,305-307).  CPU only."""
+(pm.rs:44-45,62-75, photonmap.rs:31-74, camera.rs:77-90,212-214).  CPU only."""
 import numpy as np
 import pytest
 

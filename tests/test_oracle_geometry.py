@@ -249,7 +249,7 @@ def test_mirror_photons_never_stored():
 
 
 def test_camera_center_ray_is_eye_direction():
-    """camera.rs:151-168,244-256: the ray through screen point (lx-1/2,
+    """camera.rs:58-75,244-256: the ray through screen point (lx-1/2,
     ly-1/2) is ez, the direction from the eye to the target."""
     cam = S.Camera(64, 48, (1.0, 2.0, -4.5), (0.0, 1.0, 0.0), blur=False)
     s = cam.screen()
