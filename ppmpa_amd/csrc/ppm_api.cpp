@@ -154,16 +154,22 @@ ppm_status check_cfg_depth(ppm_ctx* c, int max_depth) {
   return PPM_OK;
 }
 
-hipError_t map_alloc(MapBuf& m, uint64_t cap, int n_cu) {
+// A map for cap photons: the box tree (grid = false) or the per-pass grid.
+hipError_t map_alloc(MapBuf& m, uint64_t cap, int n_cu, bool grid) {
   m.cap = cap;
-  uint64_t leaves = (cap + LEAF - 1) / LEAF;
-  m.lpad_max = next_pow2(leaves > 0 ? leaves : 1);
   m.sort_blocks = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * 4,
                                                                                                 (cap + 2047) / 2048)));
   hipError_t e;
   if ((e = dalloc(&m.sorted, cap)) != hipSuccess) return e;
   if ((e = dalloc(&m.src_idx, cap)) != hipSuccess) return e;
-  if ((e = dalloc(&m.nodes, 2ull * m.lpad_max)) != hipSuccess) return e;
+  if (grid) {
+    m.cells_max = (uint32_t)std::min<uint64_t>(GRID_MAX_BUCKETS, next_pow2(std::max<uint64_t>(cap, 1024)));
+    if ((e = dalloc(&m.cells, m.cells_max)) != hipSuccess) return e;
+  } else {
+    const uint64_t leaves = (cap + LEAF - 1) / LEAF;
+    m.lpad_max = next_pow2(leaves > 0 ? leaves : 1);
+    if ((e = dalloc(&m.nodes, 2ull * m.lpad_max)) != hipSuccess) return e;
+  }
   for (int i = 0; i < 2; ++i) {
     if ((e = dalloc(&m.keys[i], cap)) != hipSuccess) return e;
     if ((e = dalloc(&m.vals[i], cap)) != hipSuccess) return e;
@@ -178,6 +184,7 @@ void map_free(MapBuf& m) {
   dfree(m.sorted64);
   dfree(m.src_idx);
   dfree(m.nodes);
+  dfree(m.cells);
   for (int i = 0; i < 2; ++i) {
     dfree(m.keys[i]);
     dfree(m.vals[i]);
@@ -897,7 +904,7 @@ ppm_status ppm_map_create(ppm_ctx* c, uint64_t capacity, ppm_map** out) {
   if (capacity == 0 || capacity > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "capacity must be in [1, 2^31)");
   ppm_map* m = new ppm_map();
   m->ctx = c;
-  hipError_t e = map_alloc(m->m, capacity, c->n_cu);
+  hipError_t e = map_alloc(m->m, capacity, c->n_cu, false);
   if (e == hipSuccess) e = dalloc(&m->st, 1);
   if (e != hipSuccess) {
     ppm_map_destroy(m);
@@ -1147,7 +1154,7 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
     s->sbuf.lights = s->lights;
     if ((e = photons_alloc(s->pb, cap, trace_waves(total, c->n_cu))) != hipSuccess) break;
     photons_layout(s->pb, total, cfg->max_depth, c->n_cu);
-    if ((e = map_alloc(s->map, cap, c->n_cu)) != hipSuccess) break;
+    if ((e = map_alloc(s->map, cap, c->n_cu, true)) != hipSuccess) break;
     if ((e = eye_alloc(s->eb, sc->buf.cam, cfg->max_depth, c->n_cu)) != hipSuccess) break;
     const uint64_t n3 = 3ull * s->eb.n_pix;
     if ((e = dalloc(&s->sum, n3)) != hipSuccess) break;
@@ -1181,7 +1188,7 @@ static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
   if ((e = launch_trace_photons(st, s->sbuf, s->st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
     return e;
   if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
-  if ((e = launch_map_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
+  if ((e = launch_grid_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
   if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, s->ctx->n_cu)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
@@ -1324,7 +1331,7 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
         break;
       if ((e = launch_trace_photons(q, s->sbuf, st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
         break;
-      if ((e = launch_map_build(q, s->map, st, s->pb, bmin, bmax)) != hipSuccess) break;
+      if ((e = launch_grid_build(q, s->map, st, s->pb, bmin, bmax)) != hipSuccess) break;
       if ((e = launch_eye(q, s->sbuf, st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) != hipSuccess)
         break;
       if ((e = launch_gather(q, s->sbuf, s->map, st, s->eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
@@ -1463,7 +1470,8 @@ hipError_t batch_layout(ppm_batch* b) {
     cnt[BK_MORTON] = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1, (s->map.cap + 255) / 256));
     cnt[BK_PSORT] = it.psort_blocks;
     cnt[BK_SCAN] = 256;
-    cnt[BK_LEAVES] = std::min<uint32_t>(16384, (s->map.lpad_max + 63) / 64);
+    cnt[BK_LEAVES] = grid_photon_blocks(s->map.cap);
+    cnt[BK_CLEAR] = grid_clear_blocks(s->map.cells_max);
     cnt[BK_EYE] = (it.eye_waves + 3) / 4;
     cnt[BK_PIX] = (s->eb.q_primary + 255) / 256;
     cnt[BK_QKEYS] = (s->eb.q_primary + s->eb.ovf_cap + 255) / 256;

@@ -858,6 +858,249 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 }
 
 // ===========================================================================
+// Photon grid of a pass.  Every gather query of a pass has the same radius r
+// (util/iterator.rb:33-38), so the pass's photons are binned into cubic cells
+// of side c >= GRID_CELL_FACTOR * reach, reach a little above r: the box of a
+// query's ball then meets at most two cells per axis.  Cells are hashed
+// ((x p1) xor (y p2) xor (z p3) with the primes of Teschner et al. 2003,
+// "Optimized Spatial Hashing for Collision Detection of Deformable Objects")
+// into a power-of-two table sized from the pass's photon count; the photons
+// are sorted by bucket with the stable radix sort, and cells[b] holds the
+// range [start, end) of bucket b in that order.  A bucket may also hold
+// photons of other cells (collisions): they fail the distance test like any
+// photon outside the ball, so a query finds exactly { j : dist2_nc(q, p_j) <=
+// r^2 }, the set KdTree::within returns in float (tracer.rs:180).
+//
+// Coverage: the cell coordinate floor(clamp((v - bmin) * inv_c, -1, lim)) is
+// monotonic in v, and a photon the float test accepts has |p - q| <= r (1 +
+// 2^-21) per axis; a query visits the cells of q -+ reach per axis, with
+// reach = r (1 + 2^-10) + 2.4e-7 |q| on that axis covering the rounding of
+// q -+ reach as well, so the photon's own cell is always among them.
+
+constexpr uint32_t GRID_P1 = 73856093u, GRID_P2 = 19349663u, GRID_P3 = 83492791u;
+constexpr float GRID_CELL_FACTOR = 2.1f;  // cell side / (2 * reach) >= 1.05
+constexpr float GRID_MAX_CELLS = 32768.0f;  // cells per axis over the bounds, at most
+constexpr float GRID_REL_MARGIN = 2.4e-7f;  // per-point margin, times max |coordinate|
+
+struct GridView {
+  float4 bmin;
+  float inv_c, reach;
+  uint32_t mask;
+  int32_t lim;
+};
+
+__device__ __forceinline__ GridView grid_view(const PassState* st, float4 bmin) {
+  GridView g;
+  g.bmin = bmin;
+  g.inv_c = st->g_inv_c;
+  g.reach = st->g_reach;
+  g.mask = st->g_mask;
+  g.lim = st->g_lim;
+  return g;
+}
+
+__device__ __forceinline__ int grid_coord(float v, float b, float inv_c, int32_t lim) {
+  const float f = (v - b) * inv_c;
+  return (int)floorf(fminf(fmaxf(f, -1.0f), (float)lim));
+}
+
+__device__ __forceinline__ uint32_t grid_bucket(int x, int y, int z, uint32_t mask) {
+  return (((uint32_t)x * GRID_P1) ^ ((uint32_t)y * GRID_P2) ^ ((uint32_t)z * GRID_P3)) & mask;
+}
+
+// One thread: the pass's cell side, reach, bucket count (next power of two
+// >= stored photons, at least 1024, at most the table) and clamp.
+__device__ __forceinline__ void k_grid_setup_body(PassState* st, float4 bmin, float4 bmax, uint32_t cells_max,
+                                                  Blk blk) {
+  if (threadIdx.x != 0 || blk.b != 0) return;
+  const float r = sqrtf(st->r2f);
+  const float reach = r * (1.0f + 1.0f / 1024.0f);
+  const float mb = fmaxf(fmaxf(fmaxf(fabsf(bmin.x), fabsf(bmin.y)), fmaxf(fabsf(bmin.z), fabsf(bmax.x))),
+                         fmaxf(fabsf(bmax.y), fabsf(bmax.z)));
+  const float ext = fmaxf(fmaxf(bmax.x - bmin.x, bmax.y - bmin.y), fmaxf(bmax.z - bmin.z, 1e-30f));
+  const float c = fmaxf(GRID_CELL_FACTOR * (reach + GRID_REL_MARGIN * mb), ext / GRID_MAX_CELLS);
+  st->g_inv_c = 1.0f / c;
+  st->g_reach = reach;
+  st->g_lim = (int32_t)(ext / c) + 1;
+  uint32_t t = 1024u;
+  while (t < st->n_stored && t < cells_max) t <<= 1;
+  st->g_mask = min(t, cells_max) - 1u;
+}
+
+__global__ void k_grid_setup(PassState* st, float4 bmin, float4 bmax, uint32_t cells_max) {
+  k_grid_setup_body(st, bmin, bmax, cells_max, Blk{blockIdx.x, gridDim.x});
+}
+
+__device__ __forceinline__ void k_grid_keys_body(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
+                                                 uint32_t region_cap, const PassState* st, uint32_t* keys,
+                                                 uint32_t* vals, float4 bmin, uint32_t* nonfinite, Blk blk) {
+  const uint32_t n = st->n_stored;
+  const GridView g = grid_view(st, bmin);
+  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+    const uint64_t src = region_index(offset, n_regions, region_cap, i);
+    const float4 p = ph[src].pm;
+    if (!(isfinite(p.x) && isfinite(p.y) && isfinite(p.z))) atomicOr(nonfinite, 1u);
+    keys[i] = grid_bucket(grid_coord(p.x, g.bmin.x, g.inv_c, g.lim), grid_coord(p.y, g.bmin.y, g.inv_c, g.lim),
+                          grid_coord(p.z, g.bmin.z, g.inv_c, g.lim), g.mask);
+    vals[i] = (uint32_t)src;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grid_keys(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
+                                                   uint32_t region_cap, const PassState* st, uint32_t* keys,
+                                                   uint32_t* vals, float4 bmin, uint32_t* nonfinite) {
+  k_grid_keys_body(ph, offset, n_regions, region_cap, st, keys, vals, bmin, nonfinite, Blk{blockIdx.x, gridDim.x});
+}
+
+// Every bucket of the pass empty (a kernel, not a memset, so that a captured
+// pass graph replays it faithfully).
+__device__ __forceinline__ void k_grid_clear_body(uint2* cells, const PassState* st, Blk blk) {
+  const uint32_t nb = st->g_mask + 1u;
+  for (uint32_t b = blk.b * blockDim.x + threadIdx.x; b < nb; b += blk.n * blockDim.x) cells[b] = make_uint2(0u, 0u);
+}
+
+__global__ __launch_bounds__(256) void k_grid_clear(uint2* cells, const PassState* st) {
+  k_grid_clear_body(cells, st, Blk{blockIdx.x, gridDim.x});
+}
+
+// Bucket ranges from the sorted keys: the first photon of a bucket writes its
+// start, the last its end (separate 32-bit words).
+__device__ __forceinline__ void k_grid_cells_body(const uint32_t* keys, const PassState* st, uint2* cells, Blk blk) {
+  const uint32_t n = st->n_stored;
+  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+    const uint32_t b = keys[i];
+    if (i == 0 || keys[i - 1] != b) cells[b].x = i;
+    if (i + 1 == n || keys[i + 1] != b) cells[b].y = i + 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grid_cells(const uint32_t* keys, const PassState* st, uint2* cells) {
+  k_grid_cells_body(keys, st, cells, Blk{blockIdx.x, gridDim.x});
+}
+
+// Photon records (and double positions, if any) in bucket order.
+__device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const double* in64, const uint32_t* order,
+                                                    PhotonRec* out, double* out64, uint32_t* src_idx,
+                                                    const PassState* st, Blk blk) {
+  const uint32_t n = st->n_stored;
+  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+    const uint32_t src = order[i];
+    out[i] = in[src];
+    src_idx[i] = src;
+    if (in64) {
+      out64[3 * (uint64_t)i + 0] = in64[3 * (uint64_t)src + 0];
+      out64[3 * (uint64_t)i + 1] = in64[3 * (uint64_t)src + 1];
+      out64[3 * (uint64_t)i + 2] = in64[3 * (uint64_t)src + 2];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_grid_permute(const PhotonRec* in, const double* in64, const uint32_t* order,
+                                                      PhotonRec* out, double* out64, uint32_t* src_idx,
+                                                      const PassState* st) {
+  k_grid_permute_body(in, in64, order, out, out64, src_idx, st, Blk{blockIdx.x, gridDim.x});
+}
+
+hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin,
+                             float4 bmax) {
+  if (!m.cells || m.cells_max == 0 || m.cells_max > GRID_MAX_BUCKETS || (m.cells_max & (m.cells_max - 1)) != 0)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, s, st, bmin, bmax, m.cells_max);
+  uint32_t kblocks = (uint32_t)((m.cap + 255) / 256);
+  if (kblocks > 8192) kblocks = 8192;
+  if (kblocks == 0) kblocks = 1;
+  hipLaunchKernelGGL(k_grid_keys, dim3(kblocks), dim3(256), 0, s, in.rec, in.offset, in.n_regions, in.region_cap,
+                     (const PassState*)st, m.keys[0], m.vals[0], bmin, &st->nonfinite);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int cur = 0;
+  e = radix_sort(s, m.keys, m.vals, &st->n_stored, m.ghist, m.totals, m.sort_blocks, GRID_KEY_BITS, &cur);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_grid_clear, dim3(grid_clear_blocks(m.cells_max)), dim3(256), 0, s, m.cells,
+                     (const PassState*)st);
+  const uint32_t pblocks = grid_photon_blocks(m.cap);
+  hipLaunchKernelGGL(k_grid_cells, dim3(pblocks), dim3(256), 0, s, m.keys[cur], (const PassState*)st, m.cells);
+  hipLaunchKernelGGL(k_grid_permute, dim3(pblocks), dim3(256), 0, s, in.rec, in.rec64, m.vals[cur], m.sorted,
+                     m.sorted64, m.src_idx, (const PassState*)st);
+  return hipGetLastError();
+}
+
+// Every photon of the pass's grid within the ball (q, r2): on_photon(j, pm,
+// dm, d2) for each sorted index j with dist2_nc(q, p_j) <= r2, each once.
+// Buckets are visited in cell order (x fastest, then y, then z), photons in
+// sorted order within a bucket, so a query's sum is formed in a fixed order.
+// Per lane: no cross-lane operations, so inactive lanes may skip it.
+template <typename OnPhoton>
+__device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const PhotonRec* ph, uint32_t b, float qx,
+                                                 float qy, float qz, float r2, OnPhoton& on_photon) {
+  const uint2 se = cells[b];
+  uint32_t j = se.x;
+  // two whole records per step: both loads are in flight together
+  for (; j + 1 < se.y; j += 2) {
+    const PhotonRec p0 = ph[j], p1 = ph[j + 1];
+    const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
+    const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
+    if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
+    if (d1 <= r2) on_photon(j + 1, p1.pm, p1.dm, d1);
+  }
+  if (j < se.y) {
+    const PhotonRec p0 = ph[j];
+    const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
+    if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
+  }
+}
+
+// Returns false (and scans only the first three cells of an axis) if the
+// query's cells span more than three per axis, which the cell side rules out
+// for every finite point (the caller raises a flag).
+template <typename OnPhoton>
+__device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells, const PhotonRec* ph, float qx,
+                                           float qy, float qz, float r2, OnPhoton&& on_photon) {
+  const float rx = g.reach + GRID_REL_MARGIN * fabsf(qx);
+  const float ry = g.reach + GRID_REL_MARGIN * fabsf(qy);
+  const float rz = g.reach + GRID_REL_MARGIN * fabsf(qz);
+  const int x0 = grid_coord(qx - rx, g.bmin.x, g.inv_c, g.lim), x1 = grid_coord(qx + rx, g.bmin.x, g.inv_c, g.lim);
+  const int y0 = grid_coord(qy - ry, g.bmin.y, g.inv_c, g.lim), y1 = grid_coord(qy + ry, g.bmin.y, g.inv_c, g.lim);
+  const int z0 = grid_coord(qz - rz, g.bmin.z, g.inv_c, g.lim), z1 = grid_coord(qz + rz, g.bmin.z, g.inv_c, g.lim);
+  if (x1 - x0 <= 1 && y1 - y0 <= 1 && z1 - z0 <= 1) {
+    // at most 2 x 2 x 2 cells (the cell side makes this the rule): buckets in
+    // registers, a bucket met twice (two cells hashed alike) scanned once
+    uint32_t b[8];
+    bool v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int x = x0 + (k & 1), y = y0 + ((k >> 1) & 1), z = z0 + (k >> 2);
+      b[k] = grid_bucket(x, y, z, g.mask);
+      v[k] = x <= x1 && y <= y1 && z <= z1;
+#pragma unroll
+      for (int j = 0; j < k; ++j) v[k] = v[k] && !(v[j] && b[j] == b[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (v[k]) grid_bucket_scan(cells, ph, b[k], qx, qy, qz, r2, on_photon);
+    return true;
+  }
+  // three cells on some axis (float rounding at the edge of the bound): the
+  // same order, duplicates found by recomputing the earlier cells' buckets
+  const bool ok = x1 - x0 <= 2 && y1 - y0 <= 2 && z1 - z0 <= 2;
+  const int xe = min(x1, x0 + 2), ye = min(y1, y0 + 2), ze = min(z1, z0 + 2);
+  for (int z = z0; z <= ze; ++z)
+    for (int y = y0; y <= ye; ++y)
+      for (int x = x0; x <= xe; ++x) {
+        const uint32_t b = grid_bucket(x, y, z, g.mask);
+        bool dup = false;
+        for (int zz = z0; zz <= z; ++zz)
+          for (int yy = y0; yy <= ye; ++yy)
+            for (int xx = x0; xx <= xe; ++xx) {
+              const bool earlier = zz < z || (zz == z && (yy < y || (yy == y && xx < x)));
+              dup = dup || (earlier && grid_bucket(xx, yy, zz, g.mask) == b);
+            }
+        if (!dup) grid_bucket_scan(cells, ph, b, qx, qy, qz, r2, on_photon);
+      }
+  return ok;
+}
+
+// ===========================================================================
 // Scene BVH, built on the device once per scene.  The reference tests every
 // object for every ray (calc_intersection, tracer.rs:306-335); here the
 // bounded shapes (spheres, polygons, parallelograms) smaller than a quarter
@@ -1300,16 +1543,66 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
   k_query_keys_body(eb, st, bmin, bscale, fq, keys, vals, Blk{blockIdx.x, gridDim.x});
 }
 
-// Shading of the queries in Morton order.  A wave takes 64 consecutive
-// sorted queries (close together in space), one per lane.  The wave
-// traverses the tree for the union of its queries' balls; each overlapped
-// leaf is staged once into LDS with coalesced loads and every lane whose
-// ball overlaps the leaf box tests its photons from LDS (broadcast reads).
-//   res[slot] = w * (direct light (when uc) + scale * Σ filter * max(0,-n.dir))
+// Shading of the queries in Morton order, one query per lane, 64 spatially
+// adjacent queries per wave (their photons mostly share cache lines).  Each
+// lane gathers over the cells of the pass's photon grid that its ball meets:
+//   res[slot] = w * (direct light (when uc) + scale * sum filter * max(0,-n.dir))
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
-// BSDF factor and path throughput.
+// BSDF factor and path throughput.  A query whose cells exceed the grid's
+// span rule raises flag 4 of the pass state.
 template <bool BVH>
-__device__ __forceinline__ void k_shade_body(SceneView sc, const Node* nodes, const PhotonRec* ph,
+__device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
+                                               EyeBuf eb, const uint32_t* order, int pf, int uc, Blk blk) {
+  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
+  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
+  const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
+  const uint32_t nq = st->n_qsort;
+  const GridView g = grid_view(st, sc.bmin);
+  const uint32_t groups = (nq + 63) / 64;
+  const uint32_t waves = blk.n * 4;
+  bool complete = true;
+  for (uint32_t gi = blk.b * 4 + wib; gi < groups; gi += waves) {
+    const uint32_t i = gi * 64 + lane;
+    const bool active = i < nq;
+    uint32_t slot = 0;
+    Query qq{};
+    if (active) {
+      slot = order[i];
+      qq = eb.q[slot];
+    }
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    if (active) {
+      const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+      complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
+                             [&](uint32_t, float4 pm, float4 dm, float d2) {
+                               const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
+                               const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
+                               const uint32_t wl = __float_as_uint(pm.w) & 3u;
+                               ar += wl == 0 ? v : 0.0f;
+                               ag += wl == 1 ? v : 0.0f;
+                               ab += wl == 2 ? v : 0.0f;
+                             });
+    }
+    C3 di = C3{ar * scale, ag * scale, ab * scale};
+    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b), stk + threadIdx.x);
+    if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
+  }
+  if (!complete) atomicOr(&st->overflow, 4u);
+}
+
+template <bool BVH>
+__global__ __launch_bounds__(256) void k_shade(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
+                                               EyeBuf eb, const uint32_t* order, int pf, int uc) {
+  k_shade_body<BVH>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+}
+
+// The same shading over a box-tree map (ppm_trace_eye on a PhotonMap, whose
+// radius comes with the call): a wave takes 64 consecutive sorted queries
+// and traverses the tree for the union of their balls; each overlapped leaf
+// is staged once into LDS with coalesced loads and every lane whose ball
+// overlaps the leaf box tests its photons from LDS (broadcast reads).
+template <bool BVH>
+__device__ __forceinline__ void k_shade_tree_body(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc, Blk blk) {
   __shared__ LeafLds lds[4];
@@ -1383,10 +1676,10 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const Node* nodes, co
 }
 
 template <bool BVH>
-__global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, const PhotonRec* ph,
+__global__ __launch_bounds__(256) void k_shade_tree(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc) {
-  k_shade_body<BVH>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+  k_shade_tree_body<BVH>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
 }
 
 // ===========================================================================
@@ -1399,23 +1692,21 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const Node* nodes, 
 //                          * filter * weight_j,
 // the photon's share of dL; the replay of the photon walks then turns
 // adj[j] into dL/dc1 over the diffuse survivals before record j.
-template <bool BVH>
-__global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const Node* nodes, const PhotonRec* ph,
-                                                    const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
+__global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const uint2* cells, const PhotonRec* ph,
+                                                    PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                     const float4* gimg, float* adj, double* grad) {
-  __shared__ LeafLds lds[4];
   __shared__ float part[3 * ADJ_LDS_MATS];
   for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
   __syncthreads();
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
-  const uint32_t n = st->n_stored, depth = st->depth;
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
+  const GridView g = grid_view(st, sc.bmin);
   const uint32_t groups = (nq + 63) / 64;
   const uint32_t waves = gridDim.x * 4;
-  const uint32_t first_leaf = (1u << depth) - 1u;
-  for (uint32_t g = blockIdx.x * 4 + wib; g < groups; g += waves) {
-    const uint32_t i = g * 64 + lane;
+  bool complete = true;
+  for (uint32_t gi = blockIdx.x * 4 + wib; gi < groups; gi += waves) {
+    const uint32_t i = gi * 64 + lane;
     const bool active = i < nq;
     Query qq{};
     float4 gp = make_float4(0.f, 0.f, 0.f, 0.f), res = gp;
@@ -1435,53 +1726,21 @@ __global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const Node* no
     // per channel: gimg * w * scale, the factor on each photon's term
     const float fr = gp.x * qq.w.x * scale, fg = gp.y * qq.w.y * scale, fb = gp.z * qq.w.z * scale;
     const bool want = active && (fr != 0.0f || fg != 0.0f || fb != 0.0f);
-    const float qx = qq.a.x, qy = qq.a.y, qz = qq.a.z;
-    const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
-    uint32_t node = 0;
-    while (n > 0) {
-      node = __builtin_amdgcn_readfirstlane(node);
-      const Node nd = ldc(nodes, node);
-      const bool ov = want && box_dist2_nc(qx, qy, qz, nd.lo, nd.hi) <= r2;
-      if (__ballot(ov)) {
-        if (node < first_leaf) {
-          node = 2 * node + 1;
-          continue;
-        }
-        const uint32_t base = (node - first_leaf) * LEAF;
-        const uint32_t cnt = base < n ? min((uint32_t)LEAF, n - base) : 0u;
-        if (lane < cnt) {
-          const PhotonRec r = ph[base + lane];
-          lds[wib].pm[lane] = r.pm;
-          lds[wib].dm[lane] = r.dm;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (ov) {
-          for (uint32_t j = 0; j < cnt; ++j) {
-            const float4 pm = lds[wib].pm[j];
-            const float d2 = dist2_nc(qx, qy, qz, pm.x, pm.y, pm.z);
-            if (d2 <= r2) {
-              const float4 dm = lds[wib].dm[j];
-              const float c = nx * dm.x + ny * dm.y + nz * dm.z;
-              if (c < 0.0f) {
-                const uint32_t wl = __float_as_uint(pm.w) & 3u;
-                const float f = wl == 0 ? fr : (wl == 1 ? fg : fb);
-                const float a = f * (-c * filter_weight(pf, d2 * inv_r2) * dm.w);
-                if (a != 0.0f) atomicAdd(&adj[base + j], a);
-              }
-            }
-          }
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      }
-      while (node != 0 && (node & 1u) == 0) node = (node - 1) >> 1;
-      if (node == 0) break;
-      node += 1;
+    if (want) {
+      const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+      complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
+                             [&](uint32_t j, float4 pm, float4 dm, float d2) {
+                               const float c = nx * dm.x + ny * dm.y + nz * dm.z;
+                               if (c < 0.0f) {
+                                 const uint32_t wl = __float_as_uint(pm.w) & 3u;
+                                 const float f = wl == 0 ? fr : (wl == 1 ? fg : fb);
+                                 const float a = f * (-c * filter_weight(pf, d2 * inv_r2) * dm.w);
+                                 if (a != 0.0f) atomicAdd(&adj[j], a);
+                               }
+                             });
     }
   }
+  if (!complete) atomicOr(&st->overflow, 4u);
   adj_flush(part, grad, sc.n_mats);
 }
 
@@ -1508,12 +1767,8 @@ hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pa
   if ((e = hipMemsetAsync(adj_slot, 0, sizeof(float) * pb.cap, s)) != hipSuccess) return e;
   const SceneView v = make_view(sc);
   const uint32_t blocks = (uint32_t)n_cu * 8u;
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_adj_gather<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb,
-                       eb.qvals[0], pfilter, gimg, adj_sorted, grad);
-  else
-    hipLaunchKernelGGL(k_adj_gather<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb,
-                       eb.qvals[0], pfilter, gimg, adj_sorted, grad);
+  hipLaunchKernelGGL(k_adj_gather, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, eb.qvals[0],
+                     pfilter, gimg, adj_sorted, grad);
   hipLaunchKernelGGL(k_adj_permute, dim3((uint32_t)((m.cap + 255) / 256)), dim3(256), 0, s, m.src_idx, adj_sorted,
                      adj_slot, (const PassState*)st);
   const uint32_t n_waves = pb.n_regions;
@@ -1604,12 +1859,21 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   if (e != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
   const SceneView v = make_view(sc);
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
-                       use_classic);
-  else
-    hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur], pfilter,
-                       use_classic);
+  if (m.cells) {
+    if (sc.bvh)
+      hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
+                         pfilter, use_classic);
+    else
+      hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
+                         pfilter, use_classic);
+  } else {
+    if (sc.bvh)
+      hipLaunchKernelGGL(k_shade_tree<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur],
+                         pfilter, use_classic);
+    else
+      hipLaunchKernelGGL(k_shade_tree<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur],
+                         pfilter, use_classic);
+  }
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);
@@ -1813,11 +2077,39 @@ __global__ __launch_bounds__(1024) void kb_scan_tiles(const BatchItem* items, ui
                      it.eb.ovf_cap);
 }
 
-__global__ __launch_bounds__(256) void kb_morton(const BatchItem* items, const uint32_t* tab, uint32_t n,
-                                                 uint32_t n_all) {
+__global__ void kb_grid_setup(const BatchItem* items, uint32_t n) {
+  if (blockIdx.x >= n) return;
+  const BatchItem it = load_item(items, blockIdx.x);
+  k_grid_setup_body(it.st, it.sc.bmin, it.sc.bmax, it.m.cells_max, Blk{0u, 1u});
+}
+
+__global__ __launch_bounds__(256) void kb_grid_keys(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                    uint32_t n_all) {
   BATCH_PROLOGUE(BK_MORTON)
-  k_morton_body(it.pb.rec, it.pb.offset, it.pb.n_regions, it.pb.region_cap, it.st, it.m.keys[0], it.m.vals[0],
-                it.sc.bmin, it.mscale, 1023u, &it.st->nonfinite, blk);
+  k_grid_keys_body(it.pb.rec, it.pb.offset, it.pb.n_regions, it.pb.region_cap, it.st, it.m.keys[0], it.m.vals[0],
+                   it.sc.bmin, &it.st->nonfinite, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_grid_clear(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_CLEAR)
+  k_grid_clear_body(it.m.cells, it.st, blk);
+}
+
+// the sorted keys / values after GRID_KEY_BITS / 8 digit passes
+constexpr uint32_t GRID_SORTED = (GRID_KEY_BITS / 8u) & 1u;
+
+__global__ __launch_bounds__(256) void kb_grid_cells(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_LEAVES)
+  k_grid_cells_body(it.m.keys[GRID_SORTED], it.st, it.m.cells, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_grid_permute(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                       uint32_t n_all) {
+  BATCH_PROLOGUE(BK_LEAVES)
+  k_grid_permute_body(it.pb.rec, it.pb.rec64, it.m.vals[GRID_SORTED], it.m.sorted, it.m.sorted64, it.m.src_idx, it.st,
+                      blk);
 }
 
 // Radix-sort digit pass `p` (shift 8p) of the photon keys (which = 0) or the
@@ -1848,19 +2140,6 @@ __global__ __launch_bounds__(RS_THREADS) void kb_rs_scatter(const BatchItem* ite
                       it.m.totals, 8u * p, blk);
 }
 
-__global__ __launch_bounds__(256) void kb_build_leaves(const BatchItem* items, const uint32_t* tab, uint32_t n,
-                                                       uint32_t n_all) {
-  BATCH_PROLOGUE(BK_LEAVES)
-  k_build_leaves_body(it.pb.rec, it.pb.rec64, it.m.vals[0], it.m.sorted, it.m.sorted64, it.m.src_idx, it.m.nodes,
-                      it.st, blk);
-}
-
-__global__ __launch_bounds__(1024) void kb_build_top(const BatchItem* items, uint32_t n) {
-  if (blockIdx.x >= n) return;
-  const BatchItem it = load_item(items, blockIdx.x);
-  k_build_top_body(it.m.nodes, it.st);
-}
-
 template <bool BVH>
 __global__ __launch_bounds__(256) void kb_eye(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                               uint32_t n_all) {
@@ -1884,7 +2163,7 @@ template <bool BVH>
 __global__ __launch_bounds__(256) void kb_shade(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                 uint32_t n_all) {
   BATCH_PROLOGUE(BK_SHADE)
-  k_shade_body<BVH>(make_view(it.sc), it.m.nodes, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc, blk);
+  k_shade_body<BVH>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc, blk);
 }
 
 __global__ __launch_bounds__(256) void kb_overflow_add(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -1911,14 +2190,16 @@ hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32
   else
     hipLaunchKernelGGL(kb_trace<false>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_scan_photons, dim3(n), T1024, 0, s, items, n);
-  hipLaunchKernelGGL(kb_morton, dim3(blocks[BK_MORTON]), T256, 0, s, items, tab, n, n_all);
-  for (uint32_t p = 0; p < 4; ++p) {
+  hipLaunchKernelGGL(kb_grid_setup, dim3(n), T64, 0, s, items, n);
+  hipLaunchKernelGGL(kb_grid_keys, dim3(blocks[BK_MORTON]), T256, 0, s, items, tab, n, n_all);
+  for (uint32_t p = 0; p < GRID_KEY_BITS / 8u; ++p) {
     hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
     hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s, items, tab, n, n_all, 0);
     hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
   }
-  hipLaunchKernelGGL(kb_build_leaves, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
-  hipLaunchKernelGGL(kb_build_top, dim3(n), T1024, 0, s, items, n);
+  hipLaunchKernelGGL(kb_grid_clear, dim3(blocks[BK_CLEAR]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_grid_cells, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_grid_permute, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
   if (bvh)
     hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
   else

@@ -39,13 +39,19 @@ struct PhotonBuf {
   uint64_t cap = 0;          // records allocated
 };
 
+// A photon map in one of two forms.  Tree (ppm_map_build: any radius per
+// call): photons in leaf order under a heap-ordered box tree.  Grid (the
+// passes of a session or batch, whose queries share the pass radius): photons
+// sorted by hashed cell, cells[b] = [start, end) of bucket b.
 struct MapBuf {
   uint64_t cap = 0;
-  PhotonRec* sorted = nullptr;  // photons in tree order
-  double* sorted64 = nullptr;   // optional double positions in tree order (3 per photon)
-  uint32_t* src_idx = nullptr;  // tree order -> index in the input set
-  Node* nodes = nullptr;        // 2 * lpad_max - 1
+  PhotonRec* sorted = nullptr;  // photons in tree / bucket order
+  double* sorted64 = nullptr;   // optional double positions in that order (3 per photon)
+  uint32_t* src_idx = nullptr;  // that order -> index in the input set
+  Node* nodes = nullptr;        // tree: 2 * lpad_max - 1
   uint32_t lpad_max = 0;
+  uint2* cells = nullptr;       // grid: cells_max buckets
+  uint32_t cells_max = 0;       // a power of two
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* ghist = nullptr;    // 256 * sort_blocks
@@ -95,11 +101,30 @@ hipError_t launch_compact(hipStream_t s, const PhotonBuf& pb, const PassState* s
 
 hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax);
 
+// The pass's photon grid (cells of side >= 2.1 r over the scene bounds,
+// hashed into m.cells): setup, bucket keys, sort, clear, cell ranges, the
+// photon records in bucket order.  Buckets per pass: the next power of two
+// >= the stored photons, at least 1024, at most m.cells_max <= 2^24.
+constexpr uint32_t GRID_KEY_BITS = 24;  // bucket keys < 2^24: three 8-bit sort passes
+constexpr uint32_t GRID_MAX_BUCKETS = 1u << GRID_KEY_BITS;
+
+// Blocks of the per-photon and per-bucket grid kernels.
+inline uint32_t grid_photon_blocks(uint64_t cap) {
+  const uint64_t b = (cap + 255) / 256;
+  return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
+}
+inline uint32_t grid_clear_blocks(uint32_t cells_max) {
+  const uint32_t b = (cells_max + 255) / 256;
+  return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+}
+
+hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax);
+
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu);
 
-// Gather + direct light of one pass over the Morton-sorted queries, then the
-// per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
+// Gather (over the pass's photon grid) + direct light of one pass over the
+// Morton-sorted queries, then the per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
 // the FP64 sums; otherwise it is left in pix.  ev (optional, 2 events)
 // brackets the k_shade launch alone.
 hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
@@ -144,10 +169,11 @@ struct BatchItem {
 // of scene s in a kind-k launch (tab[k * (n_all + 1) + n_all] the total).
 enum BatchKind : int {
   BK_TRACE = 0,  // photon tracing waves / 4
-  BK_MORTON,     // photon Morton keys
+  BK_MORTON,     // photon grid keys
   BK_PSORT,      // photon sort blocks (hist, scatter)
   BK_SCAN,       // 256 digit blocks of a sort's scan
-  BK_LEAVES,     // map leaf groups
+  BK_LEAVES,     // grid cell ranges and record permutation (photons / 256)
+  BK_CLEAR,      // grid bucket clear
   BK_EYE,        // eye waves / 4
   BK_PIX,        // primary slots / 256 (tile counts, accumulation)
   BK_QKEYS,      // (primary + overflow slots) / 256

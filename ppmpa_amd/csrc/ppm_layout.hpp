@@ -152,6 +152,13 @@ struct alignas(16) PassState {
   uint32_t pad0;
   uint64_t total_stored;
   uint64_t total_queries;
+  // photon grid of the pass (k_grid_setup): cell side 1 / g_inv_c, query
+  // reach before its per-point rounding margin, bucket mask, cell index
+  // clamp [-1, g_lim]
+  float g_inv_c;
+  float g_reach;
+  uint32_t g_mask;
+  int32_t g_lim;
 };
 
 }  // namespace ppm
