@@ -38,6 +38,7 @@ struct ppm_scene {
   std::vector<float> flux0;
   double total_flux0 = 0.0;
   double light_scale = 1.0;
+  std::vector<DevShape> lin_host;  // the shapes tested one by one, as on the device
 };
 
 struct ppm_photons {
@@ -78,6 +79,10 @@ struct ppm_session {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[10] = {};
   ppm_timings t{};
+  // the eye branch of a pass (eye rays, query order) runs on s2, beside the
+  // photon branch (trace, grid build) on the main stream: fork and join
+  hipStream_t s2 = nullptr;
+  hipEvent_t evf = nullptr, evj = nullptr;
 };
 
 namespace {
@@ -242,7 +247,9 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   if ((e = dalloc(&eb.sec_count, eb.q_primary)) != hipSuccess) return e;
   if ((e = dalloc(&eb.pix, eb.n_pix)) != hipSuccess) return e;
   if ((e = dalloc(&eb.tile_count, tiles)) != hipSuccess) return e;
+  if ((e = hipMemset(eb.tile_count, 0, sizeof(uint32_t) * tiles)) != hipSuccess) return e;
   if ((e = dalloc(&eb.tile_off, tiles)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.tile_order, tiles)) != hipSuccess) return e;
   for (int i = 0; i < 2; ++i) {
     if ((e = dalloc(&eb.qkeys[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
     if ((e = dalloc(&eb.qvals[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
@@ -250,6 +257,9 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   if ((e = dalloc(&eb.qghist, 256ull * eb.q_sort_blocks)) != hipSuccess) return e;
   if ((e = dalloc(&eb.qtotals, 256)) != hipSuccess) return e;
   if ((e = dalloc(&eb.res, inplace + eb.ovf_cap)) != hipSuccess) return e;
+  eb.estack_lanes = 64u * std::max<uint32_t>(1, std::min<uint32_t>(tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU_MAX));
+  eb.estack_depth = (uint32_t)std::max(1, std::min(max_depth, MAX_DEPTH_LIMIT));
+  if ((e = dalloc(&eb.estack, 3ull * eb.estack_depth * eb.estack_lanes)) != hipSuccess) return e;
   return hipSuccess;
 }
 
@@ -259,6 +269,7 @@ void eye_free(EyeBuf& eb) {
   dfree(eb.pix);
   dfree(eb.tile_count);
   dfree(eb.tile_off);
+  dfree(eb.tile_order);
   for (int i = 0; i < 2; ++i) {
     dfree(eb.qkeys[i]);
     dfree(eb.qvals[i]);
@@ -266,6 +277,7 @@ void eye_free(EyeBuf& eb) {
   dfree(eb.qghist);
   dfree(eb.qtotals);
   dfree(eb.res);
+  dfree(eb.estack);
 }
 
 PassState host_state() {
@@ -388,12 +400,54 @@ void shape_box(const DevShape& s, double lo[3], double hi[3]) {
   }
 }
 
+ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags);
+
+// Padded float boxes of the shapes tested one by one, for the shadow test's
+// candidate culling: a plane is marked by lo.w = 1, a point gets an empty box.
+hipError_t upload_lin_boxes(SceneBuf& b, const std::vector<DevShape>& lin) {
+  std::vector<Node> boxes(std::max<size_t>(lin.size(), 1));
+  for (size_t i = 0; i < lin.size(); ++i) {
+    const DevShape& s = lin[i];
+    Node& nd = boxes[i];
+    if (s.kind == SHAPE_PLAIN) {
+      nd.lo = make_float4(0.f, 0.f, 0.f, 1.f);
+      nd.hi = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    if (s.kind != SHAPE_SPHERE && s.kind != SHAPE_POLYGON && s.kind != SHAPE_PARALLELOGRAM) {
+      nd.lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f);
+      nd.hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+      continue;
+    }
+    double lo[3], hi[3];
+    shape_box(s, lo, hi);
+    double m = 0.0;
+    for (int a = 0; a < 3; ++a) m = std::max({m, std::fabs(lo[a]), std::fabs(hi[a])});
+    const double pad = 2e-5 * (1.0 + m);
+    nd.lo = make_float4(std::nextafter((float)(lo[0] - pad), -INFINITY), std::nextafter((float)(lo[1] - pad), -INFINITY),
+                        std::nextafter((float)(lo[2] - pad), -INFINITY), 0.f);
+    nd.hi = make_float4(std::nextafter((float)(hi[0] + pad), INFINITY), std::nextafter((float)(hi[1] + pad), INFINITY),
+                        std::nextafter((float)(hi[2] + pad), INFINITY), 0.f);
+  }
+  hipError_t e = dalloc(&b.lin_box, boxes.size());
+  if (e == hipSuccess) e = hipMemcpy(b.lin_box, boxes.data(), boxes.size() * sizeof(Node), hipMemcpyHostToDevice);
+  return e;
+}
+
 // Shapes tested one by one and, when the scene asks for one (or has at least
 // BVH_MIN_PRIMS bounded shapes in automatic mode), the device-built BVH over
 // the bounded shapes whose box sides are all within a quarter of the scene's
 // extent; planes and larger shapes (walls) are tested one by one.
 // shapes[] is already on the device as sc->buf.shapes.
 ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
+  const ppm_status r = scene_layout_bvh(c, sc, shapes, flags);
+  if (r != PPM_OK) return r;
+  const hipError_t e = upload_lin_boxes(sc->buf, sc->lin_host);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  return PPM_OK;
+}
+
+ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
   SceneBuf& b = sc->buf;
   std::vector<uint32_t> bounded;
   for (size_t i = 0; i < shapes.size(); ++i) {
@@ -404,6 +458,7 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
   if (!use_bvh || bounded.empty()) {
     b.lin = b.shapes;
     b.n_lin = b.n_shapes;
+    sc->lin_host = shapes;
     return PPM_OK;
   }
   double smin[3] = {INFINITY, INFINITY, INFINITY}, smax[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -444,6 +499,7 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
   if (in_bvh.empty()) {
     b.lin = b.shapes;
     b.n_lin = b.n_shapes;
+    sc->lin_host = shapes;
     return PPM_OK;
   }
   const uint64_t n = in_bvh.size();
@@ -494,6 +550,7 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
   b.n_lin = (uint32_t)lin.size();
   b.n_bprims = n32;
   b.bvh_depth = depth;
+  sc->lin_host = lin;
   return PPM_OK;
 }
 
@@ -649,6 +706,7 @@ void ppm_scene_destroy(ppm_scene* sc) {
   dfree(sc->buf.lights);
   dfree(sc->buf.bvh);
   dfree(sc->buf.bprims);
+  dfree(sc->buf.lin_box);
   delete sc;
 }
 
@@ -1076,6 +1134,7 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
     hs.scale = (float)(m->power / (M_PI * r2));
     hs.n_queries = 0;
     hs.overflow = 0;
+    hs.direct_next = hs.shade_next = hs.eye_next = hs.tiles_front = hs.tiles_back = 0;
     if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth, c->n_cu)) != hipSuccess) break;
     if ((e = launch_gather(c->stream, sc->buf, m->m, st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=
@@ -1110,6 +1169,12 @@ void ppm_session_destroy(ppm_session* s) {
   if (s->graph) (void)hipGraphDestroy(s->graph);
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
+  if (s->s2) {
+    (void)hipStreamSynchronize(s->s2);
+    (void)hipStreamDestroy(s->s2);
+  }
+  if (s->evf) (void)hipEventDestroy(s->evf);
+  if (s->evj) (void)hipEventDestroy(s->evj);
   dfree(s->lights);
   photons_free(s->pb);
   map_free(s->map);
@@ -1166,6 +1231,10 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
     if ((e = hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) break;
     for (auto& ev : s->ev)
       if ((e = hipEventCreate(&ev)) != hipSuccess) break;
+    if (e != hipSuccess) break;
+    if ((e = hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking)) != hipSuccess) break;
+    if ((e = hipEventCreateWithFlags(&s->evf, hipEventDisableTiming)) != hipSuccess) break;
+    if ((e = hipEventCreateWithFlags(&s->evj, hipEventDisableTiming)) != hipSuccess) break;
   } while (0);
   if (e != hipSuccess) {
     ppm_session_destroy(s);
@@ -1175,25 +1244,43 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
   return PPM_OK;
 }
 
-// Enqueue the kernels of one pass; ev != nullptr records stage events.
+// Enqueue the kernels of one pass.  The eye branch (eye rays, query order)
+// depends only on the pass state, so it runs on s2 beside the photon branch
+// (trace, grid build) and joins before the gather; ev != nullptr (stage
+// timing) runs every stage in order on one stream and records its events.
 static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
   const ppm_pass_cfg& cfg = s->cfg;
   const ppm_scene* sc = s->scene;
   hipError_t e;
   const float4 bmin = make_float4((float)sc->bmin[0], (float)sc->bmin[1], (float)sc->bmin[2], 0.f);
   const float4 bmax = make_float4((float)sc->bmax[0], (float)sc->bmax[1], (float)sc->bmax[2], 0.f);
+  const int n_cu = s->ctx->n_cu;
   if (ev && (e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
   if ((e = launch_pass_begin(st, s->st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
     return e;
+  if (!ev) {
+    if ((e = hipEventRecord(s->evf, st)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(s->s2, s->evf, 0)) != hipSuccess) return e;
+    if ((e = launch_eye(s->s2, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
+      return e;
+    if ((e = launch_query_prep(s->s2, s->sbuf, s->map, s->st, s->eb)) != hipSuccess) return e;
+    if ((e = hipEventRecord(s->evj, s->s2)) != hipSuccess) return e;
+  }
   if ((e = launch_trace_photons(st, s->sbuf, s->st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
     return e;
   if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
   if ((e = launch_grid_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
-  if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, s->ctx->n_cu)) != hipSuccess) return e;
-  if (ev && (e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
-  if ((e = launch_gather(st, s->sbuf, s->map, s->st, s->eb, cfg.pfilter, cfg.use_classic, s->ctx->n_cu, s->sum,
-                         s->sumsq, ev ? ev + 4 : nullptr)) != hipSuccess)
+  if (ev) {
+    if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
+      return e;
+    if ((e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
+    if ((e = launch_query_prep(st, s->sbuf, s->map, s->st, s->eb)) != hipSuccess) return e;
+  } else if ((e = hipStreamWaitEvent(st, s->evj, 0)) != hipSuccess) {
+    return e;
+  }
+  if ((e = launch_gather(st, s->sbuf, s->map, s->st, s->eb, cfg.pfilter, cfg.use_classic, n_cu, s->sum, s->sumsq,
+                         ev ? ev + 4 : nullptr, true)) != hipSuccess)
     return e;
   if ((e = launch_pass_end(st, s->st)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[6], st)) != hipSuccess) return e;
@@ -1456,7 +1543,7 @@ hipError_t batch_layout(ppm_batch* b) {
     it.maxd = s->cfg.max_depth;
     it.pfilter = s->cfg.pfilter;
     it.trace_per_wave = (uint32_t)(((uint64_t)s->sbuf.n_emit_total + s->pb.n_regions - 1) / s->pb.n_regions);
-    it.eye_waves = std::max(1u, std::min(s->eb.n_tiles, n_cu * ew));
+    it.eye_waves = std::max(1u, std::min(std::min(s->eb.n_tiles, n_cu * ew), s->eb.estack_lanes / 64u));
     // sort blocks from the expected key counts (about 2 stored photons per
     // emitted one, 1.5 queries per pixel), 8192 keys per block: a stable
     // radix sort gives the same order for any block count, and a batch's

@@ -28,10 +28,13 @@ __device__ __forceinline__ V3 operator-(V3 a, V3 b) { return V3{a.x - b.x, a.y -
 __device__ __forceinline__ V3 operator-(V3 a) { return V3{-a.x, -a.y, -a.z}; }
 __device__ __forceinline__ V3 operator*(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
 __device__ __forceinline__ V3 operator*(float s, V3 a) { return V3{a.x * s, a.y * s, a.z * s}; }
-__device__ __forceinline__ float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// dot and cross with their fused multiply-adds written out, so that they
+// round the same way wherever they are inlined (the intersection code that
+// the BVH and the shape list share must give bit-identical distances).
+__device__ __forceinline__ float dot(V3 a, V3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 // algebra.rs:174-180 (component order kept)
 __device__ __forceinline__ V3 cross(V3 a, V3 b) {
-  return V3{a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+  return V3{fmaf(a.y, b.z, -(b.y * a.z)), fmaf(a.z, b.x, -(b.z * a.x)), fmaf(a.x, b.y, -(b.x * a.y))};
 }
 // algebra.rs:151-158: None for a zero vector.
 __device__ __forceinline__ bool normalize(V3 a, V3& out) {
@@ -117,6 +120,7 @@ struct SceneView {
   const DevMaterial* mats;  // n_mats + 1 entries: the last one is M_AIR
   const DevLight* lights;
   const DevShape* lin;      // shapes tested one by one (all of them, or the unbounded ones beside a BVH)
+  const Node* lin_box;      // padded boxes of lin; lo.w = 1: a plane (no box)
   const Node* bvh;          // BVH nodes (null: no BVH)
   const DevShape* bprims;   // BVH primitives in leaf order
   uint32_t n_lin;
@@ -126,9 +130,41 @@ struct SceneView {
   uint32_t n_mats;          // index of M_AIR
   uint32_t n_lights;
   uint32_t n_emit;           // photons per pass over all lights
+  uint32_t lds;              // bit 0: lin and lin_box in LDS, bit 1: mats in LDS (scene_to_lds)
   DevCamera cam;
   float4 bmin, bmax;        // Morton quantisation box
 };
+
+// ---------------------------------------------------------------------------
+// LDS copies of the shapes tested one by one (with their boxes) and of the
+// materials, for the kernels that walk rays: every ray tests each of those
+// shapes, and from global memory each test waits for its shape's scalar
+// loads.  scene_to_lds(), called by all threads of a block before any ray
+// work, fills them and sets the view's lds bits; a view without those bits
+// (any other kernel, or a scene too large for the copies) reads the global
+// arrays.
+
+constexpr uint32_t LIN_LDS = 32;  // shapes
+constexpr uint32_t MAT_LDS = 48;  // materials (M_AIR included)
+__shared__ DevShape g_lin_lds[LIN_LDS];
+__shared__ Node g_linbox_lds[LIN_LDS];
+__shared__ DevMaterial g_mat_lds[MAT_LDS];
+
+__device__ __forceinline__ void copy16(float4* dst, const float4* src, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
+__device__ __forceinline__ void scene_to_lds(SceneView& sc) {
+  const bool lin = sc.n_lin <= LIN_LDS && sc.lin_box != nullptr;
+  const bool mat = sc.n_mats + 1u <= MAT_LDS;
+  if (lin) {
+    copy16((float4*)g_lin_lds, (const float4*)sc.lin, sc.n_lin * (uint32_t)(sizeof(DevShape) / 16));
+    copy16((float4*)g_linbox_lds, (const float4*)sc.lin_box, sc.n_lin * (uint32_t)(sizeof(Node) / 16));
+  }
+  if (mat) copy16((float4*)g_mat_lds, (const float4*)sc.mats, (sc.n_mats + 1u) * (uint32_t)(sizeof(DevMaterial) / 16));
+  __syncthreads();
+  sc.lds = (lin ? 1u : 0u) | (mat ? 2u : 0u);
+}
 
 // ---------------------------------------------------------------------------
 // Intersection (tracer.rs:306-359, geometry.rs:132-202).
@@ -151,6 +187,7 @@ __device__ __forceinline__ void take(Hit& h, float t, int32_t idx) {
 // method_moller (:149-165) folded into a nearest-hit update.  Candidates with
 // t < NEARLY0 are dropped (tracer.rs:324).
 __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, V3 d, Hit& h) {
+#pragma clang fp contract(off)
   switch (s.kind) {
     case SHAPE_PLAIN: {
       V3 n = mk(s.n);
@@ -205,6 +242,21 @@ __device__ __forceinline__ T ldc(const T* p, uint32_t i) {
 #else
   return p[i];
 #endif
+}
+
+// Shape i of lin, its box, material m: from LDS when scene_to_lds put them
+// there, else from global memory.
+__device__ __forceinline__ DevShape lin_shape(const SceneView& sc, uint32_t i) {
+  if (sc.lds & 1u) return g_lin_lds[i];
+  return ldc(sc.lin, i);
+}
+__device__ __forceinline__ Node lin_box(const SceneView& sc, uint32_t i) {
+  if (sc.lds & 1u) return g_linbox_lds[i];
+  return ldc(sc.lin_box, i);
+}
+__device__ __forceinline__ DevMaterial material(const SceneView& sc, uint32_t m) {
+  if (sc.lds & 2u) return g_mat_lds[m];
+  return sc.mats[m];
 }
 
 // Slab test of a node box: the ray (o, 1/d) enters the box at tn within
@@ -276,7 +328,7 @@ template <bool BVH>
 __device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d, uint32_t* stk) {
   Hit h{INF_F, -1};
   for (uint32_t i = 0; i < sc.n_lin; ++i) {
-    const DevShape s = ldc(sc.lin, i);
+    const DevShape s = lin_shape(sc, i);
     shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
   }
   if (BVH) bvh_nearest(sc, o, d, h, stk);
@@ -293,9 +345,10 @@ struct Isect {
 
 // calc_intersection tail (tracer.rs:336-348): position, normal, flip.
 __device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit h, Isect& is) {
+#pragma clang fp contract(off)
   if (h.obj < 0) return false;
   const DevShape& s = sc.shapes[h.obj];
-  V3 p = o + d * h.t;  // Ray::target (geometry.rs:56-58)
+  V3 p = mk(o.x + d.x * h.t, o.y + d.y * h.t, o.z + d.z * h.t);  // Ray::target (geometry.rs:56-58)
   V3 n;
   if (s.kind == SHAPE_SPHERE) {
     if (!normalize(p - mk(s.p), n)) return false;  // get_normal None
@@ -480,19 +533,140 @@ __device__ __forceinline__ BsdfW bsdf_weights(const DevMaterial& m, float cos0) 
 // the radiance list before the zip (light.rs:132), so sample i's cos pairs
 // with sample i-1's radiance and the last radiance is dropped: the running
 // `prev` carries that shift.
+// illuminated (tracer.rs:280-290) as a shadow test: the sample is lit iff
+// the ray p -> ld meets some shape and its nearest hit lies no more than
+// LIGHT_SLACK (in squared distance) short of the light.  The squared distance
+// |(p + t d) - p|^2 of a hit never decreases with t (every float operation in
+// it is monotonic), so that is: some shape is hit, and no hit has
+// sq_l - sq_o > LIGHT_SLACK (an occluder).  The occluders are looked for
+// among the candidates of `cand` (shapes that can meet a segment from p to
+// the light) and in the BVH, where a lane stops at its first one; only when
+// none of those shapes is hit at all are the other shapes tested, for any
+// hit.  The decision equals that of the nearest-hit test except for a
+// nearest sphere hit whose normal cannot be formed (a sphere of radius ~0).
+__device__ __forceinline__ bool occludes(const DevShape& s, V3 p, V3 d, float sq_l, bool& any) {
+#pragma clang fp contract(off)
+  Hit h{INF_F, -1};
+  shape_hit(s, 0, p, d, h);
+  if (h.obj < 0) return false;
+  any = true;
+  const V3 q = mk(p.x + d.x * h.t, p.y + d.y * h.t, p.z + d.z * h.t);  // as finish_hit forms it
+  const V3 od = q - p;
+  return sq_l - dot(od, od) > LIGHT_SLACK_F;
+}
+
+// Any-hit form of bvh_nearest for the shadow test: true at the first
+// occluder; sets any when some primitive is hit.  Once a hit is known only
+// boxes entered before t_occ (beyond which no hit can be an occluder) are
+// searched.
+__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any, uint32_t* stk) {
+  if (sc.n_bprims == 0) return false;
+  const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const float t_occ = sqrtf(fmaxf(sq_l - LIGHT_SLACK_F, 0.0f)) * 1.001f + 1e-4f;
+  const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
+  uint32_t node = 0, sp = 0;
+  float tn;
+  if (!slab(sc.bvh[0], o, inv, any ? t_occ : INF_F, tn)) return false;
+  while (true) {
+    if (node >= first_leaf) {
+      const uint32_t base = (node - first_leaf) * BVH_LEAF;
+      const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
+      for (uint32_t j = base; j < end; ++j)
+        if (occludes(sc.bprims[j], o, d, sq_l, any)) return true;
+    } else {
+      const uint32_t l = 2u * node + 1u;
+      const float tmax = any ? t_occ : INF_F;
+      float tl, tr;
+      const bool hl = slab(sc.bvh[l], o, inv, tmax, tl);
+      const bool hr = slab(sc.bvh[l + 1u], o, inv, tmax, tr);
+      if (hl && hr) {
+        const bool left_first = tl <= tr;
+        stk[sp * 256u] = left_first ? l + 1u : l;
+        ++sp;
+        node = left_first ? l : l + 1u;
+        continue;
+      }
+      if (hl || hr) {
+        node = hl ? l : l + 1u;
+        continue;
+      }
+    }
+    bool more = false;
+    while (sp > 0) {
+      --sp;
+      node = stk[sp * 256u];
+      if (slab(sc.bvh[node], o, inv, any ? t_occ : INF_F, tn)) {
+        more = true;
+        break;
+      }
+    }
+    if (!more) break;
+  }
+  return false;
+}
+
+// Linear shapes that can meet a segment from p to a point of light L's
+// parallelogram (bit i: shape i of lin; shapes from the 65th on always
+// count): a plane unless p and the four corners lie strictly on one side, a
+// bounded shape whose padded box meets the box of p and the corners.
+constexpr uint32_t CAND_BITS = 64;
+__device__ __forceinline__ uint64_t occluder_candidates(const SceneView& sc, const DevLight& L, V3 p) {
+  const V3 a = mk(L.pos), b = a + mk(L.d1), c = a + mk(L.d2), e = b + mk(L.d2);
+  V3 lo = mk(fminf(fminf(p.x, a.x), fminf(fminf(b.x, c.x), e.x)), fminf(fminf(p.y, a.y), fminf(fminf(b.y, c.y), e.y)),
+             fminf(fminf(p.z, a.z), fminf(fminf(b.z, c.z), e.z)));
+  V3 hi = mk(fmaxf(fmaxf(p.x, a.x), fmaxf(fmaxf(b.x, c.x), e.x)), fmaxf(fmaxf(p.y, a.y), fmaxf(fmaxf(b.y, c.y), e.y)),
+             fmaxf(fmaxf(p.z, a.z), fmaxf(fmaxf(b.z, c.z), e.z)));
+  const float m = fmaxf(fmaxf(fmaxf(fabsf(lo.x), fabsf(lo.y)), fmaxf(fabsf(lo.z), fabsf(hi.x))),
+                        fmaxf(fabsf(hi.y), fabsf(hi.z)));
+  const float pad = 1e-5f * (1.0f + m);
+  lo = lo - mk(pad, pad, pad);
+  hi = hi + mk(pad, pad, pad);
+  uint64_t mask = 0;
+  const uint32_t n = min(sc.n_lin, CAND_BITS);
+  for (uint32_t i = 0; i < n; ++i) {
+    const Node bx = lin_box(sc, i);
+    bool may;
+    if (bx.lo.w != 0.0f) {
+      const DevShape s = lin_shape(sc, i);
+      const V3 nn = mk(s.n);
+      const float tol = 1e-5f * (1.0f + fabsf(s.dist) + m);
+      const float e0 = dot(nn, p) + s.dist, e1 = dot(nn, a) + s.dist, e2 = dot(nn, b) + s.dist;
+      const float e3 = dot(nn, c) + s.dist, e4 = dot(nn, e) + s.dist;
+      const bool above = e0 > tol && e1 > tol && e2 > tol && e3 > tol && e4 > tol;
+      const bool below = e0 < -tol && e1 < -tol && e2 < -tol && e3 < -tol && e4 < -tol;
+      may = !(above || below);
+    } else {
+      may = !(bx.hi.x < lo.x || bx.lo.x > hi.x || bx.hi.y < lo.y || bx.lo.y > hi.y || bx.hi.z < lo.z ||
+              bx.lo.z > hi.z);
+    }
+    if (may) mask |= 1ull << i;
+  }
+  return mask;
+}
+
 template <bool BVH>
-__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, float& cos0sq,
+__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, float& cos0sq,
                                            float& sqd, uint32_t* stk) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
   if (c0 < 0.0f) return false;
-  Isect is;
-  if (!intersect<BVH>(sc, p, ld3, is, stk)) return false;
-  float sq_l = dot(ld, ld);
-  V3 od = is.pos - p;
-  float sq_o = dot(od, od);
-  if (sq_l - sq_o > LIGHT_SLACK_F) return false;
+  const float sq_l = dot(ld, ld);
+  bool any = false;
+  for (uint32_t i = 0; i < sc.n_lin; ++i) {
+    if (i < CAND_BITS && ((cand >> i) & 1ull) == 0) continue;
+    if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
+  }
+  if (BVH && bvh_shadow(sc, p, ld3, sq_l, any, stk)) return false;
+  // no occluder; lit iff the ray meets something at all (calc_intersection
+  // returns None otherwise, tracer.rs:284)
+  for (uint32_t i = 0; i < min(sc.n_lin, CAND_BITS) && !any; ++i) {
+    if ((cand >> i) & 1ull) continue;
+    Hit h{INF_F, -1};
+    shape_hit(lin_shape(sc, i), 0, p, ld3, h);
+    any = h.obj >= 0;
+  }
+  if (!any) return false;
   cos0sq = c0 * c0;
   sqd = sq_l;
   return true;
@@ -532,12 +706,13 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n, uint32_t* stk) {
     const DevLight L = ldc(sc.lights, li);
     if (L.kind != LIGHT_PARALLELOGRAM) continue;
     const V3 ln = mk(L.nvec);
+    const uint64_t cand = occluder_candidates(sc, L, p);
     C3 prev{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < 25; ++s) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BVH>(sc, p, n, ld, csq, sqd, stk)) continue;
+      if (!sample_lit<BVH>(sc, p, n, ld, cand, csq, sqd, stk)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }

@@ -34,6 +34,7 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
   v.mats = b.mats;
   v.lights = b.lights;
   v.lin = b.lin;
+  v.lin_box = b.lin_box;
   v.bvh = b.bvh;
   v.bprims = b.bprims;
   v.n_lin = b.n_lin;
@@ -43,6 +44,7 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
   v.n_mats = b.n_mats;
   v.n_lights = b.n_lights;
   v.n_emit = b.n_emit_total;
+  v.lds = 0;
   v.cam = b.cam;
   v.bmin = b.bmin;
   v.bmax = b.bmax;
@@ -74,6 +76,11 @@ __device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_
   s.n_stored = 0;
   s.next_photon = 0;
   s.n_queries = 0;
+  s.direct_next = 0;
+  s.shade_next = 0;
+  s.eye_next = 0;
+  s.tiles_front = 0;
+  s.tiles_back = 0;
   *st = s;
 }
 
@@ -162,7 +169,7 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
   if ((int)p.depth >= maxd) return false;  // tracer.rs:32
   Isect is;
   if (!intersect<BVH>(sc, p.o, p.d, is, stk)) return false;  // tracer.rs:36-39
-  const DevMaterial& M = sc.mats[is.mat];
+  const DevMaterial M = material(sc, is.mat);
   // tracer.rs:77-79
   store = (uc == 0 || p.depth > 0) && surf_store(M);
   spos = is.pos;
@@ -197,7 +204,7 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
     float ior2 = chan(M.ior, p.wl);
     if (ior2 == 0.0f) return false;
     // reflect_trans (tracer.rs:111-125)
-    float eta = relative_ior(chan(sc.mats[p.med].ior, p.wl), ior2);
+    float eta = relative_ior(chan(material(sc, p.med).ior, p.wl), ior2);
     V3 t;
     float c2;
     if (!specular_refraction(is.n, p.d, eta, t, c2)) return false;
@@ -209,7 +216,7 @@ __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int m
   }
   if (M.surface == SURF_TS) {
     // Surface::next_direction (surface.rs:211-258), medium rule tracer.rs:65-69
-    float eta = relative_ior(chan(sc.mats[p.med].ior, p.wl), chan(M.ior, p.wl));
+    float eta = relative_ior(chan(material(sc, p.med).ior, p.wl), chan(M.ior, p.wl));
     float cos1;
     V3 rdir0 = specular_reflection(is.n, p.d, cos1);
     V3 rdir = reflection_glossy(is.n, rdir0, power_glossy(M), rng);
@@ -281,6 +288,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   __shared__ float part[ADJ ? 3 * ADJ_LDS_MATS : 1];
   __shared__ uint16_t evl[ADJ ? ADJ_MAX_EV * 256 : 1];
+  scene_to_lds(sc);
   const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (ADJ) {
     for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
@@ -339,7 +347,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
             if (A != 0.0f)
               for (uint32_t k = 0; k < n_ev; ++k) {
                 const uint32_t mi = evl[k * 256u + threadIdx.x];
-                const float a = chan(sc.mats[mi].c1, swl);
+                const float a = chan(material(sc, mi).c1, swl);
                 if (a != 0.0f) adj_add(part, at.grad, mi, swl, A / a);
               }
           } else {
@@ -857,6 +865,15 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
   return hipGetLastError();
 }
 
+// Next work item (query group, tile) for this wave from a pass counter, one
+// atomic per item: the persistent kernels take items until they run out, so
+// costly ones (dense caustics, glass) do not leave other waves idle.
+__device__ __forceinline__ uint32_t wave_next(uint32_t* counter) {
+  uint32_t g = 0;
+  if (lane_id() == 0) g = atomicAdd(counter, 1u);
+  return __builtin_amdgcn_readfirstlane(g);
+}
+
 // ===========================================================================
 // Photon grid of a pass.  Every gather query of a pass has the same radius r
 // (util/iterator.rb:33-38), so the pass's photons are binned into cubic cells
@@ -1035,15 +1052,19 @@ __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const Photo
                                                  float qy, float qz, float r2, OnPhoton& on_photon) {
   const uint2 se = cells[b];
   uint32_t j = se.x;
-  // two whole records per step: both loads are in flight together
-  for (; j + 1 < se.y; j += 2) {
-    const PhotonRec p0 = ph[j], p1 = ph[j + 1];
+  // four whole records per step: their loads are in flight together
+  for (; j + 3 < se.y; j += 4) {
+    const PhotonRec p0 = ph[j], p1 = ph[j + 1], p2 = ph[j + 2], p3 = ph[j + 3];
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
     const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
+    const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
+    const float d3 = dist2_nc(qx, qy, qz, p3.pm.x, p3.pm.y, p3.pm.z);
     if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
     if (d1 <= r2) on_photon(j + 1, p1.pm, p1.dm, d1);
+    if (d2 <= r2) on_photon(j + 2, p2.pm, p2.dm, d2);
+    if (d3 <= r2) on_photon(j + 3, p3.pm, p3.dm, d3);
   }
-  if (j < se.y) {
+  for (; j < se.y; ++j) {
     const PhotonRec p0 = ph[j];
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
     if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
@@ -1231,42 +1252,71 @@ struct EyeNode {
   uint32_t meta;  // depth (5 bits) | medium (11 bits) << 5 | path (16 bits) << 16
 };
 
-// Persistent eye kernel.  Wave w owns the 8x8 tiles w, w + W, w + 2W, ...
-// (W waves; spread over the image, so that the costly pixels behind glass,
-// which cluster, are shared out) and keeps 64 trees in flight: each loop
-// step walks one node per lane, and a lane whose tree is done takes the next
-// pixel slot of the wave's tiles (ballot + mbcnt), so a deep tree holds one
-// lane, not a wave.  The pixel's queries go to its own
-// slots: the primary one to t, the k-th deeper one to q_primary*(1+k)+t,
+// Order in which the eye waves take tiles: tiles whose queries (last pass's
+// count; zero before the first pass) exceed TILE_HEAVY first, in any order,
+// the others after them.  The order changes which wave walks a pixel, never
+// its results (a pixel's queries have fixed slots).
+constexpr uint32_t TILE_HEAVY = 96;
+__device__ __forceinline__ void k_tile_order_body(EyeBuf eb, PassState* st, Blk blk) {
+  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
+  if (t >= eb.n_tiles) return;
+  if (eb.tile_count[t] > TILE_HEAVY)
+    eb.tile_order[atomicAdd(&st->tiles_front, 1u)] = t;
+  else
+    eb.tile_order[eb.n_tiles - 1u - atomicAdd(&st->tiles_back, 1u)] = t;
+}
+
+__global__ __launch_bounds__(256) void k_tile_order(EyeBuf eb, PassState* st) {
+  k_tile_order_body(eb, st, Blk{blockIdx.x, gridDim.x});
+}
+
+// Persistent eye kernel.  A wave takes tiles in the order of tile_order from
+// the pass's counter, and keeps 64 trees in flight: each loop step walks one
+// node per lane, and a lane whose tree is done takes the next pixel slot of
+// the wave's current tile (ballot + mbcnt), the wave taking a new tile when
+// its tile is used up; so a deep tree holds one lane, not a wave, and the
+// costly tiles (behind glass) start first.  The pixel's queries go to its
+// own slots: the primary one to t, the k-th deeper one to q_primary*(1+k)+t,
 // further ones to the overflow region.
 template <bool BVH>
 __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves, Blk blk) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
+  scene_to_lds(sc);
   const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (w >= n_waves) return;
   const uint32_t pass = st->pass;
-  // this wave's slots, numbered v = 64 k + lane-in-tile for its k-th tile
-  const uint32_t my_tiles = w < eb.n_tiles ? (eb.n_tiles - w + n_waves - 1) / n_waves : 0u;
-  const uint32_t s_end = my_tiles * 64u;
   // draws of camera_ray on the pixel's root stream, which node 0 continues
   const uint32_t cam_draws = (sc.cam.blur ? 2u : 0u) + (sc.cam.progressive && sc.cam.antialias ? 2u : 0u);
-  uint32_t next = 0;  // wave-uniform
+  uint32_t tile = 0, used = 64;  // wave-uniform: current tile, its slots handed out
+  bool tiles_left = true;        // wave-uniform
   bool alive = false, done = false;
   uint32_t t = 0, pixel = 0, n_sec = 0;
   C3 acc{0.0f, 0.0f, 0.0f};
   EyeNode cur;
-  EyeNode stack[MAX_DEPTH_LIMIT];
-  int sp = 0;
+  // pending refraction nodes in global memory (a private array would need
+  // scratch, which limits how many waves the hardware keeps resident)
+  const uint32_t L = eb.estack_lanes, gl = w * 64u + lane_id();
+  uint32_t sp = 0;
   while (true) {
     const bool need = !alive && !done;
     const uint64_t m = __ballot(need);
     if (m) {
+      if (used == 64u && tiles_left) {
+        const uint32_t k = wave_next(&st->eye_next);
+        if (k < eb.n_tiles) {
+          tile = eb.tile_order[k];
+          used = 0;
+        } else {
+          tiles_left = false;
+        }
+      }
+      const uint32_t avail = 64u - used;
+      const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
       if (need) {
-        const uint32_t v = next + (uint32_t)__popcll(m & lanemask_lt());
-        if (v < s_end) {
-          t = (w + (v >> 6) * n_waves) * 64u + (v & 63u);
-          const uint32_t tile = t >> 6, wi = t & 63u;
+        if (rank < avail) {
+          t = tile * 64u + used + rank;
+          const uint32_t wi = t & 63u;
           const int x = (int)((tile % (uint32_t)sc.cam.tiles_x) * 8u + (wi & 7u));
           const int y = (int)((tile / (uint32_t)sc.cam.tiles_x) * 8u + (wi >> 3));
           eb.q[t].a.w = __uint_as_float(NO_PIXEL);
@@ -1284,11 +1334,11 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
           } else {
             eb.sec_count[t] = 0;  // a slot past the image edge: no pixel
           }
-        } else {
+        } else if (!tiles_left) {
           done = true;
         }
       }
-      next += (uint32_t)__popcll(m);
+      used += min((uint32_t)__popcll(m), avail);
     }
     if (!__any(alive || !done)) break;
     if (!alive) continue;
@@ -1299,8 +1349,8 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
     EyeNode nxt;
     Isect is;
     if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is, stk + threadIdx.x)) {
-      const DevMaterial& M = sc.mats[is.mat];
-      const DevMaterial& Mm = sc.mats[med];
+      const DevMaterial M = material(sc, is.mat);
+      const DevMaterial Mm = material(sc, med);
       Rng nr;
       nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
       if (depth == 0)
@@ -1354,7 +1404,13 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
           const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
           const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
           if (rc) {
-            if (sp < MAX_DEPTH_LIMIT) stack[sp++] = c;
+            if (sp < eb.estack_depth) {
+              float4* e = eb.estack + (uint64_t)(3u * sp) * L + gl;
+              e[0] = make_float4(c.o.x, c.o.y, c.o.z, __uint_as_float(c.meta));
+              e[L] = make_float4(c.d.x, c.d.y, c.d.z, c.w.r);
+              e[2 * (uint64_t)L] = make_float4(c.w.g, c.w.b, 0.0f, 0.0f);
+              ++sp;
+            }
           } else {
             nxt = c;
             next_set = true;
@@ -1369,7 +1425,13 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
     if (next_set) {
       cur = nxt;
     } else if (sp > 0) {
-      cur = stack[--sp];
+      --sp;
+      const float4* e = eb.estack + (uint64_t)(3u * sp) * L + gl;
+      const float4 a = e[0], b = e[L], c = e[2 * (uint64_t)L];
+      cur.o = mk(a.x, a.y, a.z);
+      cur.meta = __float_as_uint(a.w);
+      cur.d = mk(b.x, b.y, b.z);
+      cur.w = C3{b.w, c.x, c.y};
     } else {
       eb.pix[pixel] = make_float4(acc.r, acc.g, acc.b, 0.0f);
       eb.sec_count[t] = n_sec;
@@ -1401,7 +1463,8 @@ __global__ __launch_bounds__(256) void k_tile_count(EyeBuf eb) {
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu) {
-  uint32_t n_waves = std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u);
+  hipLaunchKernelGGL(k_tile_order, dim3((eb.n_tiles + 255) / 256), dim3(256), 0, s, eb, st);
+  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u), eb.estack_lanes / 64u);
   if (n_waves < 1) n_waves = 1;
   const uint32_t blocks = (n_waves + 3) / 4;
   if (sc.bvh)
@@ -1544,48 +1607,43 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 }
 
 // Shading of the queries in Morton order, one query per lane, 64 spatially
-// adjacent queries per wave (their photons mostly share cache lines).  Each
-// lane gathers over the cells of the pass's photon grid that its ball meets:
-//   res[slot] = w * (direct light (when uc) + scale * sum filter * max(0,-n.dir))
+// adjacent queries per wave (their photons mostly share cache lines); waves
+// take groups of 64 from the pass counter.  Each lane gathers over the cells
+// of the pass's photon grid that its ball meets:
+//   res[slot] = w * (scale * sum filter * max(0,-n.dir) + direct light (uc))
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
-// BSDF factor and path throughput.  A query whose cells exceed the grid's
-// span rule raises flag 4 of the pass state.
+// BSDF factor and path throughput.  Gather and shadow rays share the kernel
+// so that the waves of one hide the memory latency of the other.  A query
+// whose cells exceed the grid's span rule raises flag 4 of the pass state.
 template <bool BVH>
 __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
-                                               EyeBuf eb, const uint32_t* order, int pf, int uc, Blk blk) {
+                                               EyeBuf eb, const uint32_t* order, int pf, int uc) {
   __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
+  scene_to_lds(sc);
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
   const GridView g = grid_view(st, sc.bmin);
   const uint32_t groups = (nq + 63) / 64;
-  const uint32_t waves = blk.n * 4;
   bool complete = true;
-  for (uint32_t gi = blk.b * 4 + wib; gi < groups; gi += waves) {
-    const uint32_t i = gi * 64 + lane;
-    const bool active = i < nq;
-    uint32_t slot = 0;
-    Query qq{};
-    if (active) {
-      slot = order[i];
-      qq = eb.q[slot];
-    }
+  for (uint32_t gi = wave_next(&st->shade_next); gi < groups; gi = wave_next(&st->shade_next)) {
+    const uint32_t i = gi * 64 + lane_id();
+    if (i >= nq) continue;
+    const uint32_t slot = order[i];
+    const Query qq = eb.q[slot];
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-    if (active) {
-      const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
-      complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
-                             [&](uint32_t, float4 pm, float4 dm, float d2) {
-                               const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
-                               const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
-                               const uint32_t wl = __float_as_uint(pm.w) & 3u;
-                               ar += wl == 0 ? v : 0.0f;
-                               ag += wl == 1 ? v : 0.0f;
-                               ab += wl == 2 ? v : 0.0f;
-                             });
-    }
+    const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+    complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
+                           [&](uint32_t, float4 pm, float4 dm, float d2) {
+                             const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
+                             const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
+                             const uint32_t wl = __float_as_uint(pm.w) & 3u;
+                             ar += wl == 0 ? v : 0.0f;
+                             ag += wl == 1 ? v : 0.0f;
+                             ab += wl == 2 ? v : 0.0f;
+                           });
     C3 di = C3{ar * scale, ag * scale, ab * scale};
     if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b), stk + threadIdx.x);
-    if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
+    eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
   if (!complete) atomicOr(&st->overflow, 4u);
 }
@@ -1593,7 +1651,7 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
 template <bool BVH>
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc) {
-  k_shade_body<BVH>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+  k_shade_body<BVH>(sc, cells, ph, st, eb, order, pf, uc);
 }
 
 // The same shading over a box-tree map (ppm_trace_eye on a PhotonMap, whose
@@ -1842,9 +1900,7 @@ __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, do
   k_accumulate_body(eb, cam, sum, sumsq, Blk{blockIdx.x, gridDim.x});
 }
 
-hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
-                         int pfilter, int use_classic, int n_cu, double* sum, double* sumsq, hipEvent_t* ev) {
-  const uint32_t blocks = (uint32_t)n_cu * 8u;
+hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb) {
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, eb.tile_count, eb.tile_off, eb.n_tiles,
                      &st->n_qsort, (const uint32_t*)&st->n_queries, eb.ovf_cap);
   const uint32_t qmax = (1u << m.morton_bits) - 1u;
@@ -1856,13 +1912,26 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   uint32_t* qv[2] = {eb.qvals[0], eb.qvals[1]};
   int cur = 0;
   e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 32u, &cur);
-  if (e != hipSuccess) return e;
+  if (e == hipSuccess && cur != 0) e = hipErrorInvalidValue;  // the sorted order is read from qvals[0]
+  return e;
+}
+
+hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
+                         int pfilter, int use_classic, int n_cu, double* sum, double* sumsq, hipEvent_t* ev,
+                         bool prepared) {
+  const uint32_t blocks = (uint32_t)n_cu * 8u;
+  hipError_t e;
+  if (!prepared && (e = launch_query_prep(s, sc, m, st, eb)) != hipSuccess) return e;
+  uint32_t* const* qv = eb.qvals;
+  const int cur = 0;
   if (ev && (e = hipEventRecord(ev[0], s)) != hipSuccess) return e;
   const SceneView v = make_view(sc);
   if (m.cells) {
+    // persistent: at least as many waves as stay resident (8 blocks of 4
+    // waves per CU); waves beyond that find the groups taken and leave
     if (sc.bvh)
-      hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
-                         pfilter, use_classic);
+      hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur], pfilter,
+                         use_classic);
     else
       hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
                          pfilter, use_classic);
@@ -2117,7 +2186,10 @@ __global__ __launch_bounds__(256) void kb_grid_permute(const BatchItem* items, c
 __global__ __launch_bounds__(RS_THREADS) void kb_rs_hist(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                          uint32_t n_all, int which, uint32_t p) {
   BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
-  const uint32_t* keys = which ? it.eb.qkeys[p & 1u] : it.m.keys[p & 1u];
+  // selects, not a runtime index into the item's arrays (which would put the
+  // whole item in scratch memory)
+  const bool odd = (p & 1u) != 0;
+  const uint32_t* keys = which ? (odd ? it.eb.qkeys[1] : it.eb.qkeys[0]) : (odd ? it.m.keys[1] : it.m.keys[0]);
   k_rs_hist_body(keys, which ? &it.st->n_qsort : &it.st->n_stored, which ? it.eb.qghist : it.m.ghist, 8u * p, blk);
 }
 
@@ -2131,13 +2203,21 @@ __global__ __launch_bounds__(1024) void kb_rs_scan(const BatchItem* items, const
 __global__ __launch_bounds__(RS_THREADS) void kb_rs_scatter(const BatchItem* items, const uint32_t* tab,
                                                             uint32_t n, uint32_t n_all, int which, uint32_t p) {
   BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
-  const uint32_t i0 = p & 1u, i1 = i0 ^ 1u;
+  const bool odd = (p & 1u) != 0;
   if (which)
-    k_rs_scatter_body(it.eb.qkeys[i0], it.eb.qvals[i0], it.eb.qkeys[i1], it.eb.qvals[i1], &it.st->n_qsort,
+    k_rs_scatter_body(odd ? it.eb.qkeys[1] : it.eb.qkeys[0], odd ? it.eb.qvals[1] : it.eb.qvals[0],
+                      odd ? it.eb.qkeys[0] : it.eb.qkeys[1], odd ? it.eb.qvals[0] : it.eb.qvals[1], &it.st->n_qsort,
                       it.eb.qghist, it.eb.qtotals, 8u * p, blk);
   else
-    k_rs_scatter_body(it.m.keys[i0], it.m.vals[i0], it.m.keys[i1], it.m.vals[i1], &it.st->n_stored, it.m.ghist,
-                      it.m.totals, 8u * p, blk);
+    k_rs_scatter_body(odd ? it.m.keys[1] : it.m.keys[0], odd ? it.m.vals[1] : it.m.vals[0],
+                      odd ? it.m.keys[0] : it.m.keys[1], odd ? it.m.vals[0] : it.m.vals[1], &it.st->n_stored,
+                      it.m.ghist, it.m.totals, 8u * p, blk);
+}
+
+__global__ __launch_bounds__(256) void kb_tile_order(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                     uint32_t n_all) {
+  BATCH_PROLOGUE(BK_PIX)
+  k_tile_order_body(it.eb, it.st, blk);
 }
 
 template <bool BVH>
@@ -2163,7 +2243,8 @@ template <bool BVH>
 __global__ __launch_bounds__(256) void kb_shade(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                 uint32_t n_all) {
   BATCH_PROLOGUE(BK_SHADE)
-  k_shade_body<BVH>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc, blk);
+  (void)blk;
+  k_shade_body<BVH>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc);
 }
 
 __global__ __launch_bounds__(256) void kb_overflow_add(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -2200,6 +2281,7 @@ hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32
   hipLaunchKernelGGL(kb_grid_clear, dim3(blocks[BK_CLEAR]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_grid_cells, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_grid_permute, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_tile_order, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
   if (bvh)
     hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
   else

@@ -15,6 +15,7 @@ struct SceneBuf {
   DevMaterial* mats = nullptr;  // n_mats + 1 (M_AIR last)
   DevLight* lights = nullptr;
   DevShape* lin = nullptr;      // shapes tested one by one (== shapes when there is no BVH)
+  Node* lin_box = nullptr;      // padded box of each lin shape; lo.w = 1 marks a plane
   Node* bvh = nullptr;          // 2^(bvh_depth+1) - 1 nodes, or null
   DevShape* bprims = nullptr;   // n_bprims shapes in leaf order
   uint32_t n_lin = 0, n_bprims = 0, bvh_depth = 0;
@@ -78,15 +79,25 @@ struct EyeBuf {
   float4* pix = nullptr;  // per-pixel float radiance of the pass (row-major)
   uint32_t n_pix = 0;
   uint32_t n_tiles = 0;
-  uint32_t* tile_count = nullptr;
+  uint32_t* tile_count = nullptr;   // queries per tile (of the last pass until k_tile_count)
   uint32_t* tile_off = nullptr;
+  uint32_t* tile_order = nullptr;   // the eye kernel's tile order (k_tile_order)
   uint32_t* qkeys[2] = {nullptr, nullptr};
   uint32_t* qvals[2] = {nullptr, nullptr};
   uint32_t* qghist = nullptr;
   uint32_t* qtotals = nullptr;
   uint32_t q_sort_blocks = 0;
   float4* res = nullptr;
+  // pending refraction nodes of the eye trees, per lane of the eye kernel:
+  // entry k of lane L is the three float4 estack[(3k + c) * estack_lanes + L]
+  // (lanes of a wave at the same depth touch adjacent addresses)
+  float4* estack = nullptr;
+  uint32_t estack_lanes = 0;  // 64 * the most eye waves a launch may use
+  uint32_t estack_depth = 0;  // entries per lane
 };
+
+// Eye waves per CU at most (the launch configurations a batch may tune to).
+constexpr uint32_t EYE_WAVES_PER_CU_MAX = 32;
 
 hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
                              double power);
@@ -123,13 +134,18 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu);
 
-// Gather (over the pass's photon grid) + direct light of one pass over the
-// Morton-sorted queries, then the per-pixel sum.  sum/sumsq != null: the pixel's pass radiance is added into
-// the FP64 sums; otherwise it is left in pix.  ev (optional, 2 events)
-// brackets the k_shade launch alone.
+// Query order of the gather: per-tile offsets, Morton keys of the eye
+// queries, their sort (needs only the eye pass, not the photons).
+hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb);
+
+// Gather (over the pass's photon grid or a box-tree map) + direct light of
+// one pass over the Morton-sorted queries, then the per-pixel sum (after
+// launch_query_prep, run here unless `prepared`).  sum/sumsq != null: the
+// pixel's pass radiance is added into the FP64 sums; otherwise it is left in
+// pix.  ev (optional, 2 events) brackets the k_shade launch alone.
 hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
                          int pfilter, int use_classic, int n_cu, double* sum, double* sumsq,
-                         hipEvent_t* ev = nullptr);
+                         hipEvent_t* ev = nullptr, bool prepared = false);
 
 // Test hooks (radius query / gather on host-provided query points).
 // mode 0: count, 1: list, 2: gather rgb

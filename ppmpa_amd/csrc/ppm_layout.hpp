@@ -159,6 +159,15 @@ struct alignas(16) PassState {
   float g_reach;
   uint32_t g_mask;
   int32_t g_lim;
+  // work counters of the persistent kernels (zeroed by k_pass_begin): next
+  // query group of k_shade, next tile of k_eye, and the two ends of the
+  // eye's tile order (costly tiles first)
+  uint32_t direct_next;  // unused
+  uint32_t shade_next;
+  uint32_t eye_next;
+  uint32_t tiles_front;
+  uint32_t tiles_back;
+  uint32_t pad1[3];
 };
 
 }  // namespace ppm
