@@ -156,7 +156,9 @@ typedef struct {
   uint64_t seed;
   uint64_t n_photon;     /* photons per pass (ppmpa.rs:30 'nphoton') */
   uint32_t first_pass;   /* 1-based index of the first pass to run */
-  uint32_t reserved0;
+  uint32_t pipelines;    /* passes in flight at once, each with its own buffers
+                            (util/iterator.rb:18 NPARA = 4 parallel pm | rt
+                            runs); 0 = automatic, at most 4 */
   double   radius0;      /* r_1, the radius of pass 1 (not squared) */
   double   alpha;        /* iterator.rb:17 ALPHA */
   int32_t  use_classic;  /* ppmpa.rs:16 DEF_USECLASSIC; "-nc" clears it */

@@ -112,7 +112,7 @@ class PassCfg(C.Structure):
         ("seed", C.c_uint64),
         ("n_photon", C.c_uint64),
         ("first_pass", C.c_uint32),
-        ("reserved0", C.c_uint32),
+        ("pipelines", C.c_uint32),
         ("radius0", C.c_double),
         ("alpha", C.c_double),
         ("use_classic", C.c_int32),

@@ -61,28 +61,35 @@ struct ppm_map {
   bool built = false;
 };
 
-struct ppm_session {
-  ppm_ctx* ctx = nullptr;
-  const ppm_scene* scene = nullptr;
-  ppm_pass_cfg cfg{};
-  SceneBuf sbuf;
-  DevLight* lights = nullptr;
+// One pass pipeline of a session: the buffers, pass state, FP64 sums and
+// streams of every K-th pass (K = the session's pipelines), and its captured
+// pass graph.  Passes of different pipelines run at once, as the reference's
+// iterator runs NPARA pm | rt processes at once (util/iterator.rb:18,90-117).
+struct Pipe {
   PhotonBuf pb;
   MapBuf map;
   EyeBuf eb;
   double* sum = nullptr;
   double* sumsq = nullptr;
   PassState* st = nullptr;
-  double power = 0.0;
-  uint32_t passes_done = 0;
+  hipStream_t stream = nullptr;  // this pipeline's passes, in order
+  hipStream_t s2 = nullptr;      // the eye branch of its passes
+  hipEvent_t evf = nullptr, evj = nullptr, done = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+};
+
+struct ppm_session {
+  ppm_ctx* ctx = nullptr;
+  const ppm_scene* scene = nullptr;
+  ppm_pass_cfg cfg{};
+  SceneBuf sbuf;
+  DevLight* lights = nullptr;
+  double power = 0.0;
+  uint32_t passes_done = 0;
+  std::vector<Pipe> pipes;  // pass p (from cfg.first_pass) runs on pipes[p % K]
   hipEvent_t ev[10] = {};
   ppm_timings t{};
-  // the eye branch of a pass (eye rays, query order) runs on s2, beside the
-  // photon branch (trace, grid build) on the main stream: fork and join
-  hipStream_t s2 = nullptr;
-  hipEvent_t evf = nullptr, evj = nullptr;
 };
 
 namespace {
@@ -382,7 +389,7 @@ namespace {
 // BVH_MIN_PRIMS bounded shapes in automatic mode), the device-built BVH over
 // the bounded ones.  shapes[] is already on the device as sc->buf.shapes.
 // Box of a bounded shape (sphere, polygon, parallelogram) in double.
-void shape_box(const DevShape& s, double lo[3], double hi[3]) {
+static void shape_box(const DevShape& s, double lo[3], double hi[3]) {
   const double p[3] = {s.p.x, s.p.y, s.p.z};
   if (s.kind == SHAPE_SPHERE) {
     const double r = std::fabs((double)s.radius);
@@ -400,11 +407,11 @@ void shape_box(const DevShape& s, double lo[3], double hi[3]) {
   }
 }
 
-ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags);
+static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags);
 
 // Padded float boxes of the shapes tested one by one, for the shadow test's
 // candidate culling: a plane is marked by lo.w = 1, a point gets an empty box.
-hipError_t upload_lin_boxes(SceneBuf& b, const std::vector<DevShape>& lin) {
+static hipError_t upload_lin_boxes(SceneBuf& b, const std::vector<DevShape>& lin) {
   std::vector<Node> boxes(std::max<size_t>(lin.size(), 1));
   for (size_t i = 0; i < lin.size(); ++i) {
     const DevShape& s = lin[i];
@@ -439,7 +446,7 @@ hipError_t upload_lin_boxes(SceneBuf& b, const std::vector<DevShape>& lin) {
 // the bounded shapes whose box sides are all within a quarter of the scene's
 // extent; planes and larger shapes (walls) are tested one by one.
 // shapes[] is already on the device as sc->buf.shapes.
-ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
+static ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
   const ppm_status r = scene_layout_bvh(c, sc, shapes, flags);
   if (r != PPM_OK) return r;
   const hipError_t e = upload_lin_boxes(sc->buf, sc->lin_host);
@@ -447,7 +454,7 @@ ppm_status scene_layout(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& 
   return PPM_OK;
 }
 
-ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
+static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<DevShape>& shapes, int flags) {
   SceneBuf& b = sc->buf;
   std::vector<uint32_t> bounded;
   for (size_t i = 0; i < shapes.size(); ++i) {
@@ -1162,27 +1169,43 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
 // ---------------------------------------------------------------------------
 // session
 
+static void pipe_drop_graph(Pipe& p) {
+  if (p.exec) (void)hipGraphExecDestroy(p.exec);
+  if (p.graph) (void)hipGraphDestroy(p.graph);
+  p.exec = nullptr;
+  p.graph = nullptr;
+}
+
 void ppm_session_destroy(ppm_session* s) {
   if (!s) return;
   if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
-  if (s->exec) (void)hipGraphExecDestroy(s->exec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
+  for (Pipe& p : s->pipes) {
+    if (p.stream) (void)hipStreamSynchronize(p.stream);
+    if (p.s2) (void)hipStreamSynchronize(p.s2);
+    pipe_drop_graph(p);
+    if (p.stream && p.stream != s->ctx->stream) (void)hipStreamDestroy(p.stream);
+    if (p.s2) (void)hipStreamDestroy(p.s2);
+    for (hipEvent_t e : {p.evf, p.evj, p.done})
+      if (e) (void)hipEventDestroy(e);
+    photons_free(p.pb);
+    map_free(p.map);
+    eye_free(p.eb);
+    dfree(p.sum);
+    dfree(p.sumsq);
+    dfree(p.st);
+  }
   for (auto& e : s->ev)
     if (e) (void)hipEventDestroy(e);
-  if (s->s2) {
-    (void)hipStreamSynchronize(s->s2);
-    (void)hipStreamDestroy(s->s2);
-  }
-  if (s->evf) (void)hipEventDestroy(s->evf);
-  if (s->evj) (void)hipEventDestroy(s->evj);
   dfree(s->lights);
-  photons_free(s->pb);
-  map_free(s->map);
-  eye_free(s->eb);
-  dfree(s->sum);
-  dfree(s->sumsq);
-  dfree(s->st);
   delete s;
+}
+
+// Pipelines of a session: cfg.pipelines, else 2 when a pass is large enough
+// (at least 256k photons or 256k pixels) for two passes in flight to keep
+// the GPU busier than one, else 1.
+static uint32_t session_pipelines(const ppm_pass_cfg& cfg, uint64_t n_emit, uint64_t n_pix) {
+  if (cfg.pipelines) return std::min<uint32_t>(cfg.pipelines, 4);
+  return (n_emit >= (1u << 18) || n_pix >= (1u << 18)) ? 2u : 1u;
 }
 
 ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, ppm_session** out) {
@@ -1193,6 +1216,7 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
   if (!(cfg->radius0 > 0.0) || !std::isfinite(cfg->radius0) || !std::isfinite(cfg->alpha))
     return fail(c, PPM_E_INVALID, "bad radius0/alpha");
   if (cfg->pfilter < 0 || cfg->pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
+  if (cfg->pipelines > 4) return fail(c, PPM_E_INVALID, "at most 4 pipelines");
   if (sc->lights.empty() || !(sc->total_flux > 0.0)) return fail(c, PPM_E_INVALID, "scene has no emitting light");
   ppm_session* s = new ppm_session();
   s->ctx = c;
@@ -1212,29 +1236,42 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
   }
   s->sbuf = sc->buf;
   s->sbuf.n_emit_total = (uint32_t)total;
+  const uint64_t n_pix = (uint64_t)sc->buf.cam.xreso * (uint64_t)sc->buf.cam.yreso;
+  s->cfg.pipelines = session_pipelines(s->cfg, total, n_pix);
+  s->pipes.resize(s->cfg.pipelines);
   hipError_t e = hipSuccess;
   do {
     if ((e = dalloc(&s->lights, L.size())) != hipSuccess) break;
     if ((e = hipMemcpy(s->lights, L.data(), sizeof(DevLight) * L.size(), hipMemcpyHostToDevice)) != hipSuccess) break;
     s->sbuf.lights = s->lights;
-    if ((e = photons_alloc(s->pb, cap, trace_waves(total, c->n_cu))) != hipSuccess) break;
-    photons_layout(s->pb, total, cfg->max_depth, c->n_cu);
-    if ((e = map_alloc(s->map, cap, c->n_cu, true)) != hipSuccess) break;
-    if ((e = eye_alloc(s->eb, sc->buf.cam, cfg->max_depth, c->n_cu)) != hipSuccess) break;
-    const uint64_t n3 = 3ull * s->eb.n_pix;
-    if ((e = dalloc(&s->sum, n3)) != hipSuccess) break;
-    if ((e = dalloc(&s->sumsq, n3)) != hipSuccess) break;
-    if ((e = hipMemset(s->sum, 0, sizeof(double) * n3)) != hipSuccess) break;
-    if ((e = hipMemset(s->sumsq, 0, sizeof(double) * n3)) != hipSuccess) break;
-    if ((e = dalloc(&s->st, 1)) != hipSuccess) break;
-    PassState h = host_state();
-    if ((e = hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) break;
+    for (Pipe& p : s->pipes) {
+      if ((e = photons_alloc(p.pb, cap, trace_waves(total, c->n_cu))) != hipSuccess) break;
+      photons_layout(p.pb, total, cfg->max_depth, c->n_cu);
+      if ((e = map_alloc(p.map, cap, c->n_cu, true)) != hipSuccess) break;
+      if ((e = eye_alloc(p.eb, sc->buf.cam, cfg->max_depth, c->n_cu)) != hipSuccess) break;
+      const uint64_t n3 = 3ull * p.eb.n_pix;
+      if ((e = dalloc(&p.sum, n3)) != hipSuccess) break;
+      if ((e = dalloc(&p.sumsq, n3)) != hipSuccess) break;
+      if ((e = hipMemset(p.sum, 0, sizeof(double) * n3)) != hipSuccess) break;
+      if ((e = hipMemset(p.sumsq, 0, sizeof(double) * n3)) != hipSuccess) break;
+      if ((e = dalloc(&p.st, 1)) != hipSuccess) break;
+      PassState h = host_state();
+      if ((e = hipMemcpy(p.st, &h, sizeof(h), hipMemcpyHostToDevice)) != hipSuccess) break;
+      // pipeline 0 runs on the context's stream: K pipelines then use 2K
+      // streams in all, which HIP spreads over that many hardware queues
+      // (GPU_MAX_HW_QUEUES, 4 by default) when K <= 2
+      if (&p == &s->pipes[0])
+        p.stream = c->stream;
+      else if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
+        break;
+      if ((e = hipStreamCreateWithFlags(&p.s2, hipStreamNonBlocking)) != hipSuccess) break;
+      if ((e = hipEventCreateWithFlags(&p.evf, hipEventDisableTiming)) != hipSuccess) break;
+      if ((e = hipEventCreateWithFlags(&p.evj, hipEventDisableTiming)) != hipSuccess) break;
+      if ((e = hipEventCreateWithFlags(&p.done, hipEventDisableTiming)) != hipSuccess) break;
+    }
+    if (e != hipSuccess) break;
     for (auto& ev : s->ev)
       if ((e = hipEventCreate(&ev)) != hipSuccess) break;
-    if (e != hipSuccess) break;
-    if ((e = hipStreamCreateWithFlags(&s->s2, hipStreamNonBlocking)) != hipSuccess) break;
-    if ((e = hipEventCreateWithFlags(&s->evf, hipEventDisableTiming)) != hipSuccess) break;
-    if ((e = hipEventCreateWithFlags(&s->evj, hipEventDisableTiming)) != hipSuccess) break;
   } while (0);
   if (e != hipSuccess) {
     ppm_session_destroy(s);
@@ -1244,58 +1281,81 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
   return PPM_OK;
 }
 
-// Enqueue the kernels of one pass.  The eye branch (eye rays, query order)
-// depends only on the pass state, so it runs on s2 beside the photon branch
-// (trace, grid build) and joins before the gather; ev != nullptr (stage
-// timing) runs every stage in order on one stream and records its events.
-static hipError_t enqueue_pass(ppm_session* s, hipStream_t st, hipEvent_t* ev) {
+// Enqueue the kernels of one pass of pipeline p on stream st.  The eye
+// branch (eye rays, query order) depends only on the pass state, so it runs
+// on p.s2 beside the photon branch (trace, grid build) and joins before the
+// gather; ev != nullptr (stage timing) runs every stage in order on st and
+// records its events.
+static hipError_t enqueue_pass(ppm_session* s, uint32_t j, hipStream_t st, hipEvent_t* ev) {
   const ppm_pass_cfg& cfg = s->cfg;
   const ppm_scene* sc = s->scene;
+  Pipe& p = s->pipes[j];
   hipError_t e;
   const float4 bmin = make_float4((float)sc->bmin[0], (float)sc->bmin[1], (float)sc->bmin[2], 0.f);
   const float4 bmax = make_float4((float)sc->bmax[0], (float)sc->bmax[1], (float)sc->bmax[2], 0.f);
   const int n_cu = s->ctx->n_cu;
+  const uint32_t K = (uint32_t)s->pipes.size();
   if (ev && (e = hipEventRecord(ev[0], st)) != hipSuccess) return e;
-  if ((e = launch_pass_begin(st, s->st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
+  if ((e = launch_pass_begin(st, p.st, cfg.first_pass + j, cfg.radius0 * cfg.radius0, cfg.alpha, s->power, K)) !=
+      hipSuccess)
     return e;
   if (!ev) {
-    if ((e = hipEventRecord(s->evf, st)) != hipSuccess) return e;
-    if ((e = hipStreamWaitEvent(s->s2, s->evf, 0)) != hipSuccess) return e;
-    if ((e = launch_eye(s->s2, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
+    if ((e = hipEventRecord(p.evf, st)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(p.s2, p.evf, 0)) != hipSuccess) return e;
+    if ((e = launch_eye(p.s2, s->sbuf, p.st, p.eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
       return e;
-    if ((e = launch_query_prep(s->s2, s->sbuf, s->map, s->st, s->eb)) != hipSuccess) return e;
-    if ((e = hipEventRecord(s->evj, s->s2)) != hipSuccess) return e;
+    if ((e = launch_query_prep(p.s2, s->sbuf, p.map, p.st, p.eb)) != hipSuccess) return e;
+    if ((e = hipEventRecord(p.evj, p.s2)) != hipSuccess) return e;
   }
-  if ((e = launch_trace_photons(st, s->sbuf, s->st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
+  if ((e = launch_trace_photons(st, s->sbuf, p.st, p.pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
     return e;
   if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
-  if ((e = launch_grid_build(st, s->map, s->st, s->pb, bmin, bmax)) != hipSuccess) return e;
+  if ((e = launch_grid_build(st, p.map, p.st, p.pb, bmin, bmax)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
   if (ev) {
-    if ((e = launch_eye(st, s->sbuf, s->st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
+    if ((e = launch_eye(st, s->sbuf, p.st, p.eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
       return e;
     if ((e = hipEventRecord(ev[3], st)) != hipSuccess) return e;
-    if ((e = launch_query_prep(st, s->sbuf, s->map, s->st, s->eb)) != hipSuccess) return e;
-  } else if ((e = hipStreamWaitEvent(st, s->evj, 0)) != hipSuccess) {
+    if ((e = launch_query_prep(st, s->sbuf, p.map, p.st, p.eb)) != hipSuccess) return e;
+  } else if ((e = hipStreamWaitEvent(st, p.evj, 0)) != hipSuccess) {
     return e;
   }
-  if ((e = launch_gather(st, s->sbuf, s->map, s->st, s->eb, cfg.pfilter, cfg.use_classic, n_cu, s->sum, s->sumsq,
+  if ((e = launch_gather(st, s->sbuf, p.map, p.st, p.eb, cfg.pfilter, cfg.use_classic, n_cu, p.sum, p.sumsq,
                          ev ? ev + 4 : nullptr, true)) != hipSuccess)
     return e;
-  if ((e = launch_pass_end(st, s->st)) != hipSuccess) return e;
+  if ((e = launch_pass_end(st, p.st)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[6], st)) != hipSuccess) return e;
   return hipSuccess;
+}
+
+// The pass graph of pipeline j, captured once.
+static hipError_t pipe_graph(ppm_session* s, uint32_t j) {
+  Pipe& p = s->pipes[j];
+  if (p.exec) return hipSuccess;
+  hipStream_t cap;
+  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  if (e != hipSuccess) return e;
+  e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const hipError_t e2 = enqueue_pass(s, j, cap, nullptr);
+    e = hipStreamEndCapture(cap, &p.graph);
+    if (e2 != hipSuccess) e = e2;
+  }
+  if (e == hipSuccess) e = hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0);
+  (void)hipStreamDestroy(cap);
+  return e;
 }
 
 ppm_status ppm_session_run(ppm_session* s, uint32_t n_pass) {
   if (!s) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;
   hipStream_t st = c->stream;
+  const uint32_t K = (uint32_t)s->pipes.size();
   const bool timing = (s->cfg.flags & PPM_SESSION_STAGE_TIMING) != 0;
   const bool graph = !timing && (s->cfg.flags & PPM_SESSION_NO_GRAPH) == 0;
   if (timing) {
     for (uint32_t i = 0; i < n_pass; ++i) {
-      HIP_TRY(c, enqueue_pass(s, st, s->ev));
+      HIP_TRY(c, enqueue_pass(s, (s->passes_done + i) % K, st, s->ev));
       HIP_TRY(c, hipEventSynchronize(s->ev[6]));
       float ms[6];
       for (int k = 0; k < 6; ++k) HIP_TRY(c, hipEventElapsedTime(&ms[k], s->ev[k], s->ev[k + 1]));
@@ -1307,25 +1367,26 @@ ppm_status ppm_session_run(ppm_session* s, uint32_t n_pass) {
       s->t.ms_total += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
     }
   } else {
-    if (graph && !s->exec) {
-      hipStream_t cap;
-      HIP_TRY(c, hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        hipError_t e2 = enqueue_pass(s, cap, nullptr);
-        e = hipStreamEndCapture(cap, &s->graph);
-        if (e2 != hipSuccess) e = e2;
+    if (graph)
+      for (uint32_t j = 0; j < K; ++j) {
+        const hipError_t e = pipe_graph(s, j);
+        if (e != hipSuccess) return fail(c, PPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
       }
-      if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
-      (void)hipStreamDestroy(cap);
-      if (e != hipSuccess) return fail(c, PPM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
-    }
+    // every pipeline starts after the context stream's earlier work and the
+    // context stream waits for all of them: the caller sees one stream
     HIP_TRY(c, hipEventRecord(s->ev[8], st));
+    for (Pipe& p : s->pipes) HIP_TRY(c, hipStreamWaitEvent(p.stream, s->ev[8], 0));
     for (uint32_t i = 0; i < n_pass; ++i) {
+      const uint32_t j = (s->passes_done + i) % K;
+      Pipe& p = s->pipes[j];
       if (graph)
-        HIP_TRY(c, hipGraphLaunch(s->exec, st));
+        HIP_TRY(c, hipGraphLaunch(p.exec, p.stream));
       else
-        HIP_TRY(c, enqueue_pass(s, st, nullptr));
+        HIP_TRY(c, enqueue_pass(s, j, p.stream, nullptr));
+    }
+    for (Pipe& p : s->pipes) {
+      HIP_TRY(c, hipEventRecord(p.done, p.stream));
+      HIP_TRY(c, hipStreamWaitEvent(st, p.done, 0));
     }
     HIP_TRY(c, hipEventRecord(s->ev[9], st));
     HIP_TRY(c, hipEventSynchronize(s->ev[9]));
@@ -1338,15 +1399,46 @@ ppm_status ppm_session_run(ppm_session* s, uint32_t n_pass) {
   return PPM_OK;
 }
 
+// The pass states of all pipelines (after the context stream's work).
+static hipError_t pipe_states(ppm_session* s, std::vector<PassState>& h) {
+  h.resize(s->pipes.size());
+  hipError_t e = hipStreamSynchronize(s->ctx->stream);
+  for (size_t j = 0; j < s->pipes.size() && e == hipSuccess; ++j)
+    e = hipMemcpy(&h[j], s->pipes[j].st, sizeof(PassState), hipMemcpyDeviceToHost);
+  return e;
+}
+
+// Sum and sum of squares over all passes: the pipelines' FP64 sums added
+// in pipeline order.
+static hipError_t merged_sums(ppm_session* s, std::vector<double>& sum, std::vector<double>* sq) {
+  const uint64_t n3 = 3ull * s->pipes[0].eb.n_pix;
+  sum.assign(n3, 0.0);
+  if (sq) sq->assign(n3, 0.0);
+  std::vector<double> a(n3);
+  for (Pipe& p : s->pipes) {
+    hipError_t e = hipMemcpy(a.data(), p.sum, sizeof(double) * n3, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    for (uint64_t i = 0; i < n3; ++i) sum[i] += a[i];
+    if (sq) {
+      if ((e = hipMemcpy(a.data(), p.sumsq, sizeof(double) * n3, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+      for (uint64_t i = 0; i < n3; ++i) (*sq)[i] += a[i];
+    }
+  }
+  return hipSuccess;
+}
+
 ppm_status ppm_session_timings(ppm_session* s, ppm_timings* t) {
   if (!s || !t) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;
-  PassState h;
-  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::vector<PassState> h;
+  HIP_TRY(c, pipe_states(s, h));
   *t = s->t;
-  t->photons_stored = h.total_stored;
-  t->gather_queries = h.total_queries + (uint64_t)s->passes_done * 0;
+  t->photons_stored = 0;
+  t->gather_queries = 0;
+  for (const PassState& x : h) {
+    t->photons_stored += x.total_stored;
+    t->gather_queries += x.total_queries;
+  }
   t->photons_emitted = (uint64_t)s->sbuf.n_emit_total * s->passes_done;
   t->passes = s->passes_done;
   return PPM_OK;
@@ -1365,16 +1457,15 @@ ppm_status ppm_session_reset(ppm_session* s) {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemcpy(s->lights, L.data(), sizeof(DevLight) * L.size(), hipMemcpyHostToDevice));
   s->power = power;
-  const uint64_t n3 = 3ull * s->eb.n_pix;
-  HIP_TRY(c, hipMemset(s->sum, 0, sizeof(double) * n3));
-  HIP_TRY(c, hipMemset(s->sumsq, 0, sizeof(double) * n3));
   const PassState h = host_state();
-  HIP_TRY(c, hipMemcpy(s->st, &h, sizeof(h), hipMemcpyHostToDevice));
-  // the captured pass has the photon power as an argument: capture again
-  if (s->exec) (void)hipGraphExecDestroy(s->exec);
-  if (s->graph) (void)hipGraphDestroy(s->graph);
-  s->exec = nullptr;
-  s->graph = nullptr;
+  for (Pipe& p : s->pipes) {
+    const uint64_t n3 = 3ull * p.eb.n_pix;
+    HIP_TRY(c, hipMemset(p.sum, 0, sizeof(double) * n3));
+    HIP_TRY(c, hipMemset(p.sumsq, 0, sizeof(double) * n3));
+    HIP_TRY(c, hipMemcpy(p.st, &h, sizeof(h), hipMemcpyHostToDevice));
+    // the captured pass has the photon power as an argument: capture again
+    pipe_drop_graph(p);
+  }
   s->passes_done = 0;
   s->t = ppm_timings{};
   return PPM_OK;
@@ -1387,7 +1478,8 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
   if (K == 0) return fail(c, PPM_E_INVALID, "backward before any pass");
   const ppm_pass_cfg& cfg = s->cfg;
   const ppm_scene* sc = s->scene;
-  const uint32_t n_pix = s->eb.n_pix;
+  Pipe& p0 = s->pipes[0];
+  const uint32_t n_pix = p0.eb.n_pix;
   const uint32_t nm = s->sbuf.n_mats;
   // dL/d(pass image) = dL/d(mean image) / K
   std::vector<float4> g(n_pix);
@@ -1404,8 +1496,8 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
   const float4 bmax = make_float4((float)sc->bmax[0], (float)sc->bmax[1], (float)sc->bmax[2], 0.f);
   hipStream_t q = c->stream;
   hipError_t e;
-  if ((e = dalloc(&gimg, n_pix)) == hipSuccess && (e = dalloc(&adj_sorted, s->map.cap)) == hipSuccess &&
-      (e = dalloc(&adj_slot, s->pb.cap)) == hipSuccess && (e = dalloc(&grad, 3ull * nm)) == hipSuccess &&
+  if ((e = dalloc(&gimg, n_pix)) == hipSuccess && (e = dalloc(&adj_sorted, p0.map.cap)) == hipSuccess &&
+      (e = dalloc(&adj_slot, p0.pb.cap)) == hipSuccess && (e = dalloc(&grad, 3ull * nm)) == hipSuccess &&
       (e = dalloc(&st, 1)) == hipSuccess &&
       (e = hipMemcpy(gimg, g.data(), sizeof(float4) * n_pix, hipMemcpyHostToDevice)) == hipSuccess &&
       (e = hipMemset(grad, 0, sizeof(double) * 3 * nm)) == hipSuccess) {
@@ -1416,15 +1508,15 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
     for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
       if ((e = launch_pass_begin(q, st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
         break;
-      if ((e = launch_trace_photons(q, s->sbuf, st, s->pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
+      if ((e = launch_trace_photons(q, s->sbuf, st, p0.pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
         break;
-      if ((e = launch_grid_build(q, s->map, st, s->pb, bmin, bmax)) != hipSuccess) break;
-      if ((e = launch_eye(q, s->sbuf, st, s->eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) != hipSuccess)
+      if ((e = launch_grid_build(q, p0.map, st, p0.pb, bmin, bmax)) != hipSuccess) break;
+      if ((e = launch_eye(q, s->sbuf, st, p0.eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) != hipSuccess)
         break;
-      if ((e = launch_gather(q, s->sbuf, s->map, st, s->eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
+      if ((e = launch_gather(q, s->sbuf, p0.map, st, p0.eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
                              nullptr)) != hipSuccess)
         break;
-      if ((e = launch_adjoint(q, s->sbuf, s->map, st, s->pb, s->eb, cfg.pfilter, c->n_cu, cfg.seed,
+      if ((e = launch_adjoint(q, s->sbuf, p0.map, st, p0.pb, p0.eb, cfg.pfilter, c->n_cu, cfg.seed,
                               cfg.use_classic, cfg.max_depth, gimg, adj_sorted, adj_slot, grad)) != hipSuccess)
         break;
       e = launch_pass_end(q, st);
@@ -1442,8 +1534,8 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
   for (size_t i = 0; i < hg.size(); ++i) grad_albedo[i] = hg[i];
   if (grad_light) {
     // the image is proportional to the light scale: dL/ds = sum g * mean / s
-    std::vector<double> sum(3ull * n_pix);
-    HIP_TRY(c, hipMemcpy(sum.data(), s->sum, sizeof(double) * sum.size(), hipMemcpyDeviceToHost));
+    std::vector<double> sum;
+    HIP_TRY(c, merged_sums(s, sum, nullptr));
     double acc = 0.0;
     for (size_t i = 0; i < sum.size(); ++i) acc += grad_image[i] * (sum[i] / K);
     *grad_light = sc->light_scale > 0.0 ? acc / sc->light_scale : 0.0;
@@ -1454,26 +1546,28 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
 ppm_status ppm_session_radius2(ppm_session* s, double* r2) {
   if (!s || !r2) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;
-  PassState h;
-  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
-  *r2 = h.r2;
+  std::vector<PassState> h;
+  HIP_TRY(c, pipe_states(s, h));
+  // the state of the pipeline that ran the last pass
+  const uint32_t K = (uint32_t)s->pipes.size();
+  *r2 = h[s->passes_done ? (s->passes_done - 1) % K : 0].r2;
   return PPM_OK;
 }
 
 ppm_status ppm_session_image(ppm_session* s, double* mean_out, double* var_out, uint32_t* passes_done) {
   if (!s) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;
-  PassState h;
-  HIP_TRY(c, hipMemcpyAsync(&h, s->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::vector<PassState> h;
+  HIP_TRY(c, pipe_states(s, h));
   if (passes_done) *passes_done = s->passes_done;
-  if (h.overflow) return fail(c, PPM_E_RANGE, "capacity exceeded during a pass (flags " + std::to_string(h.overflow) + ")");
-  if (h.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
-  const uint64_t n3 = 3ull * s->eb.n_pix;
-  std::vector<double> sum(n3), sq(n3);
-  HIP_TRY(c, hipMemcpy(sum.data(), s->sum, sizeof(double) * n3, hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(sq.data(), s->sumsq, sizeof(double) * n3, hipMemcpyDeviceToHost));
+  for (const PassState& x : h) {
+    if (x.overflow)
+      return fail(c, PPM_E_RANGE, "capacity exceeded during a pass (flags " + std::to_string(x.overflow) + ")");
+    if (x.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
+  }
+  std::vector<double> sum, sq;
+  HIP_TRY(c, merged_sums(s, sum, &sq));
+  const uint64_t n3 = sum.size();
   const double n = (double)s->passes_done;
   for (uint64_t i = 0; i < n3; ++i) {
     if (mean_out) mean_out[i] = n > 0 ? sum[i] / n : 0.0;
@@ -1527,12 +1621,12 @@ hipError_t batch_layout(ppm_batch* b) {
     BatchItem& it = items[j];
     std::memset(&it, 0, sizeof(it));
     it.sc = s->sbuf;
-    it.st = s->st;
-    it.pb = s->pb;
-    it.m = s->map;
-    it.eb = s->eb;
-    it.sum = s->sum;
-    it.sumsq = s->sumsq;
+    it.st = s->pipes[0].st;
+    it.pb = s->pipes[0].pb;
+    it.m = s->pipes[0].map;
+    it.eb = s->pipes[0].eb;
+    it.sum = s->pipes[0].sum;
+    it.sumsq = s->pipes[0].sumsq;
     it.seed = s->cfg.seed;
     it.r2_first = s->cfg.radius0 * s->cfg.radius0;
     it.alpha = s->cfg.alpha;
@@ -1542,29 +1636,29 @@ hipError_t batch_layout(ppm_batch* b) {
     it.uc = s->cfg.use_classic;
     it.maxd = s->cfg.max_depth;
     it.pfilter = s->cfg.pfilter;
-    it.trace_per_wave = (uint32_t)(((uint64_t)s->sbuf.n_emit_total + s->pb.n_regions - 1) / s->pb.n_regions);
-    it.eye_waves = std::max(1u, std::min(std::min(s->eb.n_tiles, n_cu * ew), s->eb.estack_lanes / 64u));
+    it.trace_per_wave = (uint32_t)(((uint64_t)s->sbuf.n_emit_total + it.pb.n_regions - 1) / it.pb.n_regions);
+    it.eye_waves = std::max(1u, std::min(std::min(it.eb.n_tiles, n_cu * ew), it.eb.estack_lanes / 64u));
     // sort blocks from the expected key counts (about 2 stored photons per
     // emitted one, 1.5 queries per pixel), 8192 keys per block: a stable
     // radix sort gives the same order for any block count, and a batch's
     // sorts together need no more blocks than fill the GPU
     it.psort_blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(s->map.sort_blocks, (2ull * s->sbuf.n_emit_total + 8191) / 8192));
+        1, std::min<uint64_t>(it.m.sort_blocks, (2ull * s->sbuf.n_emit_total + 8191) / 8192));
     it.qsort_blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(s->eb.q_sort_blocks, (3ull * s->eb.q_primary / 2 + 8191) / 8192));
+        1, std::min<uint64_t>(it.eb.q_sort_blocks, (3ull * it.eb.q_primary / 2 + 8191) / 8192));
     uint32_t cnt[BK_KINDS];
-    cnt[BK_TRACE] = (s->pb.n_regions + 3) / 4;
-    cnt[BK_MORTON] = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1, (s->map.cap + 255) / 256));
+    cnt[BK_TRACE] = (it.pb.n_regions + 3) / 4;
+    cnt[BK_MORTON] = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1, (it.m.cap + 255) / 256));
     cnt[BK_PSORT] = it.psort_blocks;
     cnt[BK_SCAN] = 256;
-    cnt[BK_LEAVES] = grid_photon_blocks(s->map.cap);
-    cnt[BK_CLEAR] = grid_clear_blocks(s->map.cells_max);
+    cnt[BK_LEAVES] = grid_photon_blocks(it.m.cap);
+    cnt[BK_CLEAR] = grid_clear_blocks(it.m.cells_max);
     cnt[BK_EYE] = (it.eye_waves + 3) / 4;
-    cnt[BK_PIX] = (s->eb.q_primary + 255) / 256;
-    cnt[BK_QKEYS] = (s->eb.q_primary + s->eb.ovf_cap + 255) / 256;
+    cnt[BK_PIX] = (it.eb.q_primary + 255) / 256;
+    cnt[BK_QKEYS] = (it.eb.q_primary + it.eb.ovf_cap + 255) / 256;
     cnt[BK_QSORT] = it.qsort_blocks;
-    cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, s->eb.q_primary / 128u));
-    cnt[BK_OVF] = (s->eb.ovf_cap + 255) / 256;
+    cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, it.eb.q_primary / 128u));
+    cnt[BK_OVF] = (it.eb.ovf_cap + 255) / 256;
     for (int k = 0; k < BK_KINDS; ++k) b->htab[(size_t)k * (n + 1) + j + 1] = b->htab[(size_t)k * (n + 1) + j] + cnt[k];
   }
   hipError_t e;
@@ -1604,11 +1698,12 @@ hipError_t batch_round(ppm_batch* b, uint32_t act) {
 hipError_t batch_reset(ppm_batch* b) {
   const PassState h = host_state();
   for (ppm_session* s : b->sess) {
-    const uint64_t n3 = 3ull * s->eb.n_pix;
+    const uint64_t n3 = 3ull * s->pipes[0].eb.n_pix;
     hipError_t e;
-    if ((e = hipMemsetAsync(s->sum, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(s->sumsq, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(s->st, &h, sizeof(h), hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(s->pipes[0].sum, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(s->pipes[0].sumsq, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(s->pipes[0].st, &h, sizeof(h), hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess)
+      return e;
     s->passes_done = 0;
   }
   b->rounds_done = 0;
@@ -1646,6 +1741,7 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
     }
     ppm_pass_cfg cfg = cfgs[i];
     cfg.flags = 0;
+    cfg.pipelines = 1;  // a batch overlaps its scenes instead
     ppm_session* s = nullptr;
     if (ppm_status st = ppm_session_create(c, scenes[i], &cfg, &s)) {
       ppm_batch_destroy(b);

@@ -55,7 +55,11 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
 // Pass state (util/iterator.rb:33-38): r_{p}^2 = r_{p-1}^2 (p - 1 + alpha) / p,
 // in double on the device so that a captured pass loop needs no host input.
 
-__device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power, Blk blk) {
+// A state that runs every stride-th pass (one of several pass pipelines)
+// steps through the passes in between, so r2 of every pass is the same FP64
+// product as in a single pipeline.
+__device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_pass, double r2_first, double alpha,
+                                                  double power, uint32_t stride, Blk blk) {
   if (threadIdx.x != 0 || blk.b != 0) return;
   PassState s = *st;
   if (s.pass == 0) {
@@ -67,9 +71,11 @@ __device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_
     for (uint32_t k = 1; k < first_pass; ++k) r2 *= ((double)k + alpha) / ((double)k + 1.0);
     s.r2 = r2;
   } else {
-    s.pass += 1;
-    double k = (double)(s.pass - 1);
-    s.r2 *= (k + s.alpha) / (k + 1.0);
+    for (uint32_t j = 0; j < stride; ++j) {
+      s.pass += 1;
+      double k = (double)(s.pass - 1);
+      s.r2 *= (k + s.alpha) / (k + 1.0);
+    }
   }
   s.r2f = (float)s.r2;
   s.scale = (float)(s.power / (3.14159265358979323846 * s.r2));
@@ -84,8 +90,9 @@ __device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_
   *st = s;
 }
 
-__global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power) {
-  k_pass_begin_body(st, first_pass, r2_first, alpha, power, Blk{blockIdx.x, gridDim.x});
+__global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power,
+                             uint32_t stride) {
+  k_pass_begin_body(st, first_pass, r2_first, alpha, power, stride, Blk{blockIdx.x, gridDim.x});
 }
 
 __device__ __forceinline__ void k_pass_end_body(PassState* st, Blk blk) {
@@ -99,8 +106,8 @@ __global__ void k_pass_end(PassState* st) {
 }
 
 hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
-                             double power) {
-  hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, s, st, first_pass, r2_first, alpha, power);
+                             double power, uint32_t stride) {
+  hipLaunchKernelGGL(k_pass_begin, dim3(1), dim3(64), 0, s, st, first_pass, r2_first, alpha, power, stride);
   return hipGetLastError();
 }
 hipError_t launch_pass_end(hipStream_t s, PassState* st) {
@@ -2114,7 +2121,7 @@ __device__ __forceinline__ BatchItem load_item(const BatchItem* items, uint32_t 
 __global__ void kb_pass_begin(const BatchItem* items, uint32_t n) {
   if (blockIdx.x >= n) return;
   const BatchItem it = load_item(items, blockIdx.x);
-  k_pass_begin_body(it.st, it.first_pass, it.r2_first, it.alpha, it.power, Blk{0u, 1u});
+  k_pass_begin_body(it.st, it.first_pass, it.r2_first, it.alpha, it.power, 1u, Blk{0u, 1u});
 }
 
 __global__ void kb_pass_end(const BatchItem* items, uint32_t n) {

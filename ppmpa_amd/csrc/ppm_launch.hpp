@@ -99,8 +99,10 @@ struct EyeBuf {
 // Eye waves per CU at most (the launch configurations a batch may tune to).
 constexpr uint32_t EYE_WAVES_PER_CU_MAX = 32;
 
+// Next pass of the state (the first one: first_pass); a state that runs
+// every stride-th pass advances by stride.
 hipError_t launch_pass_begin(hipStream_t s, PassState* st, uint32_t first_pass, double r2_first, double alpha,
-                             double power);
+                             double power, uint32_t stride = 1);
 hipError_t launch_pass_end(hipStream_t s, PassState* st);
 
 // Number of tracing waves (= photon regions) used for n photons.

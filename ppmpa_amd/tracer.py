@@ -274,9 +274,11 @@ def trace_eye(ctx, dscene, pmap, seed, pass_, r2, use_classic=True, max_depth=10
 
 
 def pass_cfg(seed, n_photon, radius0, alpha, max_depth, use_classic=True, pfilter=A.FILTER_NONE, first_pass=1,
-             flags=0):
+             flags=0, pipelines=0):
+    """ppm_pass_cfg; pipelines = passes in flight at once (0: automatic)."""
     c = A.PassCfg()
     c.seed, c.n_photon, c.first_pass = int(seed), int(n_photon), int(first_pass)
+    c.pipelines = int(pipelines)
     c.radius0, c.alpha = float(radius0), float(alpha)
     c.use_classic, c.max_depth, c.pfilter, c.flags = int(use_classic), int(max_depth), int(pfilter), int(flags)
     return c
