@@ -1508,19 +1508,29 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
     for (uint32_t k = 0; k < K && e == hipSuccess; ++k) {
       if ((e = launch_pass_begin(q, st, cfg.first_pass, cfg.radius0 * cfg.radius0, cfg.alpha, s->power)) != hipSuccess)
         break;
+      // the eye branch beside the photon branch, as in a forward pass
+      if ((e = hipEventRecord(p0.evf, q)) != hipSuccess || (e = hipStreamWaitEvent(p0.s2, p0.evf, 0)) != hipSuccess)
+        break;
+      if ((e = launch_eye(p0.s2, s->sbuf, st, p0.eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) !=
+          hipSuccess)
+        break;
+      if ((e = launch_query_prep(p0.s2, s->sbuf, p0.map, st, p0.eb)) != hipSuccess) break;
+      if ((e = hipEventRecord(p0.evj, p0.s2)) != hipSuccess) break;
       if ((e = launch_trace_photons(q, s->sbuf, st, p0.pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
         break;
       if ((e = launch_grid_build(q, p0.map, st, p0.pb, bmin, bmax)) != hipSuccess) break;
-      if ((e = launch_eye(q, s->sbuf, st, p0.eb, cfg.seed, cfg.use_classic, cfg.max_depth, c->n_cu)) != hipSuccess)
-        break;
+      if ((e = hipStreamWaitEvent(q, p0.evj, 0)) != hipSuccess) break;
       if ((e = launch_gather(q, s->sbuf, p0.map, st, p0.eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
-                             nullptr)) != hipSuccess)
+                             nullptr, nullptr, true)) != hipSuccess)
         break;
       if ((e = launch_adjoint(q, s->sbuf, p0.map, st, p0.pb, p0.eb, cfg.pfilter, c->n_cu, cfg.seed,
                               cfg.use_classic, cfg.max_depth, gimg, adj_sorted, adj_slot, grad)) != hipSuccess)
         break;
       e = launch_pass_end(q, st);
     }
+    // an error may leave the eye branch running: drain it before the frees
+    const hipError_t e2 = hipStreamSynchronize(p0.s2);
+    if (e == hipSuccess) e = e2;
     if (e == hipSuccess) e = hipStreamSynchronize(q);
   }
   std::vector<double> hg(3ull * nm, 0.0);
@@ -1678,17 +1688,25 @@ hipError_t batch_round(ppm_batch* b, uint32_t act) {
   if (!b->exec[act - 1]) {
     uint32_t blocks[BK_KINDS];
     for (int k = 0; k < BK_KINDS; ++k) blocks[k] = b->htab[(size_t)k * (n + 1) + act];
-    hipStream_t cap;
+    // the capture stream, the eye branch's stream and the fork/join events
+    // live only while the graph is captured
+    hipStream_t cap = nullptr, cap2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
-    if (e != hipSuccess) return e;
-    e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&cap2, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      const hipError_t e2 = launch_batch_pass(cap, b->items, b->tab, blocks, act, n, b->bvh);
+      const hipError_t e2 = launch_batch_pass(cap, cap2, fork, join, b->items, b->tab, blocks, act, n, b->bvh);
       e = hipStreamEndCapture(cap, &b->graph[act - 1]);
       if (e2 != hipSuccess) e = e2;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[act - 1], b->graph[act - 1], nullptr, nullptr, 0);
-    (void)hipStreamDestroy(cap);
+    if (join) (void)hipEventDestroy(join);
+    if (fork) (void)hipEventDestroy(fork);
+    if (cap2) (void)hipStreamDestroy(cap2);
+    if (cap) (void)hipStreamDestroy(cap);
     if (e != hipSuccess) return e;
   }
   return hipGraphLaunch(b->exec[act - 1], st);

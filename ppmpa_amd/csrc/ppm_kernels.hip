@@ -2268,11 +2268,32 @@ __global__ __launch_bounds__(256) void kb_accumulate(const BatchItem* items, con
 
 #undef BATCH_PROLOGUE
 
-hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32_t* tab, const uint32_t* blocks,
-                             uint32_t n, uint32_t n_all, bool bvh) {
-  // blocks[k] = total blocks of kind k over the first n scenes
+hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join,
+                             const BatchItem* items, const uint32_t* tab, const uint32_t* blocks, uint32_t n,
+                             uint32_t n_all, bool bvh) {
+  // blocks[k] = total blocks of kind k over the first n scenes.  The eye
+  // branch (tile order, eye trace, query sort) runs on s2 beside the photon
+  // branch (trace, grid build): the two touch disjoint buffers and counters.
   const dim3 T256(256), T1024(1024), T64(64), TRS(RS_THREADS);
+  hipError_t e;
   hipLaunchKernelGGL(kb_pass_begin, dim3(n), T64, 0, s, items, n);
+  if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess) return e;
+  hipLaunchKernelGGL(kb_tile_order, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
+  if (bvh)
+    hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
+  else
+    hipLaunchKernelGGL(kb_eye<false>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
+  hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s2, items, n);
+  hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s2, items, tab, n, n_all);
+  for (uint32_t p = 0; p < 4; ++p) {
+    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);
+    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s2, items, tab, n, n_all, 1);
+    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipEventRecord(join, s2)) != hipSuccess) return e;
   if (bvh)
     hipLaunchKernelGGL(kb_trace<true>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
   else
@@ -2288,19 +2309,8 @@ hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32
   hipLaunchKernelGGL(kb_grid_clear, dim3(blocks[BK_CLEAR]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_grid_cells, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_grid_permute, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
-  hipLaunchKernelGGL(kb_tile_order, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
-  if (bvh)
-    hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
-  else
-    hipLaunchKernelGGL(kb_eye<false>, dim3(blocks[BK_EYE]), T256, 0, s, items, tab, n, n_all);
-  hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
-  hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s, items, n);
-  hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s, items, tab, n, n_all);
-  for (uint32_t p = 0; p < 4; ++p) {
-    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_QSORT]), TRS, 0, s, items, tab, n, n_all, 1, p);
-    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s, items, tab, n, n_all, 1);
-    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_QSORT]), TRS, 0, s, items, tab, n, n_all, 1, p);
-  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
   if (bvh)
     hipLaunchKernelGGL(kb_shade<true>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
   else

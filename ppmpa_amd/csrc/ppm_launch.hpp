@@ -213,8 +213,9 @@ hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pa
 
 // One pass of the first n scenes of a batch (scenes sorted by pass count,
 // so the scenes still running are a prefix); blocks[k] = tab[k*(n_all+1)+n].
-hipError_t launch_batch_pass(hipStream_t s, const BatchItem* items, const uint32_t* tab, const uint32_t* blocks,
-                             uint32_t n, uint32_t n_all, bool bvh);
+hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join,
+                             const BatchItem* items, const uint32_t* tab, const uint32_t* blocks, uint32_t n,
+                             uint32_t n_all, bool bvh);
 
 // Test hook: nearest hit of n rays (unit directions).
 hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
