@@ -361,7 +361,10 @@ ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
  * pass is one launch over all scenes still running (each scene's blocks run
  * the single-scene kernel body on the scene's own buffers), captured as one
  * hipGraph per number of running scenes.  Each scene's image equals what its
- * own session would produce. */
+ * own session would produce.  Rounds (one pass of every running scene) are
+ * spread over K pipelines as a session's passes are: round r runs on
+ * pipeline r mod K, with its own buffers in every scene, so K rounds are in
+ * flight at once.  K = the smallest cfgs[i].pipelines, 2 where that is 0. */
 typedef struct ppm_batch ppm_batch;
 
 /* Launch sizes of a batch (0 = the defaults 16 and 8). */

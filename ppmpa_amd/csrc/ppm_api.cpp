@@ -1208,7 +1208,10 @@ static uint32_t session_pipelines(const ppm_pass_cfg& cfg, uint64_t n_emit, uint
   return (n_emit >= (1u << 18) || n_pix >= (1u << 18)) ? 2u : 1u;
 }
 
-ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, ppm_session** out) {
+// own_streams = false: a batch's session, whose passes the batch launches on
+// its own streams; the pipelines get no streams or events of their own.
+static ppm_status session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, ppm_session** out,
+                                 bool own_streams) {
   if (!c || !sc || !cfg || !out) return fail(c, PPM_E_INVALID, "null argument");
   *out = nullptr;
   if (ppm_status s = check_cfg_depth(c, cfg->max_depth)) return s;
@@ -1260,10 +1263,11 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
       // pipeline 0 runs on the context's stream: K pipelines then use 2K
       // streams in all, which HIP spreads over that many hardware queues
       // (GPU_MAX_HW_QUEUES, 4 by default) when K <= 2
-      if (&p == &s->pipes[0])
+      if (&p == &s->pipes[0] || !own_streams)
         p.stream = c->stream;
       else if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
         break;
+      if (!own_streams) continue;
       if ((e = hipStreamCreateWithFlags(&p.s2, hipStreamNonBlocking)) != hipSuccess) break;
       if ((e = hipEventCreateWithFlags(&p.evf, hipEventDisableTiming)) != hipSuccess) break;
       if ((e = hipEventCreateWithFlags(&p.evj, hipEventDisableTiming)) != hipSuccess) break;
@@ -1279,6 +1283,10 @@ ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cf
   }
   *out = s;
   return PPM_OK;
+}
+
+ppm_status ppm_session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass_cfg* cfg, ppm_session** out) {
+  return session_create(c, sc, cfg, out, true);
 }
 
 // Enqueue the kernels of one pass of pipeline p on stream st.  The eye
@@ -1596,13 +1604,16 @@ struct ppm_batch {
   std::vector<ppm_session*> sess;  // caller's order
   std::vector<uint32_t> n_pass;    // caller's order
   std::vector<uint32_t> order;     // caller indices by pass count, descending
-  BatchItem* items = nullptr;      // device, in `order`
+  uint32_t K = 1;                  // pipelines: round r runs on pipeline r mod K
+  BatchItem* items = nullptr;      // device, K x n (pipeline-major), each in `order`
   uint32_t* tab = nullptr;         // device, BK_KINDS x (n + 1)
   std::vector<uint32_t> htab;
   bool bvh = false;
   ppm_launch_cfg lc{};
-  std::vector<hipGraphExec_t> exec;  // per number of running scenes (index n - 1)
+  std::vector<hipGraphExec_t> exec;  // per pipeline j and number of running scenes a: j * n + a - 1
   std::vector<hipGraph_t> graph;
+  std::vector<hipStream_t> ps;       // pipeline j's stream (0: the context's)
+  std::vector<hipEvent_t> pdone;     // pipeline j's last round
   hipEvent_t ev[2] = {};
   double ms_total = 0.0;
   uint32_t rounds_done = 0;
@@ -1624,7 +1635,7 @@ hipError_t batch_layout(ppm_batch* b) {
   const uint32_t n_cu = (uint32_t)b->ctx->n_cu;
   const uint32_t ew = b->lc.eye_waves_per_cu ? b->lc.eye_waves_per_cu : 16u;
   const uint32_t sb = b->lc.shade_blocks_per_cu ? b->lc.shade_blocks_per_cu : 8u;
-  std::vector<BatchItem> items(n);
+  std::vector<BatchItem> items((size_t)b->K * n);
   b->htab.assign((size_t)BK_KINDS * (n + 1), 0u);
   for (uint32_t j = 0; j < n; ++j) {
     const ppm_session* s = b->sess[b->order[j]];
@@ -1643,6 +1654,7 @@ hipError_t batch_layout(ppm_batch* b) {
     it.power = s->power;
     it.mscale = morton_scale(s->sbuf.bmin, s->sbuf.bmax, 1023u);
     it.first_pass = s->cfg.first_pass;
+    it.pass_stride = b->K;
     it.uc = s->cfg.use_classic;
     it.maxd = s->cfg.max_depth;
     it.pfilter = s->cfg.pfilter;
@@ -1670,22 +1682,38 @@ hipError_t batch_layout(ppm_batch* b) {
     cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, it.eb.q_primary / 128u));
     cnt[BK_OVF] = (it.eb.ovf_cap + 255) / 256;
     for (int k = 0; k < BK_KINDS; ++k) b->htab[(size_t)k * (n + 1) + j + 1] = b->htab[(size_t)k * (n + 1) + j] + cnt[k];
+    // the other pipelines: the same sizes on their own buffers, passes
+    // first_pass + q, + q + K, ...
+    for (uint32_t q = 1; q < b->K; ++q) {
+      BatchItem& iq = items[(size_t)q * n + j];
+      const Pipe& p = s->pipes[q];
+      iq = it;
+      iq.st = p.st;
+      iq.pb = p.pb;
+      iq.m = p.map;
+      iq.eb = p.eb;
+      iq.sum = p.sum;
+      iq.sumsq = p.sumsq;
+      iq.first_pass = s->cfg.first_pass + q;
+    }
   }
   hipError_t e;
-  if ((e = hipMemcpy(b->items, items.data(), n * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess) return e;
+  if ((e = hipMemcpy(b->items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess)
+    return e;
   if ((e = hipMemcpy(b->tab, b->htab.data(), b->htab.size() * sizeof(uint32_t), hipMemcpyHostToDevice)) != hipSuccess)
     return e;
   batch_drop_graphs(b);
-  b->exec.assign(n, nullptr);
-  b->graph.assign(n, nullptr);
+  b->exec.assign((size_t)b->K * n, nullptr);
+  b->graph.assign((size_t)b->K * n, nullptr);
   return hipSuccess;
 }
 
-// One pass of the first `act` scenes (in `order`), through a captured graph.
-hipError_t batch_round(ppm_batch* b, uint32_t act) {
+// One pass of the first `act` scenes (in `order`) on pipeline q, through a
+// captured graph.
+hipError_t batch_round(ppm_batch* b, uint32_t act, uint32_t q) {
   const uint32_t n = (uint32_t)b->order.size();
-  hipStream_t st = b->ctx->stream;
-  if (!b->exec[act - 1]) {
+  const size_t gi = (size_t)q * n + act - 1;
+  if (!b->exec[gi]) {
     uint32_t blocks[BK_KINDS];
     for (int k = 0; k < BK_KINDS; ++k) blocks[k] = b->htab[(size_t)k * (n + 1) + act];
     // the capture stream, the eye branch's stream and the fork/join events
@@ -1698,30 +1726,32 @@ hipError_t batch_round(ppm_batch* b, uint32_t act) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-      const hipError_t e2 = launch_batch_pass(cap, cap2, fork, join, b->items, b->tab, blocks, act, n, b->bvh);
-      e = hipStreamEndCapture(cap, &b->graph[act - 1]);
+      const hipError_t e2 =
+          launch_batch_pass(cap, cap2, fork, join, b->items + (size_t)q * n, b->tab, blocks, act, n, b->bvh);
+      e = hipStreamEndCapture(cap, &b->graph[gi]);
       if (e2 != hipSuccess) e = e2;
     }
-    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[act - 1], b->graph[act - 1], nullptr, nullptr, 0);
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[gi], b->graph[gi], nullptr, nullptr, 0);
     if (join) (void)hipEventDestroy(join);
     if (fork) (void)hipEventDestroy(fork);
     if (cap2) (void)hipStreamDestroy(cap2);
     if (cap) (void)hipStreamDestroy(cap);
     if (e != hipSuccess) return e;
   }
-  return hipGraphLaunch(b->exec[act - 1], st);
+  return hipGraphLaunch(b->exec[gi], b->ps[q]);
 }
 
 // Every scene back to before its first pass.
 hipError_t batch_reset(ppm_batch* b) {
   const PassState h = host_state();
   for (ppm_session* s : b->sess) {
-    const uint64_t n3 = 3ull * s->pipes[0].eb.n_pix;
-    hipError_t e;
-    if ((e = hipMemsetAsync(s->pipes[0].sum, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(s->pipes[0].sumsq, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(s->pipes[0].st, &h, sizeof(h), hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess)
-      return e;
+    for (Pipe& p : s->pipes) {
+      const uint64_t n3 = 3ull * p.eb.n_pix;
+      hipError_t e;
+      if ((e = hipMemsetAsync(p.sum, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+      if ((e = hipMemsetAsync(p.sumsq, 0, sizeof(double) * n3, b->ctx->stream)) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(p.st, &h, sizeof(h), hipMemcpyHostToDevice, b->ctx->stream)) != hipSuccess) return e;
+    }
     s->passes_done = 0;
   }
   b->rounds_done = 0;
@@ -1735,6 +1765,10 @@ extern "C" {
 void ppm_batch_destroy(ppm_batch* b) {
   if (!b) return;
   if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
+  for (size_t q = 1; q < b->ps.size(); ++q)
+    if (b->ps[q]) (void)hipStreamSynchronize(b->ps[q]), (void)hipStreamDestroy(b->ps[q]);
+  for (hipEvent_t e : b->pdone)
+    if (e) (void)hipEventDestroy(e);
   batch_drop_graphs(b);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
@@ -1752,6 +1786,10 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
   ppm_batch* b = new ppm_batch();
   b->ctx = c;
   b->n_pass.assign(n_passes, n_passes + n);
+  // pipelines: the smallest cfgs[i].pipelines, 2 where it is 0 (automatic)
+  b->K = 4;
+  for (uint32_t i = 0; i < n; ++i) b->K = std::min<uint32_t>(b->K, cfgs[i].pipelines ? cfgs[i].pipelines : 2u);
+  if (b->K == 0) b->K = 1;
   for (uint32_t i = 0; i < n; ++i) {
     if (!scenes[i]) {
       ppm_batch_destroy(b);
@@ -1759,9 +1797,9 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
     }
     ppm_pass_cfg cfg = cfgs[i];
     cfg.flags = 0;
-    cfg.pipelines = 1;  // a batch overlaps its scenes instead
+    cfg.pipelines = b->K;
     ppm_session* s = nullptr;
-    if (ppm_status st = ppm_session_create(c, scenes[i], &cfg, &s)) {
+    if (ppm_status st = session_create(c, scenes[i], &cfg, &s, false)) {
       ppm_batch_destroy(b);
       return st;
     }
@@ -1773,7 +1811,16 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
   std::stable_sort(b->order.begin(), b->order.end(),
                    [&](uint32_t a, uint32_t d) { return b->n_pass[a] > b->n_pass[d]; });
   hipError_t e;
-  if ((e = dalloc(&b->items, n)) != hipSuccess || (e = dalloc(&b->tab, (uint64_t)BK_KINDS * (n + 1))) != hipSuccess ||
+  b->ps.assign(b->K, nullptr);
+  b->pdone.assign(b->K, nullptr);
+  b->ps[0] = c->stream;
+  e = hipSuccess;
+  for (uint32_t q = 1; q < b->K && e == hipSuccess; ++q) {
+    e = hipStreamCreateWithFlags(&b->ps[q], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->pdone[q], hipEventDisableTiming);
+  }
+  if (e != hipSuccess || (e = dalloc(&b->items, (uint64_t)b->K * n)) != hipSuccess ||
+      (e = dalloc(&b->tab, (uint64_t)BK_KINDS * (n + 1))) != hipSuccess ||
       (e = hipEventCreate(&b->ev[0])) != hipSuccess || (e = hipEventCreate(&b->ev[1])) != hipSuccess ||
       (e = batch_layout(b)) != hipSuccess) {
     ppm_batch_destroy(b);
@@ -1810,12 +1857,17 @@ ppm_status ppm_batch_run(ppm_batch* b, uint32_t max_rounds) {
   uint32_t end = total;
   if (max_rounds && b->rounds_done + max_rounds < end) end = b->rounds_done + max_rounds;
   HIP_TRY(c, hipEventRecord(b->ev[0], c->stream));
+  for (uint32_t q = 1; q < b->K; ++q) HIP_TRY(c, hipStreamWaitEvent(b->ps[q], b->ev[0], 0));
   for (uint32_t r = b->rounds_done; r < end; ++r) {
     uint32_t act = n;
     while (act > 0 && b->n_pass[b->order[act - 1]] <= r) --act;
     if (act == 0) break;
-    HIP_TRY(c, batch_round(b, act));
+    HIP_TRY(c, batch_round(b, act, r % b->K));
     for (uint32_t j = 0; j < act; ++j) b->sess[b->order[j]]->passes_done += 1;
+  }
+  for (uint32_t q = 1; q < b->K; ++q) {
+    HIP_TRY(c, hipEventRecord(b->pdone[q], b->ps[q]));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, b->pdone[q], 0));
   }
   HIP_TRY(c, hipEventRecord(b->ev[1], c->stream));
   HIP_TRY(c, hipEventSynchronize(b->ev[1]));

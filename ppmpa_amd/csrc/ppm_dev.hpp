@@ -44,9 +44,14 @@ __device__ __forceinline__ bool normalize(V3 a, V3& out) {
   out = a * inv;
   return true;
 }
+// c ? a : b per component: a ternary over whole V3 values can make the
+// compiler pick between two temporaries by address, which puts them in
+// scratch memory.
+__device__ __forceinline__ V3 sel(bool c, V3 a, V3 b) { return mk(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
 __device__ __forceinline__ V3 normalize_or(V3 a, V3 fallback) {
-  V3 r;
-  return normalize(a, r) ? r : fallback;
+  V3 r = fallback;
+  const bool ok = normalize(a, r);
+  return sel(ok, r, fallback);
 }
 
 // RGB triple (physics.rs Color / optics.rs Radiance share this shape).
@@ -130,7 +135,7 @@ struct SceneView {
   uint32_t n_mats;          // index of M_AIR
   uint32_t n_lights;
   uint32_t n_emit;           // photons per pass over all lights
-  uint32_t lds;              // bit 0: lin and lin_box in LDS, bit 1: mats in LDS (scene_to_lds)
+  uint32_t lds;              // in LDS (scene_to_lds): bit 0 lin and lin_box, 1 mats, 2 bvh, 3 compact bprims
   DevCamera cam;
   float4 bmin, bmax;        // Morton quantisation box
 };
@@ -154,6 +159,17 @@ __device__ __forceinline__ void copy16(float4* dst, const float4* src, uint32_t 
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// A small BVH (kernels that walk it, BVH = true): every node of a tree of at
+// most BVH_LDS_NODES nodes, and its primitives, up to BVH_LDS_PRIMS, as
+// compact spheres (centre, radius^2; w = -1 for a primitive of another kind,
+// which is then read from global memory) with their scene indices.
+constexpr uint32_t BVH_LDS_NODES = 255;  // leaf level <= 7: 128 leaves
+constexpr uint32_t BVH_LDS_PRIMS = 512;
+__shared__ Node g_bvh_lds[BVH_LDS_NODES];
+__shared__ float4 g_bsph_lds[BVH_LDS_PRIMS];
+__shared__ int32_t g_bidx_lds[BVH_LDS_PRIMS];
+
+template <bool BVH = false>
 __device__ __forceinline__ void scene_to_lds(SceneView& sc) {
   const bool lin = sc.n_lin <= LIN_LDS && sc.lin_box != nullptr;
   const bool mat = sc.n_mats + 1u <= MAT_LDS;
@@ -162,8 +178,22 @@ __device__ __forceinline__ void scene_to_lds(SceneView& sc) {
     copy16((float4*)g_linbox_lds, (const float4*)sc.lin_box, sc.n_lin * (uint32_t)(sizeof(Node) / 16));
   }
   if (mat) copy16((float4*)g_mat_lds, (const float4*)sc.mats, (sc.n_mats + 1u) * (uint32_t)(sizeof(DevMaterial) / 16));
+  bool nodes = false, prims = false;
+  if (BVH && sc.n_bprims > 0) {
+    const uint32_t n_nodes = (2u << sc.bvh_depth) - 1u;
+    nodes = n_nodes <= BVH_LDS_NODES;
+    prims = sc.n_bprims <= BVH_LDS_PRIMS;
+    if (nodes) copy16((float4*)g_bvh_lds, (const float4*)sc.bvh, n_nodes * (uint32_t)(sizeof(Node) / 16));
+    if (prims)
+      for (uint32_t j = threadIdx.x; j < sc.n_bprims; j += blockDim.x) {
+        const DevShape s = sc.bprims[j];
+        g_bsph_lds[j] = s.kind == SHAPE_SPHERE ? make_float4(s.p.x, s.p.y, s.p.z, s.radius * s.radius)
+                                               : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        g_bidx_lds[j] = (int32_t)__float_as_uint(s.p.w);
+      }
+  }
   __syncthreads();
-  sc.lds = (lin ? 1u : 0u) | (mat ? 2u : 0u);
+  sc.lds = (lin ? 1u : 0u) | (mat ? 2u : 0u) | (nodes ? 4u : 0u) | (prims ? 8u : 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -183,6 +213,23 @@ __device__ __forceinline__ void take(Hit& h, float t, int32_t idx) {
   }
 }
 
+// distance_sphere (geometry.rs:179-193) of the sphere (c, radius^2 = rr).
+__device__ __forceinline__ void sphere_hit(V3 c, float rr, int32_t idx, V3 o, V3 d, Hit& h) {
+#pragma clang fp contract(off)
+  V3 oc = c - o;
+  float t0 = dot(oc, d);
+  float t1 = rr - (dot(oc, oc) - t0 * t0);
+  if (t1 > 0.0f) {
+    float t2 = sqrtf(t1);
+    if (t2 == 0.0f) {
+      take(h, t0, idx);
+    } else {
+      take(h, t0 - t2, idx);
+      take(h, t0 + t2, idx);
+    }
+  }
+}
+
 // distance_plain (geometry.rs:170-177), distance_sphere (:179-193) and
 // method_moller (:149-165) folded into a nearest-hit update.  Candidates with
 // t < NEARLY0 are dropped (tracer.rs:324).
@@ -195,21 +242,9 @@ __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, 
       if (c != 0.0f) take(h, (s.dist + dot(n, o)) / -c, idx);
       break;
     }
-    case SHAPE_SPHERE: {
-      V3 oc = mk(s.p) - o;
-      float t0 = dot(oc, d);
-      float t1 = s.radius * s.radius - (dot(oc, oc) - t0 * t0);
-      if (t1 > 0.0f) {
-        float t2 = sqrtf(t1);
-        if (t2 == 0.0f) {
-          take(h, t0, idx);
-        } else {
-          take(h, t0 - t2, idx);
-          take(h, t0 + t2, idx);
-        }
-      }
+    case SHAPE_SPHERE:
+      sphere_hit(mk(s.p), s.radius * s.radius, idx, o, d, h);
       break;
-    }
     case SHAPE_POLYGON:
     case SHAPE_PARALLELOGRAM: {
       V3 d1 = mk(s.d1), d2 = mk(s.d2);
@@ -272,36 +307,64 @@ __device__ __forceinline__ bool slab(const Node& b, V3 o, V3 inv, float tmax, fl
   return t0 <= t1;
 }
 
+// BVH node i and the nearest-hit test of primitive j: from LDS when
+// scene_to_lds put the tree there, else from global memory.
+__device__ __forceinline__ Node bvh_node(const SceneView& sc, uint32_t i) {
+  if (sc.lds & 4u) return g_bvh_lds[i];
+  return sc.bvh[i];
+}
+__device__ __forceinline__ void bprim_hit(const SceneView& sc, uint32_t j, V3 o, V3 d, Hit& h) {
+  if (sc.lds & 8u) {
+    const float4 c = g_bsph_lds[j];
+    if (c.w >= 0.0f) {
+      sphere_hit(mk(c.x, c.y, c.z), c.w, g_bidx_lds[j], o, d, h);
+      return;
+    }
+  }
+  const DevShape s = sc.bprims[j];
+  shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
+}
+
+// The BVH is a complete binary tree in heap order (children of n: 2n+1,
+// 2n+2), and a depth-first walk only ever defers the sibling of the child it
+// enters.  So instead of a stack, a bit trail (Laine, "Restart trail for
+// stackless BVH traversal", HPG 2010) keeps bit k set while the sibling of
+// the walk's level-k node is still to be visited; trail_pop takes the
+// deepest one, the node a stack would pop, from the current node's
+// ancestor at that level.  No LDS or scratch per lane.
+__device__ __forceinline__ uint32_t heap_level(uint32_t n) { return 31u - __clz(n + 1u); }
+__device__ __forceinline__ uint32_t trail_pop(uint32_t& trail, uint32_t node) {
+  const uint32_t k = 31u - __clz(trail);
+  trail &= ~(1u << k);
+  const uint32_t a = ((node + 1u) >> (heap_level(node) - k)) - 1u;  // the level-k ancestor
+  return ((a - 1u) ^ 1u) + 1u;                                       // its sibling
+}
+
 // Nearest hit over the BVH primitives: ordered depth-first traversal (the
-// nearer child first), the farther child on the lane's LDS stack `stk`
-// (BVH_STACK entries 256 words apart: one column per lane of a 256-thread
-// block, so the lanes of a wave hit distinct banks).  A node is entered while its box starts no later than
-// the best hit, so equal distances still reach the lower shape index.
-__device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h, uint32_t* stk) {
+// nearer child first, the farther one deferred on the bit trail).  A node is
+// entered while its box starts no later than the best hit, so equal
+// distances still reach the lower shape index.
+__device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
   if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
   const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
-  uint32_t node = 0, sp = 0;
+  uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(sc.bvh[0], o, inv, h.t, tn)) return;
+  if (!slab(bvh_node(sc, 0), o, inv, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * BVH_LEAF;
       const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
-      for (uint32_t j = base; j < end; ++j) {
-        const DevShape s = sc.bprims[j];
-        shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
-      }
+      for (uint32_t j = base; j < end; ++j) bprim_hit(sc, j, o, d, h);
     } else {
       const uint32_t l = 2u * node + 1u;
       float tl, tr;
-      const bool hl = slab(sc.bvh[l], o, inv, h.t, tl);
-      const bool hr = slab(sc.bvh[l + 1u], o, inv, h.t, tr);
+      const bool hl = slab(bvh_node(sc, l), o, inv, h.t, tl);
+      const bool hr = slab(bvh_node(sc, l + 1u), o, inv, h.t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
-        stk[sp * 256u] = left_first ? l + 1u : l;
-        ++sp;
         node = left_first ? l : l + 1u;
+        trail |= 1u << heap_level(node);
         continue;
       }
       if (hl || hr) {
@@ -310,10 +373,9 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
       }
     }
     bool more = false;
-    while (sp > 0) {
-      --sp;
-      node = stk[sp * 256u];
-      if (slab(sc.bvh[node], o, inv, h.t, tn)) {
+    while (trail != 0u) {
+      node = trail_pop(trail, node);
+      if (slab(bvh_node(sc, node), o, inv, h.t, tn)) {
         more = true;
         break;
       }
@@ -325,13 +387,13 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
 // calc_intersection (tracer.rs:306-335): the nearest hit over every shape.
 // BVH = false: the scene has no BVH and `lin` holds all shapes.
 template <bool BVH>
-__device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d, uint32_t* stk) {
+__device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d) {
   Hit h{INF_F, -1};
   for (uint32_t i = 0; i < sc.n_lin; ++i) {
     const DevShape s = lin_shape(sc, i);
     shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
   }
-  if (BVH) bvh_nearest(sc, o, d, h, stk);
+  if (BVH) bvh_nearest(sc, o, d, h);
   return h;
 }
 
@@ -357,7 +419,7 @@ __device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit 
   }
   bool flip = dot(n, d) > 0.0f;
   is.pos = p;
-  is.n = flip ? -n : n;
+  is.n = sel(flip, -n, n);
   is.obj = h.obj;
   is.mat = s.mat;
   is.inside_to_out = flip;
@@ -365,8 +427,8 @@ __device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit 
 }
 
 template <bool BVH>
-__device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is, uint32_t* stk) {
-  return finish_hit(sc, o, d, nearest<BVH>(sc, o, d, stk), is);
+__device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is) {
+  return finish_hit(sc, o, d, nearest<BVH>(sc, o, d), is);
 }
 
 // ---------------------------------------------------------------------------
@@ -417,10 +479,9 @@ __device__ __forceinline__ V3 diffuse_reflection(V3 n, Rng& rng) {
   float phi = TWO_PI_F * rng.next();
   float xi = 2.0f * rng.next() - 1.0f;
   float xi2 = sqrtf(fmaxf(1.0f - xi * xi, 0.0f));
-  float sp, cp;
-  __sincosf(phi, &sp, &cp);
+  const float sp = __sinf(phi), cp = __cosf(phi);  // = __sincosf, without its pointer outputs
   V3 d = normalize_or(mk(xi2 * cp, xi, xi2 * sp), mk(0.0f, 1.0f, 0.0f));
-  return dot(n, d) > 0.0f ? d : -d;
+  return sel(dot(n, d) > 0.0f, d, -d);
 }
 
 // reflection_glossy (physics.rs:232-261).  xi1 = xi0^(pw c0) is formed as
@@ -445,8 +506,7 @@ __device__ __forceinline__ V3 reflection_glossy(V3 n, V3 r, float pw, Rng& rng) 
     one_m = -expm1f(t);
   }
   float s = sqrtf(fmaxf(one_m * (1.0f + xi1), 0.0f));
-  float sp, cp;
-  __sincosf(xi2, &sp, &cp);
+  const float sp = __sinf(xi2), cp = __cosf(xi2);
   float x = cp * s, y = xi1, z = sp * s;
   V3 wi = x * u + y * r + z * v;
   if (dot(n, wi) < 0.0f) wi = (-x) * u + y * r - z * v;
@@ -544,46 +604,60 @@ __device__ __forceinline__ BsdfW bsdf_weights(const DevMaterial& m, float cos0) 
 // none of those shapes is hit at all are the other shapes tested, for any
 // hit.  The decision equals that of the nearest-hit test except for a
 // nearest sphere hit whose normal cannot be formed (a sphere of radius ~0).
-__device__ __forceinline__ bool occludes(const DevShape& s, V3 p, V3 d, float sq_l, bool& any) {
+__device__ __forceinline__ bool occluder_hit(const Hit& h, V3 p, V3 d, float sq_l, bool& any) {
 #pragma clang fp contract(off)
-  Hit h{INF_F, -1};
-  shape_hit(s, 0, p, d, h);
   if (h.obj < 0) return false;
   any = true;
   const V3 q = mk(p.x + d.x * h.t, p.y + d.y * h.t, p.z + d.z * h.t);  // as finish_hit forms it
   const V3 od = q - p;
   return sq_l - dot(od, od) > LIGHT_SLACK_F;
 }
+__device__ __forceinline__ bool occludes(const DevShape& s, V3 p, V3 d, float sq_l, bool& any) {
+  Hit h{INF_F, -1};
+  shape_hit(s, 0, p, d, h);
+  return occluder_hit(h, p, d, sq_l, any);
+}
+// occludes() for BVH primitive j (LDS compact sphere when there is one)
+__device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, V3 p, V3 d, float sq_l, bool& any) {
+  if (sc.lds & 8u) {
+    const float4 c = g_bsph_lds[j];
+    if (c.w >= 0.0f) {
+      Hit h{INF_F, -1};
+      sphere_hit(mk(c.x, c.y, c.z), c.w, 0, p, d, h);
+      return occluder_hit(h, p, d, sq_l, any);
+    }
+  }
+  return occludes(sc.bprims[j], p, d, sq_l, any);
+}
 
 // Any-hit form of bvh_nearest for the shadow test: true at the first
 // occluder; sets any when some primitive is hit.  Once a hit is known only
 // boxes entered before t_occ (beyond which no hit can be an occluder) are
 // searched.
-__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any, uint32_t* stk) {
+__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
   if (sc.n_bprims == 0) return false;
   const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   const float t_occ = sqrtf(fmaxf(sq_l - LIGHT_SLACK_F, 0.0f)) * 1.001f + 1e-4f;
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
-  uint32_t node = 0, sp = 0;
+  uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(sc.bvh[0], o, inv, any ? t_occ : INF_F, tn)) return false;
+  if (!slab(bvh_node(sc, 0), o, inv, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * BVH_LEAF;
       const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
       for (uint32_t j = base; j < end; ++j)
-        if (occludes(sc.bprims[j], o, d, sq_l, any)) return true;
+        if (bprim_occludes(sc, j, o, d, sq_l, any)) return true;
     } else {
       const uint32_t l = 2u * node + 1u;
       const float tmax = any ? t_occ : INF_F;
       float tl, tr;
-      const bool hl = slab(sc.bvh[l], o, inv, tmax, tl);
-      const bool hr = slab(sc.bvh[l + 1u], o, inv, tmax, tr);
+      const bool hl = slab(bvh_node(sc, l), o, inv, tmax, tl);
+      const bool hr = slab(bvh_node(sc, l + 1u), o, inv, tmax, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
-        stk[sp * 256u] = left_first ? l + 1u : l;
-        ++sp;
         node = left_first ? l : l + 1u;
+        trail |= 1u << heap_level(node);
         continue;
       }
       if (hl || hr) {
@@ -592,10 +666,9 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
       }
     }
     bool more = false;
-    while (sp > 0) {
-      --sp;
-      node = stk[sp * 256u];
-      if (slab(sc.bvh[node], o, inv, any ? t_occ : INF_F, tn)) {
+    while (trail != 0u) {
+      node = trail_pop(trail, node);
+      if (slab(bvh_node(sc, node), o, inv, any ? t_occ : INF_F, tn)) {
         more = true;
         break;
       }
@@ -646,7 +719,7 @@ __device__ __forceinline__ uint64_t occluder_candidates(const SceneView& sc, con
 
 template <bool BVH>
 __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, float& cos0sq,
-                                           float& sqd, uint32_t* stk) {
+                                           float& sqd) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
@@ -657,7 +730,7 @@ __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 l
     if (i < CAND_BITS && ((cand >> i) & 1ull) == 0) continue;
     if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
   }
-  if (BVH && bvh_shadow(sc, p, ld3, sq_l, any, stk)) return false;
+  if (BVH && bvh_shadow(sc, p, ld3, sq_l, any)) return false;
   // no occluder; lit iff the ray meets something at all (calc_intersection
   // returns None otherwise, tracer.rs:284)
   for (uint32_t i = 0; i < min(sc.n_lin, CAND_BITS) && !any; ++i) {
@@ -700,7 +773,7 @@ __device__ __forceinline__ V3 tss_point(const DevLight& L, int s) {
 }
 
 template <bool BVH>
-__device__ C3 direct_light(const SceneView& sc, V3 p, V3 n, uint32_t* stk) {
+__device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
   C3 sum{0.0f, 0.0f, 0.0f};
   for (uint32_t li = 0; li < sc.n_lights; ++li) {
     const DevLight L = ldc(sc.lights, li);
@@ -712,7 +785,7 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n, uint32_t* stk) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BVH>(sc, p, n, ld, cand, csq, sqd, stk)) continue;
+      if (!sample_lit<BVH>(sc, p, n, ld, cand, csq, sqd)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }

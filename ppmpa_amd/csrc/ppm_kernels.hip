@@ -133,11 +133,15 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
   // select_wavelength / decide_wavelength (light.rs:157-160, physics.rs:74-84)
   float u = rng.next();
   p.wl = u < L.color.x ? 0u : (u < L.color.x + L.color.y ? 1u : 2u);
+  // origin and direction are set once after the branches: stores to
+  // different fields in each branch get merged into one store through a
+  // chosen address, which keeps the photon state out of registers
+  V3 o, d;
   if (L.kind == LIGHT_PARALLELOGRAM) {
     float t1 = rng.next();
     float t2 = rng.next();
-    p.d = diffuse_reflection(mk(L.nvec), rng);
-    p.o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
+    d = diffuse_reflection(mk(L.nvec), rng);
+    o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
   } else if (L.kind == LIGHT_POINT) {
     // generate_random_dir (algebra.rs:211-223): rejection in the unit ball
     V3 v = mk(0.0f, 1.0f, 0.0f);
@@ -152,14 +156,16 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
         break;
       }
     }
-    p.d = v;
-    p.o = mk(L.pos);
+    d = v;
+    o = mk(L.pos);
   } else {  // SunLight
     float t1 = rng.next();
     float t2 = rng.next();
-    p.o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
-    p.d = mk(L.dir);
+    o = mk(L.pos) + t1 * mk(L.d1) + t2 * mk(L.d2);
+    d = mk(L.dir);
   }
+  p.o = o;
+  p.d = d;
   p.med = sc.n_mats;  // M_AIR
   p.depth = 0;
   p.pid = pid;
@@ -170,12 +176,12 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
 // reports a photon-map record (position, INCOMING direction, channel).
 template <bool BVH>
 __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int maxd, PhotonState& p, Rng& rng,
-                                              bool& store, V3& spos, uint32_t* stk, int& ev_mat) {
+                                              bool& store, V3& spos, int& ev_mat) {
   store = false;
   ev_mat = -1;
   if ((int)p.depth >= maxd) return false;  // tracer.rs:32
   Isect is;
-  if (!intersect<BVH>(sc, p.o, p.d, is, stk)) return false;  // tracer.rs:36-39
+  if (!intersect<BVH>(sc, p.o, p.d, is)) return false;  // tracer.rs:36-39
   const DevMaterial M = material(sc, is.mat);
   // tracer.rs:77-79
   store = (uc == 0 || p.depth > 0) && surf_store(M);
@@ -292,10 +298,9 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
                                                        uint64_t seed, int uc, int maxd, Blk blk,
                                                        AdjTrace at = AdjTrace{nullptr, nullptr}) {
-  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   __shared__ float part[ADJ ? 3 * ADJ_LDS_MATS : 1];
   __shared__ uint16_t evl[ADJ ? ADJ_MAX_EV * 256 : 1];
-  scene_to_lds(sc);
+  scene_to_lds<BVH>(sc);
   const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (ADJ) {
     for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
@@ -343,7 +348,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
     const uint32_t swl = p.wl;
     const uint32_t smeta = pack_meta(p.wl, p.depth, p.pid);
     int ev_mat = -1;
-    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, stk + threadIdx.x, ev_mat);
+    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, ev_mat);
     const uint64_t sm = __ballot(store);
     if (sm) {
       if (store) {
@@ -1277,173 +1282,240 @@ __global__ __launch_bounds__(256) void k_tile_order(EyeBuf eb, PassState* st) {
   k_tile_order_body(eb, st, Blk{blockIdx.x, gridDim.x});
 }
 
+// Lanes of the calling wave with `pred` whose 6-bit `key` equals this lane's
+// (6 ballots; called by the whole wave).
+__device__ __forceinline__ uint64_t peer_mask(uint32_t key, bool pred) {
+  uint64_t m = __ballot(pred);
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    const uint64_t bb = __ballot(((key >> b) & 1u) != 0u);
+    m &= ((key >> b) & 1u) ? bb : ~bb;
+  }
+  return m;
+}
+
 // Persistent eye kernel.  A wave takes tiles in the order of tile_order from
-// the pass's counter, and keeps 64 trees in flight: each loop step walks one
-// node per lane, and a lane whose tree is done takes the next pixel slot of
-// the wave's current tile (ballot + mbcnt), the wave taking a new tile when
-// its tile is used up; so a deep tree holds one lane, not a wave, and the
-// costly tiles (behind glass) start first.  The pixel's queries go to its
-// own slots: the primary one to t, the k-th deeper one to q_primary*(1+k)+t,
-// further ones to the overflow region.
+// the pass's counter and walks the eye trees of a tile's 64 pixels (one per
+// lane to start with) together: each loop step processes one node per busy
+// lane; a node with both a reflection and a refraction child continues with
+// the reflection and puts the refraction child on the wave's pending pool
+// (LIFO, in global memory), and idle lanes take the newest pool entries, so
+// the deep trees of a tile behind glass are spread over all 64 lanes.  A
+// node's results depend only on the node (its random stream is keyed by
+// pixel, depth and path); the tile's per-pixel state lives in LDS: the
+// count of in-place deeper query slots used and the emitted radiance.  Slots
+// and sums are assigned in lane order within a step, and a wave holds one
+// tile at a time, so the schedule, and with it every slot and sum, depends
+// only on the tile: results are reproducible.  Deeper queries beyond a
+// pixel's sec_max slots go to the overflow region (atomic slots, added in
+// any order).
 template <bool BVH>
 __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves, Blk blk) {
-  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  scene_to_lds(sc);
-  const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
+  __shared__ uint32_t s_nsec[4 * 64];  // per wave of the block and pixel of its tile
+  __shared__ float s_acc[4 * 64 * 3];
+  scene_to_lds<BVH>(sc);
+  const uint32_t wb = threadIdx.x >> 6;
+  const uint32_t w = blk.b * (blockDim.x / 64) + wb;
   if (w >= n_waves) return;
+  uint32_t* const nsec = s_nsec + wb * 64u;
+  float* const acc3 = s_acc + wb * 192u;
+  const uint32_t lane = lane_id();
   const uint32_t pass = st->pass;
   // draws of camera_ray on the pixel's root stream, which node 0 continues
   const uint32_t cam_draws = (sc.cam.blur ? 2u : 0u) + (sc.cam.progressive && sc.cam.antialias ? 2u : 0u);
-  uint32_t tile = 0, used = 64;  // wave-uniform: current tile, its slots handed out
-  bool tiles_left = true;        // wave-uniform
-  bool alive = false, done = false;
-  uint32_t t = 0, pixel = 0, n_sec = 0;
-  C3 acc{0.0f, 0.0f, 0.0f};
-  EyeNode cur;
-  // pending refraction nodes in global memory (a private array would need
-  // scratch, which limits how many waves the hardware keeps resident)
-  const uint32_t L = eb.estack_lanes, gl = w * 64u + lane_id();
-  uint32_t sp = 0;
-  while (true) {
-    const bool need = !alive && !done;
-    const uint64_t m = __ballot(need);
-    if (m) {
-      if (used == 64u && tiles_left) {
-        const uint32_t k = wave_next(&st->eye_next);
-        if (k < eb.n_tiles) {
-          tile = eb.tile_order[k];
-          used = 0;
-        } else {
-          tiles_left = false;
-        }
-      }
-      const uint32_t avail = 64u - used;
-      const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
-      if (need) {
-        if (rank < avail) {
-          t = tile * 64u + used + rank;
-          const uint32_t wi = t & 63u;
-          const int x = (int)((tile % (uint32_t)sc.cam.tiles_x) * 8u + (wi & 7u));
-          const int y = (int)((tile / (uint32_t)sc.cam.tiles_x) * 8u + (wi >> 3));
-          eb.q[t].a.w = __uint_as_float(NO_PIXEL);
-          if (x < sc.cam.xreso && y < sc.cam.yreso) {
-            pixel = (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x;
-            Rng rng;
-            rng.init(seed, pixel, pass, TAG_EYE);
-            camera_ray(sc.cam, x, y, rng, cur.o, cur.d);
-            cur.w = C3{1.0f, 1.0f, 1.0f};
-            cur.meta = (sc.n_mats << 5);
-            acc = C3{0.0f, 0.0f, 0.0f};
-            n_sec = 0;
-            sp = 0;
-            alive = true;
-          } else {
-            eb.sec_count[t] = 0;  // a slot past the image edge: no pixel
-          }
-        } else if (!tiles_left) {
-          done = true;
-        }
-      }
-      used += min((uint32_t)__popcll(m), avail);
+  const uint32_t cap = 64u * eb.estack_depth;  // pool entries of this wave
+  float4* const pool = eb.estack + (uint64_t)w * cap * 3u;
+  const uint32_t tx = (uint32_t)sc.cam.tiles_x;
+  for (uint32_t k = wave_next(&st->eye_next); k < eb.n_tiles; k = wave_next(&st->eye_next)) {
+    const uint32_t tile = eb.tile_order[k];
+    const uint32_t x0 = (tile % tx) * 8u, y0 = (tile / tx) * 8u;
+    const uint32_t t = tile * 64u + lane;
+    const int x = (int)(x0 + (lane & 7u)), y = (int)(y0 + (lane >> 3));
+    const bool inside = x < sc.cam.xreso && y < sc.cam.yreso;
+    const uint32_t my_pixel = inside ? (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x : 0u;
+    nsec[lane] = 0u;
+    acc3[3u * lane] = 0.0f;
+    acc3[3u * lane + 1u] = 0.0f;
+    acc3[3u * lane + 2u] = 0.0f;
+    eb.q[t].a.w = __uint_as_float(NO_PIXEL);
+    bool alive = false;
+    uint32_t own = lane;  // the pixel (position in the tile) of the lane's node
+    EyeNode cur;
+    if (inside) {
+      Rng rng;
+      rng.init(seed, my_pixel, pass, TAG_EYE);
+      camera_ray(sc.cam, x, y, rng, cur.o, cur.d);
+      cur.w = C3{1.0f, 1.0f, 1.0f};
+      cur.meta = (sc.n_mats << 5);
+      alive = true;
     }
-    if (!__any(alive || !done)) break;
-    if (!alive) continue;
-    const uint32_t depth = cur.meta & 31u;
-    const uint32_t med = (cur.meta >> 5) & 2047u;
-    const uint32_t path = cur.meta >> 16;
-    bool next_set = false;
-    EyeNode nxt;
-    Isect is;
-    if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is, stk + threadIdx.x)) {
-      const DevMaterial M = material(sc, is.mat);
-      const DevMaterial Mm = material(sc, med);
-      Rng nr;
-      nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
-      if (depth == 0)
-        for (uint32_t k = 0; k < cam_draws; ++k) (void)nr.next();
-      acc = acc + cur.w * (c3(M.em) * SR_HALF_F);
-      float cos1;
-      V3 rdir = specular_reflection(is.n, cur.d, cos1);
-      const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
-      V3 tdir = mk(0.0f, 0.0f, 0.0f);
-      bool has_t = false;
-      float cos2 = 0.0f;
-      if (spec_part) {
-        // tracer.rs:150-161
-        rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
-        const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
-        const V3 hvec = normalize_or(rdir - cur.d, is.n);
-        has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+    uint32_t top = 0;  // wave-uniform: pool entries
+    while (true) {
+      // idle lanes take the newest pool entries, in lane order
+      const uint64_t idle = __ballot(!alive);
+      const uint32_t take = min((uint32_t)__popcll(idle), top);
+      if (take) {
+        if (!alive) {
+          const uint32_t r = (uint32_t)__popcll(idle & lanemask_lt());
+          if (r < take) {
+            const float4* e = pool + (uint64_t)(top - 1u - r) * 3u;
+            const float4 a = e[0], b = e[1], c = e[2];
+            cur.o = mk(a.x, a.y, a.z);
+            cur.meta = __float_as_uint(a.w);
+            cur.d = mk(b.x, b.y, b.z);
+            cur.w = C3{b.w, c.x, c.y};
+            own = __float_as_uint(c.z);
+            alive = true;
+          }
+        }
+        top -= take;
       }
-      const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
-      const BsdfW W = bsdf_weights(M, cos0);
-      const C3 wd = cur.w * W.wd;
-      if (nonzero(wd)) {
-        // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
-        // both are evaluated per query by k_shade
-        uint32_t slot = t;
-        if (depth > 0) {
-          // deeper query: the pixel's own slots, else the overflow region
-          if (n_sec < eb.sec_max) {
-            slot = eb.q_primary * (1u + n_sec) + t;
-            ++n_sec;
+      if (!__any(alive)) break;
+      const uint32_t px = (uint32_t)y0 + (own >> 3), qx = x0 + (own & 7u);
+      const uint32_t pixel = px * (uint32_t)sc.cam.xreso + qx;
+      const uint32_t to = tile * 64u + own;  // the pixel's primary slot
+      bool next_set = false, push = false, want_q = false;
+      EyeNode nxt, pc;
+      C3 emit{0.0f, 0.0f, 0.0f};
+      Query qq;
+      const uint32_t depth = cur.meta & 31u;
+      if (alive) {
+        const uint32_t med = (cur.meta >> 5) & 2047u;
+        const uint32_t path = cur.meta >> 16;
+        Isect is;
+        if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is)) {
+          const DevMaterial M = material(sc, is.mat);
+          const DevMaterial Mm = material(sc, med);
+          Rng nr;
+          nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
+          if (depth == 0)
+            for (uint32_t j = 0; j < cam_draws; ++j) (void)nr.next();
+          emit = cur.w * (c3(M.em) * SR_HALF_F);
+          float cos1;
+          V3 rdir = specular_reflection(is.n, cur.d, cos1);
+          const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
+          V3 tdir = mk(0.0f, 0.0f, 0.0f);
+          bool has_t = false;
+          float cos2 = 0.0f;
+          if (spec_part) {
+            // tracer.rs:150-161
+            rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
+            const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
+            const V3 hvec = normalize_or(rdir - cur.d, is.n);
+            has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+          }
+          const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
+          const BsdfW W = bsdf_weights(M, cos0);
+          const C3 wd = cur.w * W.wd;
+          if (nonzero(wd)) {
+            // di = direct light (when uc) + photon estimate (tracer.rs:136-145):
+            // both are evaluated per query by k_shade
+            want_q = true;
+            qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
+            qq.b = make_float4(is.n.x, is.n.y, is.n.z, __uint_as_float(is.mat));
+            qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
+          }
+          if ((int)depth + 1 < maxd) {
+            const C3 ws = cur.w * W.ws;
+            const C3 wt = cur.w * W.wt;
+            const bool rc = surf_reflect(M, cos1) && nonzero(ws);
+            const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
+            if (tc) {
+              const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
+              const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
+              if (rc) {
+                pc = c;
+                push = true;
+              } else {
+                nxt = c;
+                next_set = true;
+              }
+            }
+            if (rc) {
+              nxt = EyeNode{is.pos, rdir, ws, (depth + 1) | (med << 5) | (path << 16)};
+              next_set = true;
+            }
+          }
+        }
+      }
+      // emitted radiance: each pixel's contributions of this step, in lane
+      // order, added by the pixel's lowest lane to the pixel's sum
+      const bool has_e = alive && nonzero(emit);
+      if (__any(has_e)) {
+        const uint64_t pe = peer_mask(own, has_e);
+        const bool lead = has_e && (pe & lanemask_lt()) == 0u;
+        uint64_t rest = pe;
+        float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+        if (lead) {
+          ar = acc3[3u * own];
+          ag = acc3[3u * own + 1u];
+          ab = acc3[3u * own + 2u];
+        }
+        while (__any(lead && rest != 0u)) {
+          const bool go = lead && rest != 0u;
+          const int src = go ? __ffsll((unsigned long long)rest) - 1 : (int)lane;
+          const float er = __shfl(emit.r, src), eg = __shfl(emit.g, src), eb_ = __shfl(emit.b, src);
+          if (go) {
+            ar += er;
+            ag += eg;
+            ab += eb_;
+            rest &= rest - 1u;
+          }
+        }
+        if (lead) {
+          acc3[3u * own] = ar;
+          acc3[3u * own + 1u] = ag;
+          acc3[3u * own + 2u] = ab;
+        }
+      }
+      // query slots: the root's is the pixel's primary slot; deeper ones take
+      // the pixel's next in-place slots in lane order, then the overflow region
+      const bool need = want_q && depth > 0;
+      uint32_t slot = want_q ? to : 0xFFFFFFFFu;
+      if (__any(need)) {
+        const uint64_t pq = peer_mask(own, need);
+        const uint32_t rank = (uint32_t)__popcll(pq & lanemask_lt());
+        const uint32_t base = need ? nsec[own] : 0u;
+        __asm__ volatile("" ::: "memory");  // every lane reads before the leader writes
+        if (need) {
+          const uint32_t idx = base + rank;
+          if (idx < eb.sec_max) {
+            slot = eb.q_primary * (1u + idx) + to;
           } else {
             const uint32_t j = atomicAdd(&st->n_queries, 1u);
             slot = j < eb.ovf_cap ? eb.q_primary * (1u + eb.sec_max) + j : 0xFFFFFFFFu;
             if (j >= eb.ovf_cap) atomicOr(&st->overflow, 2u);
           }
-        }
-        if (slot != 0xFFFFFFFFu) {
-          Query qq;
-          qq.a = make_float4(is.pos.x, is.pos.y, is.pos.z, __uint_as_float(pixel));
-          qq.b = make_float4(is.n.x, is.n.y, is.n.z, __uint_as_float(is.mat));
-          qq.w = make_float4(wd.r, wd.g, wd.b, 0.0f);
-          eb.q[slot] = qq;
+          if (rank == 0u) nsec[own] = min(base + (uint32_t)__popcll(pq), eb.sec_max);
         }
       }
-      if ((int)depth + 1 < maxd) {
-        const C3 ws = cur.w * W.ws;
-        const C3 wt = cur.w * W.wt;
-        const bool rc = surf_reflect(M, cos1) && nonzero(ws);
-        const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
-        if (tc) {
-          const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
-          const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
-          if (rc) {
-            if (sp < eb.estack_depth) {
-              float4* e = eb.estack + (uint64_t)(3u * sp) * L + gl;
-              e[0] = make_float4(c.o.x, c.o.y, c.o.z, __uint_as_float(c.meta));
-              e[L] = make_float4(c.d.x, c.d.y, c.d.z, c.w.r);
-              e[2 * (uint64_t)L] = make_float4(c.w.g, c.w.b, 0.0f, 0.0f);
-              ++sp;
-            }
+      if (want_q && slot != 0xFFFFFFFFu) eb.q[slot] = qq;
+      // pending refraction children onto the pool, in lane order
+      const uint64_t pm = __ballot(push);
+      if (pm) {
+        if (push) {
+          const uint32_t e = top + (uint32_t)__popcll(pm & lanemask_lt());
+          if (e < cap) {
+            float4* q = pool + (uint64_t)e * 3u;
+            q[0] = make_float4(pc.o.x, pc.o.y, pc.o.z, __uint_as_float(pc.meta));
+            q[1] = make_float4(pc.d.x, pc.d.y, pc.d.z, pc.w.r);
+            q[2] = make_float4(pc.w.g, pc.w.b, __uint_as_float(own), 0.0f);
           } else {
-            nxt = c;
-            next_set = true;
+            atomicOr(&st->overflow, 8u);
           }
         }
-        if (rc) {
-          nxt = EyeNode{is.pos, rdir, ws, (depth + 1) | (med << 5) | (path << 16)};
-          next_set = true;
-        }
+        top = min(top + (uint32_t)__popcll(pm), cap);
       }
+      if (next_set)
+        cur = nxt;
+      else
+        alive = false;
     }
-    if (next_set) {
-      cur = nxt;
-    } else if (sp > 0) {
-      --sp;
-      const float4* e = eb.estack + (uint64_t)(3u * sp) * L + gl;
-      const float4 a = e[0], b = e[L], c = e[2 * (uint64_t)L];
-      cur.o = mk(a.x, a.y, a.z);
-      cur.meta = __float_as_uint(a.w);
-      cur.d = mk(b.x, b.y, b.z);
-      cur.w = C3{b.w, c.x, c.y};
-    } else {
-      eb.pix[pixel] = make_float4(acc.r, acc.g, acc.b, 0.0f);
-      eb.sec_count[t] = n_sec;
-      alive = false;
-    }
+    // the tile is done: each lane writes its pixel's emitted radiance and
+    // in-place query count
+    if (inside) eb.pix[my_pixel] = make_float4(acc3[3u * lane], acc3[3u * lane + 1u], acc3[3u * lane + 2u], 0.0f);
+    eb.sec_count[t] = inside ? nsec[lane] : 0u;
   }
 }
 
@@ -1625,8 +1697,7 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 template <bool BVH>
 __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc) {
-  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
-  scene_to_lds(sc);
+  scene_to_lds<BVH>(sc);
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
   const GridView g = grid_view(st, sc.bmin);
@@ -1649,7 +1720,7 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
                              ab += wl == 2 ? v : 0.0f;
                            });
     C3 di = C3{ar * scale, ag * scale, ab * scale};
-    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b), stk + threadIdx.x);
+    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b));
     eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
   if (!complete) atomicOr(&st->overflow, 4u);
@@ -1671,7 +1742,6 @@ __device__ __forceinline__ void k_shade_tree_body(SceneView sc, const Node* node
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc, Blk blk) {
   __shared__ LeafLds lds[4];
-  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t wib = threadIdx.x >> 6, lane = lane_id();
   const uint32_t n = st->n_stored, depth = st->depth;
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
@@ -1735,7 +1805,7 @@ __device__ __forceinline__ void k_shade_tree_body(SceneView sc, const Node* node
       node += 1;
     }
     C3 di = C3{ar * scale, ag * scale, ab * scale};
-    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b), stk + threadIdx.x);
+    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b));
     if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
 }
@@ -2064,10 +2134,9 @@ namespace ppm {
 template <bool BVH>
 __global__ __launch_bounds__(256) void k_intersect(SceneView sc, const float4* o, const float4* d, uint32_t n,
                                                    float* t_out, int32_t* obj_out) {
-  __shared__ uint32_t stk[BVH ? BVH_STACK * 256 : 1];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Hit h = nearest<BVH>(sc, mk(o[i]), mk(d[i]), stk + threadIdx.x);
+  const Hit h = nearest<BVH>(sc, mk(o[i]), mk(d[i]));
   t_out[i] = h.obj >= 0 ? h.t : INF_F;
   obj_out[i] = h.obj;
 }
@@ -2121,7 +2190,7 @@ __device__ __forceinline__ BatchItem load_item(const BatchItem* items, uint32_t 
 __global__ void kb_pass_begin(const BatchItem* items, uint32_t n) {
   if (blockIdx.x >= n) return;
   const BatchItem it = load_item(items, blockIdx.x);
-  k_pass_begin_body(it.st, it.first_pass, it.r2_first, it.alpha, it.power, 1u, Blk{0u, 1u});
+  k_pass_begin_body(it.st, it.first_pass, it.r2_first, it.alpha, it.power, it.pass_stride, Blk{0u, 1u});
 }
 
 __global__ void kb_pass_end(const BatchItem* items, uint32_t n) {

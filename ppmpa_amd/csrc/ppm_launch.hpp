@@ -88,12 +88,13 @@ struct EyeBuf {
   uint32_t* qtotals = nullptr;
   uint32_t q_sort_blocks = 0;
   float4* res = nullptr;
-  // pending refraction nodes of the eye trees, per lane of the eye kernel:
-  // entry k of lane L is the three float4 estack[(3k + c) * estack_lanes + L]
-  // (lanes of a wave at the same depth touch adjacent addresses)
+  // pending refraction nodes of the eye trees: wave w of the eye kernel
+  // keeps a pool of 64 * estack_depth entries, entry e being the three
+  // float4 estack[(w * 64 * estack_depth + e) * 3 + c] (origin + node meta,
+  // direction + weight r, weight g b + pixel of the tile)
   float4* estack = nullptr;
   uint32_t estack_lanes = 0;  // 64 * the most eye waves a launch may use
-  uint32_t estack_depth = 0;  // entries per lane
+  uint32_t estack_depth = 0;  // pool entries per lane of a wave
 };
 
 // Eye waves per CU at most (the launch configurations a batch may tune to).
@@ -175,7 +176,8 @@ struct BatchItem {
   uint64_t seed;
   double r2_first, alpha, power;
   float4 mscale;             // Morton scale of the scene bounds (1023 cells per axis)
-  uint32_t first_pass;
+  uint32_t first_pass;       // this pipeline's first pass
+  uint32_t pass_stride;      // passes between two of this pipeline's (the batch's pipelines)
   int32_t uc, maxd, pfilter;
   uint32_t trace_per_wave;   // photons per tracing wave
   uint32_t eye_waves;        // persistent eye waves

@@ -47,12 +47,11 @@ constexpr int MAX_DEPTH_LIMIT = 16;  // deepest MAX_TRACE the kernels accept
 constexpr int LEAF = PPM_LEAF;       // photons per map leaf (<= one wavefront)
 
 // Scene BVH: a heap-ordered complete binary tree over leaves of BVH_LEAF
-// primitives in Morton order of their centroids.  Traversal keeps one
-// BVH_STACK-entry stack per lane in LDS; a descent pushes at most one node
-// per level, so BVH_STACK >= BVH_MAX_DEPTH suffices.
+// primitives in Morton order of their centroids.  Traversal defers at most
+// one node per level, kept as one bit per level of a 32-bit trail, so
+// BVH_MAX_DEPTH < 32.
 constexpr uint32_t BVH_LEAF = 4;
 constexpr uint32_t BVH_MAX_DEPTH = 22;   // up to 4M leaves (16M primitives)
-constexpr uint32_t BVH_STACK = 24;
 constexpr uint32_t BVH_MIN_PRIMS = 64;   // automatic mode: fewer bounded shapes are tested one by one
 constexpr float BVH_EMPTY = 3.0e38f;     // an empty node is the point box (BVH_EMPTY,)^3
 

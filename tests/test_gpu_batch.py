@@ -1,9 +1,11 @@
 """GPU: batched passes over many scenes.  A batch runs every pass stage as one
 launch over all its scenes; each scene's blocks run the single-scene kernel
 bodies on that scene's buffers, so every scene's image must equal the image
-of its own session with the same schedule, bit for bit except in pixels
-whose eye tree overflowed its 2 * max_depth query slots (their extra queries
-are added with float atomics, in no fixed order, in a session as well)."""
+of its own session with the same schedule and the same number of pass
+pipelines (a batch runs round r on pipeline r mod K, as a session runs pass
+r), bit for bit except in pixels whose eye tree overflowed its 2 * max_depth
+query slots (their extra queries are added with float atomics, in no fixed
+order, in a session as well)."""
 import numpy as np
 import pytest
 
@@ -26,15 +28,15 @@ def _same(a, b):
     return np.allclose(a, b, rtol=1e-6, atol=0.0) and (a == b).mean() >= 0.99
 
 
-def _cfg(c):
+def _cfg(c, pipelines=2):
     return P.pass_cfg(seed=c["seed"], n_photon=c["n_photon"], radius0=c["radius0"], alpha=c["alpha"],
-                      max_depth=c["max_depth"])
+                      max_depth=c["max_depth"], pipelines=pipelines)
 
 
-def _plain_images(c, items, dss):
+def _plain_images(c, items, dss, pipelines=2):
     imgs = []
     for (sc, cfg, passes), ds in zip(items, dss):
-        s = P.Session(c, ds, _cfg(cfg))
+        s = P.Session(c, ds, _cfg(cfg, pipelines))
         s.run(passes)
         m, v, k = s.image()
         s.close()
@@ -43,15 +45,15 @@ def _plain_images(c, items, dss):
     return imgs
 
 
-@pytest.mark.parametrize("seed", [3, 11])
-def test_batch_equals_sessions(seed):
-    say("batch == sessions", seed)
+@pytest.mark.parametrize("seed,pipelines", [(3, 2), (11, 1), (7, 3)])
+def test_batch_equals_sessions(seed, pipelines):
+    say("batch == sessions", seed, pipelines)
     c = ctx()
     items = _small_batch(10, seed)
     dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
     kinds = {ds.stats()[1] > 0 for ds in dss}
-    plain = _plain_images(c, items, dss)
-    b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    plain = _plain_images(c, items, dss, pipelines)
+    b = P.Batch(c, dss, [_cfg(cfg, pipelines) for _, cfg, _ in items], [p for _, _, p in items])
     b.run()
     for i, (sc, cfg, passes) in enumerate(items):
         m, v, k = b.image(i)
