@@ -698,7 +698,31 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
   if (bs != PPM_OK) {
     ppm_scene_destroy(sc);
     return bs;
+  }  // a ParallelogramLight's own shape: a parallelogram of the shapes tested
+  // one by one with exactly the light's corner and sides (pos.w holds its
+  // index there, else ~0); the shadow test knows it is hit, not an occluder
+  for (DevLight& l : sc->lights) {
+    uint32_t self = 0xFFFFFFFFu;
+    if (l.kind == LIGHT_PARALLELOGRAM)
+      for (size_t k = 0; k < sc->lin_host.size() && k < 64; ++k) {
+        const DevShape& sh = sc->lin_host[k];
+        if (sh.kind == SHAPE_PARALLELOGRAM && sh.p.x == l.pos.x && sh.p.y == l.pos.y && sh.p.z == l.pos.z &&
+            sh.d1.x == l.d1.x && sh.d1.y == l.d1.y && sh.d1.z == l.d1.z && sh.d2.x == l.d2.x &&
+            sh.d2.y == l.d2.y && sh.d2.z == l.d2.z) {
+          self = (uint32_t)k;
+          break;
+        }
+      }
+    std::memcpy(&l.pos.w, &self, sizeof(self));
   }
+  if (!sc->lights.empty()) {
+    e = hipMemcpy(b.lights, sc->lights.data(), sizeof(DevLight) * sc->lights.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      ppm_scene_destroy(sc);
+      return fail(c, PPM_E_HIP, hipGetErrorString(e));
+    }
+  }
+
   *out = sc;
   return PPM_OK;
 }

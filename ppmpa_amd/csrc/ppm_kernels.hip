@@ -1064,9 +1064,8 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
 // sorted order within a bucket, so a query's sum is formed in a fixed order.
 // Per lane: no cross-lane operations, so inactive lanes may skip it.
 template <typename OnPhoton>
-__device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const PhotonRec* ph, uint32_t b, float qx,
-                                                 float qy, float qz, float r2, OnPhoton& on_photon) {
-  const uint2 se = cells[b];
+__device__ __forceinline__ void grid_range_scan(const PhotonRec* ph, uint2 se, float qx, float qy, float qz, float r2,
+                                                OnPhoton& on_photon) {
   uint32_t j = se.x;
   // four whole records per step: their loads are in flight together
   for (; j + 3 < se.y; j += 4) {
@@ -1085,6 +1084,12 @@ __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const Photo
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
     if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
   }
+}
+
+template <typename OnPhoton>
+__device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const PhotonRec* ph, uint32_t b, float qx,
+                                                 float qy, float qz, float r2, OnPhoton& on_photon) {
+  grid_range_scan(ph, cells[b], qx, qy, qz, r2, on_photon);
 }
 
 // Returns false (and scans only the first three cells of an axis) if the
@@ -1112,9 +1117,13 @@ __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells
 #pragma unroll
       for (int j = 0; j < k; ++j) v[k] = v[k] && !(v[j] && b[j] == b[k]);
     }
+    // the cells' ranges are loaded together before any is scanned
+    uint2 se[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) se[k] = v[k] ? cells[b[k]] : make_uint2(0u, 0u);
 #pragma unroll
     for (int k = 0; k < 8; ++k)
-      if (v[k]) grid_bucket_scan(cells, ph, b[k], qx, qy, qz, r2, on_photon);
+      if (v[k]) grid_range_scan(ph, se[k], qx, qy, qz, r2, on_photon);
     return true;
   }
   // three cells on some axis (float rounding at the edge of the bound): the

@@ -89,7 +89,7 @@ struct alignas(16) DevLight {
   uint32_t first_photon;  // photons [first, first + count) of a pass come from here
   uint32_t n_photon;
   float4 color;
-  float4 pos;
+  float4 pos;    // w: the light's own shape among the shapes tested one by one (uint bits; ~0: none)
   float4 nvec;
   float4 d1;
   float4 d2;
