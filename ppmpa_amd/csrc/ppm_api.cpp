@@ -1165,7 +1165,8 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
     hs.scale = (float)(m->power / (M_PI * r2));
     hs.n_queries = 0;
     hs.overflow = 0;
-    hs.direct_next = hs.shade_next = hs.eye_next = hs.tiles_front = hs.tiles_back = 0;
+    hs.tiles_front = hs.tiles_back = 0;
+    std::memset(hs.work, 0, sizeof(hs.work));
     if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth, c->n_cu)) != hipSuccess) break;
     if ((e = launch_gather(c->stream, sc->buf, m->m, st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=

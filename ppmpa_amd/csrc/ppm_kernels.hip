@@ -60,34 +60,36 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
 // product as in a single pipeline.
 __device__ __forceinline__ void k_pass_begin_body(PassState* st, uint32_t first_pass, double r2_first, double alpha,
                                                   double power, uint32_t stride, Blk blk) {
-  if (threadIdx.x != 0 || blk.b != 0) return;
-  PassState s = *st;
-  if (s.pass == 0) {
-    s.pass = first_pass;
-    s.alpha = alpha;
-    s.r2_first = r2_first;
-    s.power = power;
-    double r2 = r2_first;
+  if (blk.b != 0) return;
+  if (threadIdx.x < 16) st->work[threadIdx.x >> 3][threadIdx.x & 7u][0] = 0;
+  if (threadIdx.x != 0) return;
+  uint32_t pass = st->pass;
+  double a = st->alpha, r2 = st->r2, pw = st->power;
+  if (pass == 0) {
+    pass = first_pass;
+    a = alpha;
+    pw = power;
+    st->alpha = alpha;
+    st->r2_first = r2_first;
+    st->power = power;
+    r2 = r2_first;
     for (uint32_t k = 1; k < first_pass; ++k) r2 *= ((double)k + alpha) / ((double)k + 1.0);
-    s.r2 = r2;
   } else {
     for (uint32_t j = 0; j < stride; ++j) {
-      s.pass += 1;
-      double k = (double)(s.pass - 1);
-      s.r2 *= (k + s.alpha) / (k + 1.0);
+      pass += 1;
+      double k = (double)(pass - 1);
+      r2 *= (k + a) / (k + 1.0);
     }
   }
-  s.r2f = (float)s.r2;
-  s.scale = (float)(s.power / (3.14159265358979323846 * s.r2));
-  s.n_stored = 0;
-  s.next_photon = 0;
-  s.n_queries = 0;
-  s.direct_next = 0;
-  s.shade_next = 0;
-  s.eye_next = 0;
-  s.tiles_front = 0;
-  s.tiles_back = 0;
-  *st = s;
+  st->pass = pass;
+  st->r2 = r2;
+  st->r2f = (float)r2;
+  st->scale = (float)(pw / (3.14159265358979323846 * r2));
+  st->n_stored = 0;
+  st->next_photon = 0;
+  st->n_queries = 0;
+  st->tiles_front = 0;
+  st->tiles_back = 0;
 }
 
 __global__ void k_pass_begin(PassState* st, uint32_t first_pass, double r2_first, double alpha, double power,
@@ -879,15 +881,30 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 
 // Next work item (query group, tile) for this wave from a pass counter, one
 // atomic per item: the persistent kernels take items until they run out, so
-// costly ones (dense caustics, glass) do not leave other waves idle.  Wave w
-// of W takes item w first without the counter and then W + the counter's
-// values: atomics on one address are serialised, and all W waves asking at
-// the start held the last ones up (caustic shade 0.92 -> 0.79 ms, eye 0.51
-// -> 0.45 ms).
+// costly ones (dense caustics, glass) do not leave other waves idle.
 __device__ __forceinline__ uint32_t wave_next(uint32_t* counter) {
   uint32_t g = 0;
   if (lane_id() == 0) g = atomicAdd(counter, 1u);
   return __builtin_amdgcn_readfirstlane(g);
+}
+
+// Next of n_items work items for a wave from the 8 counters ctr[c][0] of a
+// pass (counter c hands out items c, c + 8, c + 16, ...): the wave starts on
+// counter `c` (its index mod 8) and moves to the next one when that runs
+// out; n_items when all are out.  Atomics on one address are serialised, and
+// with one counter the start of a persistent kernel, every wave asking at
+// once, held the last waves up; eight counters on their own lines spread
+// that out while items are still taken in about the counters' joint order
+// (costly ones first).
+__device__ __forceinline__ uint32_t wave_next8(uint32_t (*ctr)[32], uint32_t& c, uint32_t& tries, uint32_t n_items) {
+  while (tries < 8u) {
+    const uint32_t k = wave_next(&ctr[c][0]);
+    const uint32_t i = c + 8u * k;
+    if (i < n_items) return i;
+    c = (c + 1u) & 7u;
+    ++tries;
+  }
+  return n_items;
 }
 
 // ===========================================================================
@@ -1340,7 +1357,9 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
   const uint32_t cap = 64u * eb.estack_depth;  // pool entries of this wave
   float4* const pool = eb.estack + (uint64_t)w * cap * 3u;
   const uint32_t tx = (uint32_t)sc.cam.tiles_x;
-  for (uint32_t k = w; k < eb.n_tiles; k = n_waves + wave_next(&st->eye_next)) {
+  uint32_t wc = w & 7u, wtries = 0;
+  for (uint32_t k = wave_next8(st->work[WORK_EYE], wc, wtries, eb.n_tiles); k < eb.n_tiles;
+       k = wave_next8(st->work[WORK_EYE], wc, wtries, eb.n_tiles)) {
     const uint32_t tile = eb.tile_order[k];
     const uint32_t x0 = (tile % tx) * 8u, y0 = (tile / tx) * 8u;
     const uint32_t t = tile * 64u + lane;
@@ -1716,9 +1735,9 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
   const GridView g = grid_view(st, sc.bmin);
   const uint32_t groups = (nq + 63) / 64;
   bool complete = true;
-  const uint32_t nw = blk.n * (blockDim.x / 64);
-  for (uint32_t gi = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6); gi < groups;
-       gi = nw + wave_next(&st->shade_next)) {
+  uint32_t wc = (blk.b * (blockDim.x / 64) + (threadIdx.x >> 6)) & 7u, wtries = 0;
+  for (uint32_t gi = wave_next8(st->work[WORK_SHADE], wc, wtries, groups); gi < groups;
+       gi = wave_next8(st->work[WORK_SHADE], wc, wtries, groups)) {
     const uint32_t i = gi * 64 + lane_id();
     if (i >= nq) continue;
     const uint32_t slot = order[i];

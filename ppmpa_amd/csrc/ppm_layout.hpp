@@ -132,7 +132,7 @@ struct alignas(16) Query {
 constexpr uint32_t NO_PIXEL = 0xFFFFFFFFu;
 
 // --- per-pass device state -----------------------------------------------------
-struct alignas(16) PassState {
+struct alignas(128) PassState {
   double r2;          // r_i^2 of the current pass
   double alpha;
   double r2_first;    // r_1^2
@@ -158,15 +158,16 @@ struct alignas(16) PassState {
   float g_reach;
   uint32_t g_mask;
   int32_t g_lim;
-  // work counters of the persistent kernels (zeroed by k_pass_begin): next
-  // query group of k_shade, next tile of k_eye, and the two ends of the
-  // eye's tile order (costly tiles first)
-  uint32_t direct_next;  // unused
-  uint32_t shade_next;
-  uint32_t eye_next;
+  // the two ends of the eye's tile order (costly tiles first; zeroed by
+  // k_pass_begin)
   uint32_t tiles_front;
   uint32_t tiles_back;
-  uint32_t pad1[3];
+  // work counters of the persistent kernels (zeroed by k_pass_begin):
+  // work[k][c][0] hands out items c, c + 8, c + 16, ... of kernel k (0 the
+  // query groups of k_shade, 1 the tiles of k_eye); each counter has its own
+  // 128-byte line
+  alignas(128) uint32_t work[2][8][32];
 };
+constexpr uint32_t WORK_SHADE = 0, WORK_EYE = 1;
 
 }  // namespace ppm
