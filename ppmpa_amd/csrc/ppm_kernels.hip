@@ -1898,7 +1898,9 @@ __global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const uint2* c
     if (want) {
       const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
       complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
-                             [&](uint32_t j, float4 pm, float4 dm, float d2) {
+                             // the channel factors by value: a conditional over referenced
+                             // floats selects an address, which puts the closure in scratch
+                             [&, fr, fg, fb](uint32_t j, float4 pm, float4 dm, float d2) {
                                const float c = nx * dm.x + ny * dm.y + nz * dm.z;
                                if (c < 0.0f) {
                                  const uint32_t wl = __float_as_uint(pm.w) & 3u;
