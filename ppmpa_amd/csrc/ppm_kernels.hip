@@ -879,7 +879,7 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
   return hipGetLastError();
 }
 
-// Next work item (query group, tile) for this wave from a pass counter, one
+// Next work item (query group, tile) for this wave from a counter, one
 // atomic per item: the persistent kernels take items until they run out, so
 // costly ones (dense caustics, glass) do not leave other waves idle.
 __device__ __forceinline__ uint32_t wave_next(uint32_t* counter) {
@@ -1325,7 +1325,7 @@ __device__ __forceinline__ uint64_t peer_mask(uint32_t key, bool pred) {
 }
 
 // Persistent eye kernel.  A wave takes tiles in the order of tile_order from
-// the pass's counter and walks the eye trees of a tile's 64 pixels (one per
+// the pass's work counters and walks the eye trees of a tile's 64 pixels (one per
 // lane to start with) together: each loop step processes one node per busy
 // lane; a node with both a reflection and a refraction child continues with
 // the reflection and puts the refraction child on the wave's pending pool
@@ -1719,7 +1719,7 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 
 // Shading of the queries in Morton order, one query per lane, 64 spatially
 // adjacent queries per wave (their photons mostly share cache lines); waves
-// take groups of 64 from the pass counter.  Each lane gathers over the cells
+// take groups of 64 from the pass's work counters.  Each lane gathers over the cells
 // of the pass's photon grid that its ball meets:
 //   res[slot] = w * (scale * sum filter * max(0,-n.dir) + direct light (uc))
 // per channel: the di of tracer.rs:136-145 weighted by the node's diffuse
