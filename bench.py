@@ -134,8 +134,11 @@ def main():
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
 
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.workload, args.cpu_sample_passes)
+        orc = {}
+        out["cpu_baseline"] = cpu_baseline(sc, cfgd, args.workload, args.cpu_sample_passes, keep=orc)
         out["speedup_vs_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        if orc:
+            out["oracle_agreement"] = oracle_agreement(P, ctx, ds, cfgd, orc)
     if args.workload == "caustic" and not args.no_3sigma and not args.no_cpu_baseline:
         out["time_to_3sigma"] = time_to_3sigma(P, ctx)
     print(json.dumps(out), flush=True)
@@ -359,7 +362,7 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(sc, cfgd, workload, n_sample):
+def cpu_baseline(sc, cfgd, workload, n_sample, keep=None):
     """The FP64 oracle (a restatement of the reference's CPU path, which
     tests every shape for every ray) on this host's cores, on a bounded
     sample of the same workload."""
@@ -381,12 +384,36 @@ def cpu_baseline(sc, cfgd, workload, n_sample):
                           f"{sc.n_shapes} shapes per ray, {T:.1f} s; CPU {cpu_model()}"}
     n_sample = min(n_sample, cfgd["passes"])
     t0 = time.perf_counter()
-    _, _, _, ms_trace, ms_eye = osc.run_passes(c, n_sample, threads=threads)
+    mean, var, _, ms_trace, ms_eye = osc.run_passes(c, n_sample, threads=threads)
     T = time.perf_counter() - t0
+    if keep is not None:
+        keep.update(mean=mean, var=var, passes=n_sample)
     return {"value": round(cfgd["n_photon"] * n_sample / T / 1e6, 4), "unit": "Mphotons/s", "cores": threads,
             "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle, {T:.1f} s; "
                                       f"trace+kd {ms_trace / 1e3:.1f} s, eye+gather {ms_eye / 1e3:.1f} s; "
                                       f"CPU {cpu_model()}"}
+
+
+def oracle_agreement(P, ctx, ds, cfgd, orc):
+    """The GPU's image of the same passes as the oracle's sample, on an
+    independent random stream (seed + 1000): the fraction of pixels whose
+    means agree within 3 sigma of the per-pass spread of both sides (equal
+    error at equal pass count, at this workload's own size)."""
+    import numpy as np
+    K = orc["passes"]
+    cfg = P.pass_cfg(seed=cfgd["seed"] + 1000, n_photon=cfgd["n_photon"], radius0=cfgd["radius0"],
+                     alpha=cfgd["alpha"], max_depth=cfgd["max_depth"], first_pass=1)
+    s = P.Session(ctx, ds, cfg)
+    s.run(K)
+    mg, vg, n = s.image()
+    s.close()
+    mo, vo = orc["mean"], orc["var"]
+    se = np.sqrt(vg / K + vo / K)
+    ok = se > 0
+    frac = float((np.abs(mg - mo)[ok] / se[ok] <= 3).mean())
+    rel = float(np.abs(mg.mean() - mo.mean()) / mo.mean())
+    return {"passes": K, "frac_pixels_within_3sigma": round(frac, 4), "pixels_compared": int(ok.sum()),
+            "image_mean_rel_diff": round(rel, 6)}
 
 
 def _blocks(img, b=4):
