@@ -107,6 +107,10 @@ def main():
     if traffic is not None:
         roof["traffic"] = traffic
         roof["traffic_source"] = src + " (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy probe)"
+    issue, isrc = profiled_issue(kern, args.workload)
+    if issue is not None:
+        roof["valu_busy"], roof["salu_busy"] = issue
+        roof["issue_source"] = isrc + " (rocprofv3 SQ_INSTS_VALU / SQ_INSTS_SALU per dispatch over GRBM_GUI_ACTIVE)"
 
     out = {
         "metric": "Mphotons/s traced+gathered",
@@ -339,6 +343,20 @@ def profiled_traffic(kernel, workload):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
         if len(cells) >= 5 and cells[0] == kernel:  # kernel | dispatches | FETCH KiB | WRITE KiB | MB
             return round((float(cells[2]) * sf + float(cells[3]) * sw) * 1024), rel
+    return None, None
+
+
+def profiled_issue(kernel, workload):
+    """(valu_busy, salu_busy) of `kernel` from the committed issue-rate
+    summary of this workload; (None, None) when absent."""
+    rel = os.path.join("profiles", f"pmc_issue_{workload}.md")
+    path = os.path.join(ROOT, rel)
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 9 and cells[0] == kernel:
+            return (float(cells[6]), float(cells[7])), rel
     return None, None
 
 
