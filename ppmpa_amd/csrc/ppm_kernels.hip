@@ -1103,6 +1103,32 @@ __device__ __forceinline__ void grid_range_scan(const PhotonRec* ph, uint2 se, f
   }
 }
 
+// The same scan calling on_photon(pm, dm, d2, inside) for every photon of
+// the range, inside = (d2 <= r2): for a callback whose work is a select
+// (the shade's sum adds 0 for photons outside), with no branch per photon
+// and so none of the exec-mask work that kept the scalar units half busy.
+template <typename OnPhoton>
+__device__ __forceinline__ void grid_range_scan_all(const PhotonRec* ph, uint2 se, float qx, float qy, float qz,
+                                                    float r2, OnPhoton& on_photon) {
+  uint32_t j = se.x;
+  for (; j + 3 < se.y; j += 4) {
+    const PhotonRec p0 = ph[j], p1 = ph[j + 1], p2 = ph[j + 2], p3 = ph[j + 3];
+    const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
+    const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
+    const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
+    const float d3 = dist2_nc(qx, qy, qz, p3.pm.x, p3.pm.y, p3.pm.z);
+    on_photon(p0.pm, p0.dm, d0, d0 <= r2);
+    on_photon(p1.pm, p1.dm, d1, d1 <= r2);
+    on_photon(p2.pm, p2.dm, d2, d2 <= r2);
+    on_photon(p3.pm, p3.dm, d3, d3 <= r2);
+  }
+  for (; j < se.y; ++j) {
+    const PhotonRec p0 = ph[j];
+    const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
+    on_photon(p0.pm, p0.dm, d0, d0 <= r2);
+  }
+}
+
 template <typename OnPhoton>
 __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const PhotonRec* ph, uint32_t b, float qx,
                                                  float qy, float qz, float r2, OnPhoton& on_photon) {
@@ -1112,7 +1138,9 @@ __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const Photo
 // Returns false (and scans only the first three cells of an axis) if the
 // query's cells span more than three per axis, which the cell side rules out
 // for every finite point (the caller raises a flag).
-template <typename OnPhoton>
+// ALL = true: on_photon(pm, dm, d2, inside) for every photon of the cells
+// (grid_range_scan_all), in the same order.
+template <bool ALL = false, typename OnPhoton>
 __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells, const PhotonRec* ph, float qx,
                                            float qy, float qz, float r2, OnPhoton&& on_photon) {
   const float rx = g.reach + GRID_REL_MARGIN * fabsf(qx);
@@ -1139,8 +1167,13 @@ __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells
 #pragma unroll
     for (int k = 0; k < 8; ++k) se[k] = v[k] ? cells[b[k]] : make_uint2(0u, 0u);
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (v[k]) grid_range_scan(ph, se[k], qx, qy, qz, r2, on_photon);
+    for (int k = 0; k < 8; ++k) {
+      if constexpr (ALL) {
+        grid_range_scan_all(ph, se[k], qx, qy, qz, r2, on_photon);  // an unused cell's range is empty
+      } else {
+        if (v[k]) grid_range_scan(ph, se[k], qx, qy, qz, r2, on_photon);
+      }
+    }
     return true;
   }
   // three cells on some axis (float rounding at the edge of the bound): the
@@ -1158,7 +1191,12 @@ __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells
               const bool earlier = zz < z || (zz == z && (yy < y || (yy == y && xx < x)));
               dup = dup || (earlier && grid_bucket(xx, yy, zz, g.mask) == b);
             }
-        if (!dup) grid_bucket_scan(cells, ph, b, qx, qy, qz, r2, on_photon);
+        if (!dup) {
+          if constexpr (ALL)
+            grid_range_scan_all(ph, cells[b], qx, qy, qz, r2, on_photon);
+          else
+            grid_bucket_scan(cells, ph, b, qx, qy, qz, r2, on_photon);
+        }
       }
   return ok;
 }
@@ -1744,10 +1782,10 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
     const Query qq = eb.q[slot];
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
     const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
-    complete &= grid_query(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
-                           [&](uint32_t, float4 pm, float4 dm, float d2) {
+    complete &= grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
+                                 [&](float4 pm, float4 dm, float d2, bool in) {
                              const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
-                             const float v = c < 0.0f ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
+                             const float v = (in && c < 0.0f) ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
                              const uint32_t wl = __float_as_uint(pm.w) & 3u;
                              ar += wl == 0 ? v : 0.0f;
                              ag += wl == 1 ? v : 0.0f;
