@@ -735,8 +735,8 @@ struct LightSelf {
 };
 
 template <bool BVH>
-__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, const LightSelf& ls,
-                                           float& cos0sq, float& sqd) {
+__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, uint64_t wunion,
+                                           const LightSelf& ls, float& cos0sq, float& sqd) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
@@ -747,10 +747,15 @@ __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 l
     any = true;
     cand &= ~(1ull << ls.self);
   }
-  for (uint32_t i = 0; i < sc.n_lin; ++i) {
-    if (i < CAND_BITS && ((cand >> i) & 1ull) == 0) continue;
+  // the shapes some lane of the wave may need (wunion, wave-uniform), each
+  // tested by the lanes whose candidates hold it
+  for (uint64_t m = wunion; m != 0; m &= m - 1) {
+    const uint32_t i = (uint32_t)__builtin_ctzll(m);
+    if (((cand >> i) & 1ull) == 0) continue;
     if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
   }
+  for (uint32_t i = CAND_BITS; i < sc.n_lin; ++i)
+    if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
   if (BVH && bvh_shadow(sc, p, ld3, sq_l, any)) return false;
   // no occluder; lit iff the ray meets something at all (calc_intersection
   // returns None otherwise, tracer.rs:284)
@@ -805,12 +810,19 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
     ls.self = __float_as_uint(L.pos.w);
     ls.nq = cross(mk(L.d1), mk(L.d2));
     ls.nqn = sqrtf(dot(ls.nq, ls.nq));
+    // the union of the candidates of the wave's active lanes (one ballot per
+    // shape): the 25 samples then loop over these shapes only, instead of
+    // every shape with a per-lane test (caustic shade 0.75 -> 0.64 ms)
+    uint64_t wunion = 0;
+    const uint32_t nb = min(sc.n_lin, CAND_BITS);
+    for (uint32_t i = 0; i < nb; ++i)
+      if (__ballot(((cand >> i) & 1ull) != 0)) wunion |= 1ull << i;
     C3 prev{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < 25; ++s) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BVH>(sc, p, n, ld, cand, ls, csq, sqd)) continue;
+      if (!sample_lit<BVH>(sc, p, n, ld, cand, wunion, ls, csq, sqd)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }
