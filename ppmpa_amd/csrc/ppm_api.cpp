@@ -510,7 +510,8 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
     return PPM_OK;
   }
   const uint64_t n = in_bvh.size();
-  const uint64_t leaves = (n + BVH_LEAF - 1) / BVH_LEAF;
+  const uint32_t per_leaf = n <= BVH_SMALL_PRIMS ? BVH_LEAF_SMALL : 1u;
+  const uint64_t leaves = (n + per_leaf - 1) / per_leaf;
   uint32_t depth = 0;
   while ((1ull << depth) < leaves) ++depth;
   if (depth > BVH_MAX_DEPTH) return fail(c, PPM_E_RANGE, "too many shapes for the BVH (at most 16M)");
@@ -541,8 +542,8 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
       (e = hipMemcpy(np, &n32, sizeof(n32), hipMemcpyHostToDevice)) == hipSuccess &&
       (lin.empty() ||
        (e = hipMemcpy(b.lin, lin.data(), lin.size() * sizeof(DevShape), hipMemcpyHostToDevice)) == hipSuccess) &&
-      (e = launch_bvh_build(c->stream, b.shapes, ids, n32, fmin, fscale, depth, keys, vals, np, ghist, totals,
-                            sort_blocks, pbox, b.bprims, b.bvh)) == hipSuccess)
+      (e = launch_bvh_build(c->stream, b.shapes, ids, n32, fmin, fscale, depth, per_leaf, keys, vals, np, ghist,
+                            totals, sort_blocks, pbox, b.bprims, b.bvh)) == hipSuccess)
     e = hipStreamSynchronize(c->stream);
   dfree(ids);
   dfree(keys[0]);
@@ -557,6 +558,7 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
   b.n_lin = (uint32_t)lin.size();
   b.n_bprims = n32;
   b.bvh_depth = depth;
+  b.bvh_leaf = per_leaf;
   sc->lin_host = lin;
   return PPM_OK;
 }

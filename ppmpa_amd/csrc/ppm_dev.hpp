@@ -134,6 +134,7 @@ struct SceneView {
   uint32_t n_lin;
   uint32_t n_bprims;
   uint32_t bvh_depth;       // leaves at level bvh_depth
+  uint32_t bvh_leaf;        // primitives per leaf
   uint32_t n_shapes;
   uint32_t n_mats;          // index of M_AIR
   uint32_t n_lights;
@@ -356,8 +357,8 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
   if (!slab(bvh_node(sc, 0), o, inv, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
-      const uint32_t base = (node - first_leaf) * BVH_LEAF;
-      const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
+      const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
+      const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
       for (uint32_t j = base; j < end; ++j) bprim_hit(sc, j, o, d, h);
     } else {
       const uint32_t l = 2u * node + 1u;
@@ -647,8 +648,8 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
   if (!slab(bvh_node(sc, 0), o, inv, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
-      const uint32_t base = (node - first_leaf) * BVH_LEAF;
-      const uint32_t end = min(base + BVH_LEAF, sc.n_bprims);
+      const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
+      const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
       for (uint32_t j = base; j < end; ++j)
         if (bprim_occludes(sc, j, o, d, sq_l, any)) return true;
     } else {

@@ -40,6 +40,7 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
   v.n_lin = b.n_lin;
   v.n_bprims = b.n_bprims;
   v.bvh_depth = b.bvh_depth;
+  v.bvh_leaf = b.bvh_leaf;
   v.n_shapes = b.n_shapes;
   v.n_mats = b.n_mats;
   v.n_lights = b.n_lights;
@@ -1206,7 +1207,7 @@ __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells
 // object for every ray (calc_intersection, tracer.rs:306-335); here the
 // bounded shapes (spheres, polygons, parallelograms) smaller than a quarter
 // of the scene sit in a heap-ordered
-// complete binary tree over leaves of BVH_LEAF shapes in Morton order of
+// complete binary tree over leaves of bvh_leaf shapes in Morton order of
 // their centroids, the same implicit layout as the photon map: node i has
 // children 2i+1 and 2i+2 and needs no pointers.  Planes and the large shapes
 // (walls, whose boxes would widen every node above them) stay in the linear
@@ -1270,13 +1271,13 @@ __global__ __launch_bounds__(256) void k_bvh_keys(const DevShape* shapes, const 
 // empty node).
 __global__ __launch_bounds__(256) void k_bvh_leaves(const DevShape* shapes, const uint32_t* ids,
                                                     const uint32_t* order, const Node* pbox, uint32_t n,
-                                                    uint32_t depth, DevShape* bprims, Node* nodes) {
+                                                    uint32_t depth, uint32_t per_leaf, DevShape* bprims, Node* nodes) {
   const uint32_t leaf = blockIdx.x * blockDim.x + threadIdx.x;
   if (leaf >= (1u << depth)) return;
-  const uint32_t base = leaf * BVH_LEAF;
+  const uint32_t base = leaf * per_leaf;
   Node u = empty_node();
   bool any = false;
-  for (uint32_t j = 0; j < BVH_LEAF; ++j) {
+  for (uint32_t j = 0; j < per_leaf; ++j) {
     const uint32_t k = base + j;
     if (k >= n) break;
     const uint32_t pi = order[k];
@@ -1299,7 +1300,8 @@ __global__ __launch_bounds__(256) void k_bvh_level(Node* nodes, uint32_t level) 
 }
 
 hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_t* ids, uint32_t n, float4 cmin,
-                            float4 cscale, uint32_t depth, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np,
+                            float4 cscale, uint32_t depth, uint32_t leaf, uint32_t* keys[2], uint32_t* vals[2],
+                            const uint32_t* np,
                             uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,
                             Node* nodes) {
   hipLaunchKernelGGL(k_bvh_keys, dim3((n + 255) / 256), dim3(256), 0, s, shapes, ids, n, cmin, cscale, keys[0],
@@ -1311,7 +1313,7 @@ hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_
   if (e != hipSuccess) return e;
   const uint32_t slots = 1u << depth;
   hipLaunchKernelGGL(k_bvh_leaves, dim3((slots + 255) / 256), dim3(256), 0, s, shapes, ids, vals[cur], pbox, n,
-                     depth, bprims, nodes);
+                     depth, leaf, bprims, nodes);
   for (int level = (int)depth - 1; level >= 0; --level)
     hipLaunchKernelGGL(k_bvh_level, dim3(((1u << level) + 255) / 256), dim3(256), 0, s, nodes, (uint32_t)level);
   return hipGetLastError();

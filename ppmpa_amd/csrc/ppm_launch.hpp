@@ -19,6 +19,7 @@ struct SceneBuf {
   Node* bvh = nullptr;          // 2^(bvh_depth+1) - 1 nodes, or null
   DevShape* bprims = nullptr;   // n_bprims shapes in leaf order
   uint32_t n_lin = 0, n_bprims = 0, bvh_depth = 0;
+  uint32_t bvh_leaf = 1;        // primitives per BVH leaf
   uint32_t n_shapes = 0, n_mats = 0, n_lights = 0;
   uint32_t n_emit_total = 0;  // photons per pass over all lights
   DevCamera cam{};
@@ -158,9 +159,10 @@ hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const flo
 
 // Scene BVH over the n shapes shapes[ids[i]]: Morton keys of the centroids
 // (cmin, cscale = 1023 / extent), a radix sort (count *np on the device),
-// leaves of BVH_LEAF shapes copied into bprims, levels folded up.
+// leaves of `leaf` shapes copied into bprims, levels folded up.
 hipError_t launch_bvh_build(hipStream_t s, const DevShape* shapes, const uint32_t* ids, uint32_t n, float4 cmin,
-                            float4 cscale, uint32_t depth, uint32_t* keys[2], uint32_t* vals[2], const uint32_t* np,
+                            float4 cscale, uint32_t depth, uint32_t leaf, uint32_t* keys[2], uint32_t* vals[2],
+                            const uint32_t* np,
                             uint32_t* ghist, uint32_t* totals, uint32_t sort_blocks, Node* pbox, DevShape* bprims,
                             Node* nodes);
 

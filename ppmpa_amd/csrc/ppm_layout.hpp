@@ -46,12 +46,16 @@ constexpr int MAX_DEPTH_LIMIT = 16;  // deepest MAX_TRACE the kernels accept
 #endif
 constexpr int LEAF = PPM_LEAF;       // photons per map leaf (<= one wavefront)
 
-// Scene BVH: a heap-ordered complete binary tree over leaves of BVH_LEAF
-// primitives in Morton order of their centroids.  Traversal defers at most
-// one node per level, kept as one bit per level of a 32-bit trail, so
-// BVH_MAX_DEPTH < 32.
-constexpr uint32_t BVH_LEAF = 4;
-constexpr uint32_t BVH_MAX_DEPTH = 22;   // up to 4M leaves (16M primitives)
+// Scene BVH: a heap-ordered complete binary tree over leaves of a few
+// primitives in Morton order of their centroids: BVH_LEAF_SMALL for a tree
+// of at most BVH_SMALL_PRIMS primitives (which then fits the kernels' LDS
+// copy whole), one otherwise (the mesh of 327,680 triangles: 105 ms per pass
+// with one, 111 with two, 118 with four, 139 with eight primitives per
+// leaf).  Traversal defers at most one node per level, kept as one bit per
+// level of a 32-bit trail, so BVH_MAX_DEPTH < 32.
+constexpr uint32_t BVH_LEAF_SMALL = 4;
+constexpr uint32_t BVH_SMALL_PRIMS = 512;
+constexpr uint32_t BVH_MAX_DEPTH = 24;   // up to 16M leaves
 constexpr uint32_t BVH_MIN_PRIMS = 64;   // automatic mode: fewer bounded shapes are tested one by one
 constexpr float BVH_EMPTY = 3.0e38f;     // an empty node is the point box (BVH_EMPTY,)^3
 
