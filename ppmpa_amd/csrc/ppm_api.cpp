@@ -1636,6 +1636,7 @@ struct ppm_batch {
   uint32_t* tab = nullptr;         // device, BK_KINDS x (n + 1)
   std::vector<uint32_t> htab;
   bool bvh = false;
+  uint32_t dyn = 0;                // dynamic LDS of the BVH kernels: the most any scene asks (bvh_dyn_lds)
   ppm_launch_cfg lc{};
   std::vector<hipGraphExec_t> exec;  // per pipeline j and number of running scenes a: j * n + a - 1
   std::vector<hipGraph_t> graph;
@@ -1754,7 +1755,7 @@ hipError_t batch_round(ppm_batch* b, uint32_t act, uint32_t q) {
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       const hipError_t e2 =
-          launch_batch_pass(cap, cap2, fork, join, b->items + (size_t)q * n, b->tab, blocks, act, n, b->bvh);
+          launch_batch_pass(cap, cap2, fork, join, b->items + (size_t)q * n, b->tab, blocks, act, n, b->bvh, b->dyn);
       e = hipStreamEndCapture(cap, &b->graph[gi]);
       if (e2 != hipSuccess) e = e2;
     }
@@ -1832,6 +1833,7 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
     }
     b->sess.push_back(s);
     if (scenes[i]->buf.bvh) b->bvh = true;
+    b->dyn = std::max(b->dyn, bvh_dyn_lds(scenes[i]->buf));
   }
   b->order.resize(n);
   for (uint32_t i = 0; i < n; ++i) b->order[i] = i;

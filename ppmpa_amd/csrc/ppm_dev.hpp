@@ -140,6 +140,7 @@ struct SceneView {
   uint32_t n_lights;
   uint32_t n_emit;           // photons per pass over all lights
   uint32_t lds;              // in LDS (scene_to_lds): bit 0 lin and lin_box, 1 mats, 2 bvh, 3 compact bprims
+  uint32_t dyn_lds;          // the launch gave BVH_DYN_LDS_BYTES of dynamic LDS for the BVH copies
   DevCamera cam;
   float4 bmin, bmax;        // Morton quantisation box
 };
@@ -167,13 +168,20 @@ __device__ __forceinline__ void copy16(float4* dst, const float4* src, uint32_t 
 // most BVH_LDS_NODES nodes, and its primitives, up to BVH_LDS_PRIMS, as
 // compact spheres (centre, radius^2; w = -1 for a primitive of another kind,
 // which is then read from global memory) with their scene indices.
+// They live in dynamic LDS, which a launch requests only for a scene whose
+// tree is that small (bvh_dyn_lds on the host): a kernel walking a large
+// tree (the mesh) then holds no 18 KB of unused LDS per block, which had
+// cost it occupancy (mesh 105.8 -> 98.6 ms per pass without it).
 constexpr uint32_t BVH_LDS_NODES = 255;  // leaf level <= 7: 128 leaves
 constexpr uint32_t BVH_LDS_PRIMS = 512;
-__shared__ Node g_bvh_lds[BVH_LDS_NODES];
-__shared__ float4 g_bsph_lds[BVH_LDS_PRIMS];
-__shared__ int32_t g_bidx_lds[BVH_LDS_PRIMS];
+extern __shared__ float4 g_dyn_lds[];
+__device__ __forceinline__ Node* bvh_lds_nodes() { return reinterpret_cast<Node*>(g_dyn_lds); }
+__device__ __forceinline__ float4* bvh_lds_sph() { return g_dyn_lds + 2u * BVH_LDS_NODES; }
+__device__ __forceinline__ int32_t* bvh_lds_idx() {
+  return reinterpret_cast<int32_t*>(g_dyn_lds + 2u * BVH_LDS_NODES + BVH_LDS_PRIMS);
+}
 
-template <bool BVH = false>
+template <int BV = 0, bool BVH = (BV != 0)>
 __device__ __forceinline__ void scene_to_lds(SceneView& sc) {
   const bool lin = sc.n_lin <= LIN_LDS && sc.lin_box != nullptr;
   const bool mat = sc.n_mats + 1u <= MAT_LDS;
@@ -183,17 +191,17 @@ __device__ __forceinline__ void scene_to_lds(SceneView& sc) {
   }
   if (mat) copy16((float4*)g_mat_lds, (const float4*)sc.mats, (sc.n_mats + 1u) * (uint32_t)(sizeof(DevMaterial) / 16));
   bool nodes = false, prims = false;
-  if (BVH && sc.n_bprims > 0) {
+  if (BV == 2 && sc.dyn_lds && sc.n_bprims > 0) {
     const uint32_t n_nodes = (2u << sc.bvh_depth) - 1u;
     nodes = n_nodes <= BVH_LDS_NODES;
     prims = sc.n_bprims <= BVH_LDS_PRIMS;
-    if (nodes) copy16((float4*)g_bvh_lds, (const float4*)sc.bvh, n_nodes * (uint32_t)(sizeof(Node) / 16));
+    if (nodes) copy16((float4*)bvh_lds_nodes(), (const float4*)sc.bvh, n_nodes * (uint32_t)(sizeof(Node) / 16));
     if (prims)
       for (uint32_t j = threadIdx.x; j < sc.n_bprims; j += blockDim.x) {
         const DevShape s = sc.bprims[j];
-        g_bsph_lds[j] = s.kind == SHAPE_SPHERE ? make_float4(s.p.x, s.p.y, s.p.z, s.radius * s.radius)
-                                               : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        g_bidx_lds[j] = (int32_t)__float_as_uint(s.p.w);
+        bvh_lds_sph()[j] = s.kind == SHAPE_SPHERE ? make_float4(s.p.x, s.p.y, s.p.z, s.radius * s.radius)
+                                                  : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        bvh_lds_idx()[j] = (int32_t)__float_as_uint(s.p.w);
       }
   }
   __syncthreads();
@@ -313,15 +321,17 @@ __device__ __forceinline__ bool slab(const Node& b, V3 o, V3 inv, float tmax, fl
 
 // BVH node i and the nearest-hit test of primitive j: from LDS when
 // scene_to_lds put the tree there, else from global memory.
+template <bool CACHED>
 __device__ __forceinline__ Node bvh_node(const SceneView& sc, uint32_t i) {
-  if (sc.lds & 4u) return g_bvh_lds[i];
+  if (CACHED && (sc.lds & 4u)) return bvh_lds_nodes()[i];
   return sc.bvh[i];
 }
+template <bool CACHED>
 __device__ __forceinline__ void bprim_hit(const SceneView& sc, uint32_t j, V3 o, V3 d, Hit& h) {
-  if (sc.lds & 8u) {
-    const float4 c = g_bsph_lds[j];
+  if (CACHED && (sc.lds & 8u)) {
+    const float4 c = bvh_lds_sph()[j];
     if (c.w >= 0.0f) {
-      sphere_hit(mk(c.x, c.y, c.z), c.w, g_bidx_lds[j], o, d, h);
+      sphere_hit(mk(c.x, c.y, c.z), c.w, bvh_lds_idx()[j], o, d, h);
       return;
     }
   }
@@ -348,23 +358,24 @@ __device__ __forceinline__ uint32_t trail_pop(uint32_t& trail, uint32_t node) {
 // nearer child first, the farther one deferred on the bit trail).  A node is
 // entered while its box starts no later than the best hit, so equal
 // distances still reach the lower shape index.
+template <bool CACHED>
 __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
   if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
   const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node(sc, 0), o, inv, h.t, tn)) return;
+  if (!slab(bvh_node<CACHED>(sc, 0), o, inv, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
       const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
-      for (uint32_t j = base; j < end; ++j) bprim_hit(sc, j, o, d, h);
+      for (uint32_t j = base; j < end; ++j) bprim_hit<CACHED>(sc, j, o, d, h);
     } else {
       const uint32_t l = 2u * node + 1u;
       float tl, tr;
-      const bool hl = slab(bvh_node(sc, l), o, inv, h.t, tl);
-      const bool hr = slab(bvh_node(sc, l + 1u), o, inv, h.t, tr);
+      const bool hl = slab(bvh_node<CACHED>(sc, l), o, inv, h.t, tl);
+      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), o, inv, h.t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -379,7 +390,7 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node(sc, node), o, inv, h.t, tn)) {
+      if (slab(bvh_node<CACHED>(sc, node), o, inv, h.t, tn)) {
         more = true;
         break;
       }
@@ -390,14 +401,14 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
 
 // calc_intersection (tracer.rs:306-335): the nearest hit over every shape.
 // BVH = false: the scene has no BVH and `lin` holds all shapes.
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ Hit nearest(const SceneView& sc, V3 o, V3 d) {
   Hit h{INF_F, -1};
   for (uint32_t i = 0; i < sc.n_lin; ++i) {
     const DevShape s = lin_shape(sc, i);
     shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
   }
-  if (BVH) bvh_nearest(sc, o, d, h);
+  if (BVH) bvh_nearest<BV == 2>(sc, o, d, h);
   return h;
 }
 
@@ -430,9 +441,9 @@ __device__ __forceinline__ bool finish_hit(const SceneView& sc, V3 o, V3 d, Hit 
   return true;
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ bool intersect(const SceneView& sc, V3 o, V3 d, Isect& is) {
-  return finish_hit(sc, o, d, nearest<BVH>(sc, o, d), is);
+  return finish_hit(sc, o, d, nearest<BV>(sc, o, d), is);
 }
 
 // ---------------------------------------------------------------------------
@@ -622,9 +633,10 @@ __device__ __forceinline__ bool occludes(const DevShape& s, V3 p, V3 d, float sq
   return occluder_hit(h, p, d, sq_l, any);
 }
 // occludes() for BVH primitive j (LDS compact sphere when there is one)
+template <bool CACHED>
 __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, V3 p, V3 d, float sq_l, bool& any) {
-  if (sc.lds & 8u) {
-    const float4 c = g_bsph_lds[j];
+  if (CACHED && (sc.lds & 8u)) {
+    const float4 c = bvh_lds_sph()[j];
     if (c.w >= 0.0f) {
       Hit h{INF_F, -1};
       sphere_hit(mk(c.x, c.y, c.z), c.w, 0, p, d, h);
@@ -638,6 +650,7 @@ __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, 
 // occluder; sets any when some primitive is hit.  Once a hit is known only
 // boxes entered before t_occ (beyond which no hit can be an occluder) are
 // searched.
+template <bool CACHED>
 __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
   if (sc.n_bprims == 0) return false;
   const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -645,19 +658,19 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node(sc, 0), o, inv, any ? t_occ : INF_F, tn)) return false;
+  if (!slab(bvh_node<CACHED>(sc, 0), o, inv, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
       const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
       for (uint32_t j = base; j < end; ++j)
-        if (bprim_occludes(sc, j, o, d, sq_l, any)) return true;
+        if (bprim_occludes<CACHED>(sc, j, o, d, sq_l, any)) return true;
     } else {
       const uint32_t l = 2u * node + 1u;
       const float tmax = any ? t_occ : INF_F;
       float tl, tr;
-      const bool hl = slab(bvh_node(sc, l), o, inv, tmax, tl);
-      const bool hr = slab(bvh_node(sc, l + 1u), o, inv, tmax, tr);
+      const bool hl = slab(bvh_node<CACHED>(sc, l), o, inv, tmax, tl);
+      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), o, inv, tmax, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -672,7 +685,7 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node(sc, node), o, inv, any ? t_occ : INF_F, tn)) {
+      if (slab(bvh_node<CACHED>(sc, node), o, inv, any ? t_occ : INF_F, tn)) {
         more = true;
         break;
       }
@@ -735,7 +748,7 @@ struct LightSelf {
   float nqn;
 };
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, uint64_t wunion,
                                            const LightSelf& ls, float& cos0sq, float& sqd) {
   V3 ld3;
@@ -757,7 +770,7 @@ __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 l
   }
   for (uint32_t i = CAND_BITS; i < sc.n_lin; ++i)
     if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
-  if (BVH && bvh_shadow(sc, p, ld3, sq_l, any)) return false;
+  if (BVH && bvh_shadow<BV == 2>(sc, p, ld3, sq_l, any)) return false;
   // no occluder; lit iff the ray meets something at all (calc_intersection
   // returns None otherwise, tracer.rs:284)
   for (uint32_t i = 0; i < min(sc.n_lin, CAND_BITS) && !any; ++i) {
@@ -799,7 +812,7 @@ __device__ __forceinline__ V3 tss_point(const DevLight& L, int s) {
   return mk(L.pos) + tss(s / 5) * mk(L.d1) + tss(s % 5) * mk(L.d2);
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
   C3 sum{0.0f, 0.0f, 0.0f};
   for (uint32_t li = 0; li < sc.n_lights; ++li) {
@@ -823,7 +836,7 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BVH>(sc, p, n, ld, cand, wunion, ls, csq, sqd)) continue;
+      if (!sample_lit<BV>(sc, p, n, ld, cand, wunion, ls, csq, sqd)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }

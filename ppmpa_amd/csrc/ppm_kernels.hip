@@ -46,6 +46,7 @@ __host__ __device__ inline SceneView make_view(const SceneBuf& b) {
   v.n_lights = b.n_lights;
   v.n_emit = b.n_emit_total;
   v.lds = 0;
+  v.dyn_lds = bvh_dyn_lds(b) != 0u ? 1u : 0u;
   v.cam = b.cam;
   v.bmin = b.bmin;
   v.bmax = b.bmax;
@@ -177,14 +178,14 @@ __device__ __forceinline__ void emit_photon(const SceneView& sc, uint32_t pid, R
 
 // One step of trace_photon.  Returns whether the photon lives on; `store`
 // reports a photon-map record (position, INCOMING direction, channel).
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ bool photon_bounce(const SceneView& sc, int uc, int maxd, PhotonState& p, Rng& rng,
                                               bool& store, V3& spos, int& ev_mat) {
   store = false;
   ev_mat = -1;
   if ((int)p.depth >= maxd) return false;  // tracer.rs:32
   Isect is;
-  if (!intersect<BVH>(sc, p.o, p.d, is)) return false;  // tracer.rs:36-39
+  if (!intersect<BV>(sc, p.o, p.d, is)) return false;  // tracer.rs:36-39
   const DevMaterial M = material(sc, is.mat);
   // tracer.rs:77-79
   store = (uc == 0 || p.depth > 0) && surf_store(M);
@@ -296,14 +297,14 @@ __device__ __forceinline__ void adj_flush(float* part, double* grad, uint32_t n_
     if (part[i] != 0.0f) atomicAdd(&grad[i], (double)part[i]);
 }
 
-template <bool BVH, bool ADJ = false>
+template <int BV, bool ADJ = false, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
                                                        uint64_t seed, int uc, int maxd, Blk blk,
                                                        AdjTrace at = AdjTrace{nullptr, nullptr}) {
   __shared__ float part[ADJ ? 3 * ADJ_LDS_MATS : 1];
   __shared__ uint16_t evl[ADJ ? ADJ_MAX_EV * 256 : 1];
-  scene_to_lds<BVH>(sc);
+  scene_to_lds<BV>(sc);
   const uint32_t w = blk.b * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (ADJ) {
     for (uint32_t i = threadIdx.x; i < 3 * ADJ_LDS_MATS; i += blockDim.x) part[i] = 0.0f;
@@ -351,7 +352,7 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
     const uint32_t swl = p.wl;
     const uint32_t smeta = pack_meta(p.wl, p.depth, p.pid);
     int ev_mat = -1;
-    if (alive) alive = photon_bounce<BVH>(sc, uc, maxd, p, rng, store, spos, ev_mat);
+    if (alive) alive = photon_bounce<BV>(sc, uc, maxd, p, rng, store, spos, ev_mat);
     const uint64_t sm = __ballot(store);
     if (sm) {
       if (store) {
@@ -387,11 +388,11 @@ __device__ __forceinline__ void k_trace_photons_body(SceneView sc, PassState* st
   }
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_trace_photons(SceneView sc, PassState* st, PhotonRec* rec, uint32_t* count,
                                                        uint32_t n_waves, uint32_t per_wave, uint32_t region_cap,
                                                        uint64_t seed, int uc, int maxd) {
-  k_trace_photons_body<BVH>(sc, st, rec, count, n_waves, per_wave, region_cap, seed, uc, maxd, Blk{blockIdx.x, gridDim.x});
+  k_trace_photons_body<BV>(sc, st, rec, count, n_waves, per_wave, region_cap, seed, uc, maxd, Blk{blockIdx.x, gridDim.x});
 }
 
 // Exclusive scan of up to 65536 counts in one block; the total goes to
@@ -452,12 +453,21 @@ hipError_t launch_trace_photons(hipStream_t s, const SceneBuf& sc, PassState* st
                                 int use_classic, int max_depth) {
   const uint32_t n_waves = pb.n_regions;
   const uint32_t per_wave = (uint32_t)((sc.n_emit_total + (uint64_t)n_waves - 1) / n_waves);
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_trace_photons<true>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_trace_photons<2>, dim3((n_waves + 3) / 4), dim3(256), bvh_dyn_lds(sc), s, make_view(sc),
+                       st, pb.rec,
                        pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
-  else
-    hipLaunchKernelGGL(k_trace_photons<false>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_trace_photons<1>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc),
+                       st, pb.rec,
                        pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
+      break;
+    default:
+      hipLaunchKernelGGL(k_trace_photons<0>, dim3((n_waves + 3) / 4), dim3(256), 0, s, make_view(sc), st, pb.rec,
+                       pb.count, n_waves, per_wave, pb.region_cap, seed, use_classic, max_depth);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, pb.count, pb.offset, n_waves, &st->n_stored,
@@ -1379,12 +1389,12 @@ __device__ __forceinline__ uint64_t peer_mask(uint32_t key, bool pred) {
 // only on the tile: results are reproducible.  Deeper queries beyond a
 // pixel's sec_max slots go to the overflow region (atomic slots, added in
 // any order).
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves, Blk blk) {
   __shared__ uint32_t s_nsec[4 * 64];  // per wave of the block and pixel of its tile
   __shared__ float s_acc[4 * 64 * 3];
-  scene_to_lds<BVH>(sc);
+  scene_to_lds<BV>(sc);
   const uint32_t wb = threadIdx.x >> 6;
   const uint32_t w = blk.b * (blockDim.x / 64) + wb;
   if (w >= n_waves) return;
@@ -1456,7 +1466,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
         const uint32_t med = (cur.meta >> 5) & 2047u;
         const uint32_t path = cur.meta >> 16;
         Isect is;
-        if ((int)depth < maxd && intersect<BVH>(sc, cur.o, cur.d, is)) {
+        if ((int)depth < maxd && intersect<BV>(sc, cur.o, cur.d, is)) {
           const DevMaterial M = material(sc, is.mat);
           const DevMaterial Mm = material(sc, med);
           Rng nr;
@@ -1591,10 +1601,10 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
   }
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves) {
-  k_eye_body<BVH>(sc, st, eb, seed, uc, maxd, n_waves, Blk{blockIdx.x, gridDim.x});
+  k_eye_body<BV>(sc, st, eb, seed, uc, maxd, n_waves, Blk{blockIdx.x, gridDim.x});
 }
 
 // Queries kept in place per tile (primary + deeper), for the sort's offsets.
@@ -1618,12 +1628,21 @@ hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const Ey
   uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u), eb.estack_lanes / 64u);
   if (n_waves < 1) n_waves = 1;
   const uint32_t blocks = (n_waves + 3) / 4;
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_eye<true>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_eye<2>, dim3(blocks), dim3(256), bvh_dyn_lds(sc), s, make_view(sc), st, eb, seed,
+                       use_classic,
                        max_depth, n_waves);
-  else
-    hipLaunchKernelGGL(k_eye<false>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_eye<1>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed,
+                       use_classic,
                        max_depth, n_waves);
+      break;
+    default:
+      hipLaunchKernelGGL(k_eye<0>, dim3(blocks), dim3(256), 0, s, make_view(sc), st, eb, seed, use_classic,
+                       max_depth, n_waves);
+  }
   hipLaunchKernelGGL(k_tile_count, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb);
   return hipGetLastError();
 }
@@ -1766,10 +1785,10 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 // BSDF factor and path throughput.  Gather and shadow rays share the kernel
 // so that the waves of one hide the memory latency of the other.  A query
 // whose cells exceed the grid's span rule raises flag 4 of the pass state.
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc, Blk blk) {
-  scene_to_lds<BVH>(sc);
+  scene_to_lds<BV>(sc);
   const float r2 = st->r2f, scale = st->scale, inv_r2 = 1.0f / r2;
   const uint32_t nq = st->n_qsort;
   const GridView g = grid_view(st, sc.bmin);
@@ -1794,16 +1813,16 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
                              ab += wl == 2 ? v : 0.0f;
                            });
     C3 di = C3{ar * scale, ag * scale, ab * scale};
-    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b));
+    if (uc) di = di + direct_light<BV>(sc, mk(qq.a), mk(qq.b));
     eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
   if (!complete) atomicOr(&st->overflow, 4u);
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc) {
-  k_shade_body<BVH>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+  k_shade_body<BV>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
 }
 
 // The same shading over a box-tree map (ppm_trace_eye on a PhotonMap, whose
@@ -1811,7 +1830,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView sc, const uint2* cells,
 // and traverses the tree for the union of their balls; each overlapped leaf
 // is staged once into LDS with coalesced loads and every lane whose ball
 // overlaps the leaf box tests its photons from LDS (broadcast reads).
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_shade_tree_body(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc, Blk blk) {
@@ -1879,16 +1898,16 @@ __device__ __forceinline__ void k_shade_tree_body(SceneView sc, const Node* node
       node += 1;
     }
     C3 di = C3{ar * scale, ag * scale, ab * scale};
-    if (uc) di = di + direct_light<BVH>(sc, mk(qq.a), mk(qq.b));
+    if (uc) di = di + direct_light<BV>(sc, mk(qq.a), mk(qq.b));
     if (active) eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
   }
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_shade_tree(SceneView sc, const Node* nodes, const PhotonRec* ph,
                                                const PassState* st, EyeBuf eb, const uint32_t* order, int pf,
                                                int uc) {
-  k_shade_tree_body<BVH>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+  k_shade_tree_body<BV>(sc, nodes, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
 }
 
 // ===========================================================================
@@ -1962,11 +1981,11 @@ __global__ __launch_bounds__(256) void k_adj_permute(const uint32_t* src_idx, co
   if (e < st->n_stored) adj_slot[src_idx[e]] = adj_sorted[e];
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_adj_trace(SceneView sc, PassState* st, uint32_t n_waves, uint32_t per_wave,
                                                    uint32_t region_cap, uint64_t seed, int uc, int maxd,
                                                    const float* adj_slot, double* grad) {
-  k_trace_photons_body<BVH, true>(sc, st, nullptr, nullptr, n_waves, per_wave, region_cap, seed, uc, maxd,
+  k_trace_photons_body<BV, true>(sc, st, nullptr, nullptr, n_waves, per_wave, region_cap, seed, uc, maxd,
                                   Blk{blockIdx.x, gridDim.x}, AdjTrace{adj_slot, grad});
 }
 
@@ -1984,12 +2003,21 @@ hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pa
                      adj_slot, (const PassState*)st);
   const uint32_t n_waves = pb.n_regions;
   const uint32_t per_wave = (uint32_t)((sc.n_emit_total + (uint64_t)n_waves - 1) / n_waves);
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_adj_trace<true>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves, per_wave,
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_adj_trace<2>, dim3((n_waves + 3) / 4), dim3(256), bvh_dyn_lds(sc), s, v, st, n_waves,
+                       per_wave,
                        pb.region_cap, seed, use_classic, max_depth, (const float*)adj_slot, grad);
-  else
-    hipLaunchKernelGGL(k_adj_trace<false>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves, per_wave,
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_adj_trace<1>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves,
+                       per_wave,
                        pb.region_cap, seed, use_classic, max_depth, (const float*)adj_slot, grad);
+      break;
+    default:
+      hipLaunchKernelGGL(k_adj_trace<0>, dim3((n_waves + 3) / 4), dim3(256), 0, s, v, st, n_waves, per_wave,
+                       pb.region_cap, seed, use_classic, max_depth, (const float*)adj_slot, grad);
+  }
   return hipGetLastError();
 }
 
@@ -2082,19 +2110,37 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   if (m.cells) {
     // persistent: at least as many waves as stay resident (8 blocks of 4
     // waves per CU); waves beyond that find the groups taken and leave
-    if (sc.bvh)
-      hipLaunchKernelGGL(k_shade<true>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur], pfilter,
+    switch (bvh_kind(sc)) {
+      case 2:
+        hipLaunchKernelGGL(k_shade<2>, dim3(blocks), dim3(256), bvh_dyn_lds(sc), s, v, m.cells, m.sorted, st, eb,
+                         qv[cur], pfilter,
                          use_classic);
-    else
-      hipLaunchKernelGGL(k_shade<false>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
+        break;
+      case 1:
+        hipLaunchKernelGGL(k_shade<1>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb,
+                         qv[cur], pfilter,
+                         use_classic);
+        break;
+      default:
+        hipLaunchKernelGGL(k_shade<0>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
                          pfilter, use_classic);
+    }
   } else {
-    if (sc.bvh)
-      hipLaunchKernelGGL(k_shade_tree<true>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur],
+    switch (bvh_kind(sc)) {
+      case 2:
+        hipLaunchKernelGGL(k_shade_tree<2>, dim3(blocks), dim3(256), bvh_dyn_lds(sc), s, v, m.nodes, m.sorted, st,
+                         eb, qv[cur],
                          pfilter, use_classic);
-    else
-      hipLaunchKernelGGL(k_shade_tree<false>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur],
+        break;
+      case 1:
+        hipLaunchKernelGGL(k_shade_tree<1>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st,
+                         eb, qv[cur],
                          pfilter, use_classic);
+        break;
+      default:
+        hipLaunchKernelGGL(k_shade_tree<0>, dim3(blocks), dim3(256), 0, s, v, m.nodes, m.sorted, st, eb, qv[cur],
+                         pfilter, use_classic);
+    }
   }
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
@@ -2207,12 +2253,12 @@ namespace ppm {
 
 // Test hook: nearest hit of host-given rays (calc_intersection,
 // tracer.rs:306-335): t and scene index of the shape, or (inf, -1).
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_intersect(SceneView sc, const float4* o, const float4* d, uint32_t n,
                                                    float* t_out, int32_t* obj_out) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const Hit h = nearest<BVH>(sc, mk(o[i]), mk(d[i]));
+  const Hit h = nearest<BV>(sc, mk(o[i]), mk(d[i]));
   t_out[i] = h.obj >= 0 ? h.t : INF_F;
   obj_out[i] = h.obj;
 }
@@ -2220,12 +2266,19 @@ __global__ __launch_bounds__(256) void k_intersect(SceneView sc, const float4* o
 hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
                             float* t_out, int32_t* obj_out) {
   if (n == 0) return hipSuccess;
-  if (sc.bvh)
-    hipLaunchKernelGGL(k_intersect<true>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_intersect<2>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
                        obj_out);
-  else
-    hipLaunchKernelGGL(k_intersect<false>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_intersect<1>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
                        obj_out);
+      break;
+    default:
+      hipLaunchKernelGGL(k_intersect<0>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
+                       obj_out);
+  }
   return hipGetLastError();
 }
 
@@ -2275,11 +2328,11 @@ __global__ void kb_pass_end(const BatchItem* items, uint32_t n) {
   k_pass_end_body(it.st, Blk{0u, 1u});
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void kb_trace(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                 uint32_t n_all) {
   BATCH_PROLOGUE(BK_TRACE)
-  k_trace_photons_body<BVH>(make_view(it.sc), it.st, it.pb.rec, it.pb.count, it.pb.n_regions, it.trace_per_wave,
+  k_trace_photons_body<BV>(make_view(it.sc), it.st, it.pb.rec, it.pb.count, it.pb.n_regions, it.trace_per_wave,
                             it.pb.region_cap, it.seed, it.uc, it.maxd, blk);
 }
 
@@ -2372,11 +2425,11 @@ __global__ __launch_bounds__(256) void kb_tile_order(const BatchItem* items, con
   k_tile_order_body(it.eb, it.st, blk);
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void kb_eye(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                               uint32_t n_all) {
   BATCH_PROLOGUE(BK_EYE)
-  k_eye_body<BVH>(make_view(it.sc), it.st, it.eb, it.seed, it.uc, it.maxd, it.eye_waves, blk);
+  k_eye_body<BV>(make_view(it.sc), it.st, it.eb, it.seed, it.uc, it.maxd, it.eye_waves, blk);
 }
 
 __global__ __launch_bounds__(256) void kb_tile_count(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -2391,11 +2444,11 @@ __global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, con
   k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, 1023.0f, it.eb.qkeys[0], it.eb.qvals[0], blk);
 }
 
-template <bool BVH>
+template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void kb_shade(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                 uint32_t n_all) {
   BATCH_PROLOGUE(BK_SHADE)
-  k_shade_body<BVH>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc,
+  k_shade_body<BV>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc,
                     blk);
 }
 
@@ -2415,7 +2468,7 @@ __global__ __launch_bounds__(256) void kb_accumulate(const BatchItem* items, con
 
 hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join,
                              const BatchItem* items, const uint32_t* tab, const uint32_t* blocks, uint32_t n,
-                             uint32_t n_all, bool bvh) {
+                             uint32_t n_all, bool bvh, uint32_t dyn) {
   // blocks[k] = total blocks of kind k over the first n scenes.  The eye
   // branch (tile order, eye trace, query sort) runs on s2 beside the photon
   // branch (trace, grid build): the two touch disjoint buffers and counters.
@@ -2425,10 +2478,16 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(s2, fork, 0)) != hipSuccess) return e;
   hipLaunchKernelGGL(kb_tile_order, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
-  if (bvh)
-    hipLaunchKernelGGL(kb_eye<true>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
-  else
-    hipLaunchKernelGGL(kb_eye<false>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
+  switch ((bvh ? (dyn != 0u ? 2 : 1) : 0)) {
+    case 2:
+      hipLaunchKernelGGL(kb_eye<2>, dim3(blocks[BK_EYE]), T256, dyn, s2, items, tab, n, n_all);
+      break;
+    case 1:
+      hipLaunchKernelGGL(kb_eye<1>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
+      break;
+    default:
+      hipLaunchKernelGGL(kb_eye<0>, dim3(blocks[BK_EYE]), T256, 0, s2, items, tab, n, n_all);
+  }
   hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s2, items, n);
   hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s2, items, tab, n, n_all);
@@ -2439,10 +2498,16 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   }
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipEventRecord(join, s2)) != hipSuccess) return e;
-  if (bvh)
-    hipLaunchKernelGGL(kb_trace<true>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
-  else
-    hipLaunchKernelGGL(kb_trace<false>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
+  switch ((bvh ? (dyn != 0u ? 2 : 1) : 0)) {
+    case 2:
+      hipLaunchKernelGGL(kb_trace<2>, dim3(blocks[BK_TRACE]), T256, dyn, s, items, tab, n, n_all);
+      break;
+    case 1:
+      hipLaunchKernelGGL(kb_trace<1>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
+      break;
+    default:
+      hipLaunchKernelGGL(kb_trace<0>, dim3(blocks[BK_TRACE]), T256, 0, s, items, tab, n, n_all);
+  }
   hipLaunchKernelGGL(kb_scan_photons, dim3(n), T1024, 0, s, items, n);
   hipLaunchKernelGGL(kb_grid_setup, dim3(n), T64, 0, s, items, n);
   hipLaunchKernelGGL(kb_grid_keys, dim3(blocks[BK_MORTON]), T256, 0, s, items, tab, n, n_all);
@@ -2456,10 +2521,16 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   hipLaunchKernelGGL(kb_grid_permute, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
-  if (bvh)
-    hipLaunchKernelGGL(kb_shade<true>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
-  else
-    hipLaunchKernelGGL(kb_shade<false>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
+  switch ((bvh ? (dyn != 0u ? 2 : 1) : 0)) {
+    case 2:
+      hipLaunchKernelGGL(kb_shade<2>, dim3(blocks[BK_SHADE]), T256, dyn, s, items, tab, n, n_all);
+      break;
+    case 1:
+      hipLaunchKernelGGL(kb_shade<1>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
+      break;
+    default:
+      hipLaunchKernelGGL(kb_shade<0>, dim3(blocks[BK_SHADE]), T256, 0, s, items, tab, n, n_all);
+  }
   hipLaunchKernelGGL(kb_overflow_add, dim3(blocks[BK_OVF]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_accumulate, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_pass_end, dim3(n), T64, 0, s, items, n);

@@ -26,6 +26,19 @@ struct SceneBuf {
   float4 bmin{}, bmax{};
 };
 
+// Dynamic LDS a BVH kernel's launch gives a scene: the copies of a small tree
+// (all nodes and the primitives as compact spheres), 0 for a large tree or
+// none.  The scene view's dyn_lds flag comes from the same test.
+constexpr uint32_t BVH_DYN_LDS_BYTES = 255u * 32u + 512u * (16u + 4u);
+__host__ __device__ inline uint32_t bvh_dyn_lds(const SceneBuf& b) {
+  const bool small = b.bvh != nullptr && b.n_bprims > 0 && b.n_bprims <= BVH_SMALL_PRIMS &&
+                     ((2u << b.bvh_depth) - 1u) <= 255u;
+  return small ? BVH_DYN_LDS_BYTES : 0u;
+}
+// The instantiation of a BVH kernel for scene buffers b: 0 no BVH, 1 a large
+// tree (read from global memory only), 2 a small tree with its LDS copies.
+inline int bvh_kind(const SceneBuf& b) { return b.bvh ? (bvh_dyn_lds(b) != 0u ? 2 : 1) : 0; }
+
 // Photons in regions: region r holds count[r] records starting at
 // rec + r * region_cap; offset[r] is the exclusive prefix of the counts, so
 // the set reads densely as i -> region r = last with offset[r] <= i.
@@ -219,7 +232,7 @@ hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pa
 // so the scenes still running are a prefix); blocks[k] = tab[k*(n_all+1)+n].
 hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join,
                              const BatchItem* items, const uint32_t* tab, const uint32_t* blocks, uint32_t n,
-                             uint32_t n_all, bool bvh);
+                             uint32_t n_all, bool bvh, uint32_t dyn);  // dyn: the BVH kernels' dynamic LDS
 
 // Test hook: nearest hit of n rays (unit directions).
 hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, const float4* d, uint32_t n,
