@@ -748,24 +748,133 @@ struct LightSelf {
   float nqn;
 };
 
+// min over the box bx of dot(nv, x - q): above a small pad, the box lies wholly
+// on the positive side of the plane through q with normal nv
+__device__ __forceinline__ float box_beyond(const Node& bx, V3 nv, V3 q) {
+  const float x = nv.x > 0.0f ? bx.lo.x : bx.hi.x;
+  const float y = nv.y > 0.0f ? bx.lo.y : bx.hi.y;
+  const float z = nv.z > 0.0f ? bx.lo.z : bx.hi.z;
+  return dot(nv, mk(x - q.x, y - q.y, z - q.z));
+}
+
+// The candidates (cand, a bit per shape of lin) that a shadow ray from p to a
+// sample of L still has to test.  Both cuts are exact:
+// - a parallelogram, polygon or plane whose plane holds p while the samples'
+//   area lies strictly on one side of it: a ray from p to a sample x meets
+//   that plane at |t| = |e(p)| |x - p| / |e(x) - e(p)| (e: signed distance to
+//   the plane), at most (|e(p)| + err) dmax / (emin - |e(p)| - err) with dmax
+//   the farthest corner of the area, emin the nearest corner's |e| and err a
+//   bound on the float error of the shape's own test; kept below 1e-5, the
+//   test's t is under NEARLY0 and it reports no hit, so the shape is dropped
+//   for every sample (need_o, need_s);
+// - a shape whose padded box lies beyond a face of the pyramid with apex p
+//   over the samples' area (the TSS points lie in the parallelogram of
+//   (u, v) in [0.1, 0.9]^2): an occluding hit lies on the segment from p to
+//   the sample, inside that pyramid, so such a shape is met, if at all,
+//   beyond the sample.  It never occludes and matters only through `any`,
+//   which the samples that skip the light's own shape (LightSelf) already
+//   hold: dropped for those samples only (need_s, which also drops self).
+__device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight& L, V3 p, uint64_t cand, uint64_t wcand,
+                                            uint32_t self, uint64_t& need_o, uint64_t& need_s) {
+  // corner j of the samples' area: a + u_j e1 + v_j e2, counter-clockwise in (u, v)
+  const V3 a = mk(L.pos), e1 = mk(L.d1), e2 = mk(L.d2);
+  const V3 kc = a + 0.5f * e1 + 0.5f * e2;
+  float dmax = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const V3 r = a + (j == 1 || j == 2 ? 0.9f : 0.1f) * e1 + (j >= 2 ? 0.9f : 0.1f) * e2 - p;
+    dmax = fmaxf(dmax, dot(r, r));
+  }
+  dmax = sqrtf(dmax);
+  const float m = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z))));
+  const float l1 = sqrtf(dot(e1, e1)), l2 = sqrtf(dot(e2, e2));
+  const float padk = 1e-5f * (1.0f + m + dmax + l1 + l2);
+  need_o = cand;
+  need_s = self < CAND_BITS ? cand & ~(1ull << self) : cand;
+  // pyramid faces one at a time (few registers held across the shape loop):
+  // sides j = 0..3 through p and corners j, j+1; the base (j = 4) through a.
+  // Outward normals; each pad is scaled by |x||y| of its cross product x X y,
+  // which bounds its rounding error also where the face is nearly degenerate.
+  uint64_t out = 0;
+#pragma unroll 1
+  for (int j = 0; j < 5; ++j) {
+    V3 f, q;
+    float pad;
+    if (j < 4) {
+      const int j1 = (j + 1) & 3;
+      const V3 ra = a + (j == 1 || j == 2 ? 0.9f : 0.1f) * e1 + (j >= 2 ? 0.9f : 0.1f) * e2 - p;
+      const V3 rb = a + (j1 == 1 || j1 == 2 ? 0.9f : 0.1f) * e1 + (j1 >= 2 ? 0.9f : 0.1f) * e2 - p;
+      f = cross(ra, rb);
+      f = sel(dot(f, kc - p) > 0.0f, -1.0f * f, f);
+      q = p;
+      pad = padk * sqrtf(dot(ra, ra)) * sqrtf(dot(rb, rb));
+    } else {
+      f = cross(e1, e2);
+      f = sel(dot(f, p - a) > 0.0f, -1.0f * f, f);
+      q = a;
+      pad = padk * l1 * l2;
+    }
+    for (uint64_t w = wcand; w != 0; w &= w - 1) {
+      const uint32_t i = (uint32_t)__builtin_ctzll(w);
+      const Node bx = lin_box(sc, i);
+      if (bx.lo.w == 0.0f && box_beyond(bx, f, q) > pad) out |= 1ull << i;
+    }
+  }
+  need_s &= ~out;
+  for (uint64_t w = wcand; w != 0; w &= w - 1) {
+    const uint32_t i = (uint32_t)__builtin_ctzll(w);
+    if (((cand >> i) & 1ull) == 0) continue;
+    const DevShape s = lin_shape(sc, i);
+    if (s.kind != SHAPE_PLAIN && s.kind != SHAPE_POLYGON && s.kind != SHAPE_PARALLELOGRAM) continue;
+    V3 nn, q;
+    float err;
+    if (s.kind == SHAPE_PLAIN) {
+      nn = mk(s.n);
+      q = (-s.dist) * nn;
+      err = 1e-6f * (fabsf(s.dist) + sqrtf(dot(p, p)));
+    } else {
+      const V3 d1 = mk(s.d1), d2 = mk(s.d2);
+      const V3 c = cross(d1, d2);
+      const float cl = sqrtf(dot(c, c));
+      if (!(cl > 0.0f)) continue;
+      nn = (1.0f / cl) * c;
+      q = mk(s.p);
+      const V3 pp = p - q;
+      err = 1e-6f * sqrtf(dot(pp, pp)) * sqrtf(dot(d1, d1)) * sqrtf(dot(d2, d2)) / cl;
+    }
+    // distances of p and of the four corners to the shape's plane
+    const float ep = dot(nn, p - q);
+    const float ga = dot(nn, a - q), gu = dot(nn, e1), gv = dot(nn, e2);
+    const float g0 = ga + 0.1f * gu + 0.1f * gv, g1 = ga + 0.9f * gu + 0.1f * gv;
+    const float g2 = ga + 0.9f * gu + 0.9f * gv, g3 = ga + 0.1f * gu + 0.9f * gv;
+    const float gmin = fminf(fminf(g0, g1), fminf(g2, g3)), gmax = fmaxf(fmaxf(g0, g1), fmaxf(g2, g3));
+    const float emin = gmin > 0.0f ? gmin : (gmax < 0.0f ? -gmax : 0.0f);
+    const float ap = fabsf(ep) + err + 1e-6f * (fabsf(ga) + fabsf(gu) + fabsf(gv));
+    if (emin > 2.0f * ap && ap * dmax <= 1e-5f * (emin - ap)) {
+      need_o &= ~(1ull << i);
+      need_s &= ~(1ull << i);
+    }
+  }
+}
+
 template <int BV, bool BVH = (BV != 0)>
-__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, uint64_t wunion,
-                                           const LightSelf& ls, float& cos0sq, float& sqd) {
+__device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, uint64_t need_o,
+                                           uint64_t need_s, uint64_t wu_o, uint64_t wu_s, const LightSelf& ls,
+                                           float& cos0sq, float& sqd) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
   if (c0 < 0.0f) return false;
   const float sq_l = dot(ld, ld);
-  bool any = false;
-  if (ls.self < CAND_BITS && sq_l >= 1e-4f && sq_l <= 100.0f && fabsf(dot(ld3, ls.nq)) >= 0.1f * ls.nqn) {
-    any = true;
-    cand &= ~(1ull << ls.self);
-  }
-  // the shapes some lane of the wave may need (wunion, wave-uniform), each
-  // tested by the lanes whose candidates hold it
-  for (uint64_t m = wunion; m != 0; m &= m - 1) {
+  const bool self = ls.self < CAND_BITS && sq_l >= 1e-4f && sq_l <= 100.0f && fabsf(dot(ld3, ls.nq)) >= 0.1f * ls.nqn;
+  bool any = self;
+  const uint64_t need = self ? need_s : need_o;
+  // the shapes some active lane of the wave needs (wave-uniform), each tested
+  // by the lanes that need it
+  const uint64_t wu = __ballot(!self) != 0 ? wu_o : wu_s;
+  for (uint64_t m = wu; m != 0; m &= m - 1) {
     const uint32_t i = (uint32_t)__builtin_ctzll(m);
-    if (((cand >> i) & 1ull) == 0) continue;
+    if (((need >> i) & 1ull) == 0) continue;
     if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
   }
   for (uint32_t i = CAND_BITS; i < sc.n_lin; ++i)
@@ -831,12 +940,20 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
     const uint32_t nb = min(sc.n_lin, CAND_BITS);
     for (uint32_t i = 0; i < nb; ++i)
       if (__ballot(((cand >> i) & 1ull) != 0)) wunion |= 1ull << i;
+    uint64_t need_o, need_s;
+    shadow_need(sc, L, p, cand, wunion, ls.self, need_o, need_s);
+    uint64_t wu_o = 0, wu_s = 0;
+    for (uint64_t w = wunion; w != 0; w &= w - 1) {
+      const uint32_t i = (uint32_t)__builtin_ctzll(w);
+      if (__ballot(((need_o >> i) & 1ull) != 0)) wu_o |= 1ull << i;
+      if (__ballot(((need_s >> i) & 1ull) != 0)) wu_s |= 1ull << i;
+    }
     C3 prev{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < 25; ++s) {
       const V3 ld = tss_point(L, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BV>(sc, p, n, ld, cand, wunion, ls, csq, sqd)) continue;
+      if (!sample_lit<BV>(sc, p, n, ld, cand, need_o, need_s, wu_o, wu_s, ls, csq, sqd)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }
