@@ -1680,7 +1680,7 @@ hipError_t batch_layout(ppm_batch* b) {
     it.r2_first = s->cfg.radius0 * s->cfg.radius0;
     it.alpha = s->cfg.alpha;
     it.power = s->power;
-    it.mscale = morton_scale(s->sbuf.bmin, s->sbuf.bmax, 1023u);
+    it.mscale = morton_scale(s->sbuf.bmin, s->sbuf.bmax, QUERY_MORTON_MAX);
     it.first_pass = s->cfg.first_pass;
     it.pass_stride = b->K;
     it.uc = s->cfg.use_classic;

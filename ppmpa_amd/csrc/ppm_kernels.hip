@@ -525,7 +525,8 @@ __device__ __forceinline__ uint32_t expand_bits10(uint32_t v) {
 // beyond (clamped).  Far photons (walks that leaked through a corner of the
 // geometry travel arbitrarily far) thus sort into leaves of their own instead
 // of sharing a leaf, and a huge bounding box, with photons near the scene.
-__device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, float4 bmin, float4 bscale, float fq) {
+__device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, float4 bmin, float4 bscale, float fq,
+                                                uint32_t level_shift = 30u) {
   float fx = (px - bmin.x) * bscale.x;
   float fy = (py - bmin.y) * bscale.y;
   float fz = (pz - bmin.z) * bscale.z;
@@ -540,7 +541,7 @@ __device__ __forceinline__ uint32_t morton_key(float px, float py, float pz, flo
   fx = fminf(fmaxf((fx - h) * s + h, 0.0f), fq);
   fy = fminf(fmaxf((fy - h) * s + h, 0.0f), fq);
   fz = fminf(fmaxf((fz - h) * s + h, 0.0f), fq);
-  return (level << 30) | (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) |
+  return (level << level_shift) | (expand_bits10((uint32_t)fx) << 2) | (expand_bits10((uint32_t)fy) << 1) |
          expand_bits10((uint32_t)fz);
 }
 
@@ -1741,7 +1742,7 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     if (i < n_ovf) {
       const uint32_t pos = st->n_qsort - n_ovf + i;
       const float4 a = eb.q[ovf_base + i].a;
-      keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+      keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
       vals[pos] = (uint32_t)(ovf_base + i);
     }
     return;
@@ -1758,14 +1759,14 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
   uint32_t pos = eb.tile_off[g] + incl - c;
   if (primary) {
     const float4 a = eb.q[t].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
     vals[pos] = t;
     ++pos;
   }
   for (uint32_t k = 0; k < nsec; ++k) {
     const uint32_t slot = eb.q_primary * (1u + k) + t;
     const float4 a = eb.q[slot].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq);
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
     vals[pos] = slot;
     ++pos;
   }
@@ -2084,16 +2085,17 @@ __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, do
 hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb) {
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, eb.tile_count, eb.tile_off, eb.n_tiles,
                      &st->n_qsort, (const uint32_t*)&st->n_queries, eb.ovf_cap);
-  const uint32_t qmax = (1u << m.morton_bits) - 1u;
+  const uint32_t qmax = QUERY_MORTON_MAX;
+  const int first = (int)QUERY_KEYS_FIRST;
   hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, st, sc.bmin,
-                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, eb.qkeys[0], eb.qvals[0]);
+                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, eb.qkeys[first], eb.qvals[first]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  uint32_t* qk[2] = {eb.qkeys[0], eb.qkeys[1]};
-  uint32_t* qv[2] = {eb.qvals[0], eb.qvals[1]};
+  uint32_t* qk[2] = {eb.qkeys[first], eb.qkeys[first ^ 1]};
+  uint32_t* qv[2] = {eb.qvals[first], eb.qvals[first ^ 1]};
   int cur = 0;
-  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, 32u, &cur);
-  if (e == hipSuccess && cur != 0) e = hipErrorInvalidValue;  // the sorted order is read from qvals[0]
+  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, QUERY_KEY_BITS, &cur);
+  if (e == hipSuccess && (cur ^ first) != 0) e = hipErrorInvalidValue;  // the sorted order is read from qvals[0]
   return e;
 }
 
@@ -2386,14 +2388,14 @@ __global__ __launch_bounds__(256) void kb_grid_permute(const BatchItem* items, c
                       blk);
 }
 
-// Radix-sort digit pass `p` (shift 8p) of the photon keys (which = 0) or the
-// query keys (which = 1).
+// Radix-sort digit pass `p` (shift 8p) of the photon keys (which = 0, from
+// buffer 0) or the query keys (which = 1, from buffer QUERY_KEYS_FIRST).
 __global__ __launch_bounds__(RS_THREADS) void kb_rs_hist(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                          uint32_t n_all, int which, uint32_t p) {
   BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
   // selects, not a runtime index into the item's arrays (which would put the
   // whole item in scratch memory)
-  const bool odd = (p & 1u) != 0;
+  const bool odd = ((p + (which ? QUERY_KEYS_FIRST : 0u)) & 1u) != 0;
   const uint32_t* keys = which ? (odd ? it.eb.qkeys[1] : it.eb.qkeys[0]) : (odd ? it.m.keys[1] : it.m.keys[0]);
   k_rs_hist_body(keys, which ? &it.st->n_qsort : &it.st->n_stored, which ? it.eb.qghist : it.m.ghist, 8u * p, blk);
 }
@@ -2408,7 +2410,7 @@ __global__ __launch_bounds__(1024) void kb_rs_scan(const BatchItem* items, const
 __global__ __launch_bounds__(RS_THREADS) void kb_rs_scatter(const BatchItem* items, const uint32_t* tab,
                                                             uint32_t n, uint32_t n_all, int which, uint32_t p) {
   BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
-  const bool odd = (p & 1u) != 0;
+  const bool odd = ((p + (which ? QUERY_KEYS_FIRST : 0u)) & 1u) != 0;
   if (which)
     k_rs_scatter_body(odd ? it.eb.qkeys[1] : it.eb.qkeys[0], odd ? it.eb.qvals[1] : it.eb.qvals[0],
                       odd ? it.eb.qkeys[0] : it.eb.qkeys[1], odd ? it.eb.qvals[0] : it.eb.qvals[1], &it.st->n_qsort,
@@ -2441,7 +2443,9 @@ __global__ __launch_bounds__(256) void kb_tile_count(const BatchItem* items, con
 __global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                      uint32_t n_all) {
   BATCH_PROLOGUE(BK_QKEYS)
-  k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, 1023.0f, it.eb.qkeys[0], it.eb.qvals[0], blk);
+  k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, (float)QUERY_MORTON_MAX,
+                    QUERY_KEYS_FIRST != 0u ? it.eb.qkeys[1] : it.eb.qkeys[0],
+                    QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], blk);
 }
 
 template <int BV, bool BVH = (BV != 0)>
@@ -2491,7 +2495,7 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s2, items, n);
   hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s2, items, tab, n, n_all);
-  for (uint32_t p = 0; p < 4; ++p) {
+  for (uint32_t p = 0; p < QUERY_KEY_BITS / 8u; ++p) {
     hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);
     hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s2, items, tab, n, n_all, 1);
     hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);

@@ -136,6 +136,16 @@ hipError_t launch_map_build(hipStream_t s, MapBuf& m, PassState* st, const Photo
 constexpr uint32_t GRID_KEY_BITS = 24;  // bucket keys < 2^24: three 8-bit sort passes
 constexpr uint32_t GRID_MAX_BUCKETS = 1u << GRID_KEY_BITS;
 
+// Query sort keys (gather order): a Morton code of 7 bits per axis of the
+// scene box (7.8 mm cells in a 1 m box, finer than the gather's grid cells)
+// with the 2-bit level of points outside the box above it, 23 bits: three
+// 8-bit passes.  The keys start in buffer QUERY_KEYS_FIRST so that the sorted
+// order ends in qvals[0].
+constexpr uint32_t QUERY_MORTON_MAX = 127;
+constexpr uint32_t QUERY_LEVEL_SHIFT = 21;
+constexpr uint32_t QUERY_KEY_BITS = 24;
+constexpr uint32_t QUERY_KEYS_FIRST = (QUERY_KEY_BITS / 8u) & 1u;
+
 // Blocks of the per-photon and per-bucket grid kernels.
 inline uint32_t grid_photon_blocks(uint64_t cap) {
   const uint64_t b = (cap + 255) / 256;
@@ -190,7 +200,7 @@ struct BatchItem {
   double* sumsq;
   uint64_t seed;
   double r2_first, alpha, power;
-  float4 mscale;             // Morton scale of the scene bounds (1023 cells per axis)
+  float4 mscale;             // Morton scale of the scene bounds (QUERY_MORTON_MAX + 1 cells per axis)
   uint32_t first_pass;       // this pipeline's first pass
   uint32_t pass_stride;      // passes between two of this pipeline's (the batch's pipelines)
   int32_t uc, maxd, pfilter;
