@@ -1786,7 +1786,26 @@ __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* 
 // BSDF factor and path throughput.  Gather and shadow rays share the kernel
 // so that the waves of one hide the memory latency of the other.  A query
 // whose cells exceed the grid's span rule raises flag 4 of the pass state.
-template <int BV, bool BVH = (BV != 0)>
+// The gather of one query: the summed -n.dir * filter * weight per channel
+// into ar, ag, ab.  PF >= 0 fixes the photon filter at compile time (no test
+// of it in the per-photon loop, and only that filter's registers); PF = -1
+// takes it from pf (batches, whose scenes may differ).
+template <int PF>
+__device__ __forceinline__ bool shade_gather(const GridView& g, const uint2* cells, const PhotonRec* ph, const Query& qq,
+                                             float r2, float inv_r2, int pf, float& ar, float& ag, float& ab) {
+  const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
+  const int f = PF >= 0 ? PF : pf;
+  return grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2, [&](float4 pm, float4 dm, float d2, bool in) {
+    const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
+    const float v = (in && c < 0.0f) ? -c * filter_weight(f, d2 * inv_r2) * dm.w : 0.0f;
+    const uint32_t wl = __float_as_uint(pm.w) & 3u;
+    ar += wl == 0 ? v : 0.0f;
+    ag += wl == 1 ? v : 0.0f;
+    ab += wl == 2 ? v : 0.0f;
+  });
+}
+
+template <int BV, int PF, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc, Blk blk) {
   scene_to_lds<BV>(sc);
@@ -1803,16 +1822,7 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
     const uint32_t slot = order[i];
     const Query qq = eb.q[slot];
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-    const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
-    complete &= grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
-                                 [&](float4 pm, float4 dm, float d2, bool in) {
-                             const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
-                             const float v = (in && c < 0.0f) ? -c * filter_weight(pf, d2 * inv_r2) * dm.w : 0.0f;
-                             const uint32_t wl = __float_as_uint(pm.w) & 3u;
-                             ar += wl == 0 ? v : 0.0f;
-                             ag += wl == 1 ? v : 0.0f;
-                             ab += wl == 2 ? v : 0.0f;
-                           });
+    complete &= shade_gather<PF>(g, cells, ph, qq, r2, inv_r2, pf, ar, ag, ab);
     C3 di = C3{ar * scale, ag * scale, ab * scale};
     if (uc) di = di + direct_light<BV>(sc, mk(qq.a), mk(qq.b));
     eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
@@ -1820,10 +1830,25 @@ __device__ __forceinline__ void k_shade_body(SceneView sc, const uint2* cells, c
   if (!complete) atomicOr(&st->overflow, 4u);
 }
 
-template <int BV, bool BVH = (BV != 0)>
+template <int BV, int PF>
 __global__ __launch_bounds__(256) void k_shade(SceneView sc, const uint2* cells, const PhotonRec* ph, PassState* st,
                                                EyeBuf eb, const uint32_t* order, int pf, int uc) {
-  k_shade_body<BV>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+  k_shade_body<BV, PF>(sc, cells, ph, st, eb, order, pf, uc, Blk{blockIdx.x, gridDim.x});
+}
+
+// k_shade for the pass's photon filter
+template <int BV>
+static void launch_shade(hipStream_t s, uint32_t blocks, size_t shmem, const SceneView& v, const uint2* cells,
+                         const PhotonRec* ph, PassState* st, const EyeBuf& eb, const uint32_t* order, int pf, int uc) {
+  if (pf == FILTER_CONE)
+    hipLaunchKernelGGL((k_shade<BV, FILTER_CONE>), dim3(blocks), dim3(256), shmem, s, v, cells, ph, st, eb, order, pf,
+                       uc);
+  else if (pf == FILTER_GAUSS)
+    hipLaunchKernelGGL((k_shade<BV, FILTER_GAUSS>), dim3(blocks), dim3(256), shmem, s, v, cells, ph, st, eb, order,
+                       pf, uc);
+  else
+    hipLaunchKernelGGL((k_shade<BV, FILTER_NONE>), dim3(blocks), dim3(256), shmem, s, v, cells, ph, st, eb, order, pf,
+                       uc);
 }
 
 // The same shading over a box-tree map (ppm_trace_eye on a PhotonMap, whose
@@ -2114,18 +2139,13 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
     // waves per CU); waves beyond that find the groups taken and leave
     switch (bvh_kind(sc)) {
       case 2:
-        hipLaunchKernelGGL(k_shade<2>, dim3(blocks), dim3(256), bvh_dyn_lds(sc), s, v, m.cells, m.sorted, st, eb,
-                         qv[cur], pfilter,
-                         use_classic);
+        launch_shade<2>(s, blocks, bvh_dyn_lds(sc), v, m.cells, m.sorted, st, eb, qv[cur], pfilter, use_classic);
         break;
       case 1:
-        hipLaunchKernelGGL(k_shade<1>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb,
-                         qv[cur], pfilter,
-                         use_classic);
+        launch_shade<1>(s, blocks, 0, v, m.cells, m.sorted, st, eb, qv[cur], pfilter, use_classic);
         break;
       default:
-        hipLaunchKernelGGL(k_shade<0>, dim3(blocks), dim3(256), 0, s, v, m.cells, m.sorted, st, eb, qv[cur],
-                         pfilter, use_classic);
+        launch_shade<0>(s, blocks, 0, v, m.cells, m.sorted, st, eb, qv[cur], pfilter, use_classic);
     }
   } else {
     switch (bvh_kind(sc)) {
@@ -2452,7 +2472,7 @@ template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void kb_shade(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                 uint32_t n_all) {
   BATCH_PROLOGUE(BK_SHADE)
-  k_shade_body<BV>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc,
+  k_shade_body<BV, -1>(make_view(it.sc), it.m.cells, it.m.sorted, it.st, it.eb, it.eb.qvals[0], it.pfilter, it.uc,
                     blk);
 }
 
