@@ -776,6 +776,13 @@ __device__ __forceinline__ float box_beyond(const Node& bx, V3 nv, V3 q) {
 //   hold: dropped for those samples only (need_s, which also drops self).
 __device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight& L, V3 p, uint64_t cand, uint64_t wcand,
                                             uint32_t self, uint64_t& need_o, uint64_t& need_s) {
+  need_o = cand;
+  need_s = self < CAND_BITS ? cand & ~(1ull << self) : cand;
+  // the wave's candidates other than the light's own shape (wave-uniform):
+  // none (queries whose boxes to the light meet no other shape) leaves
+  // nothing to drop
+  const uint64_t others = self < CAND_BITS ? wcand & ~(1ull << self) : wcand;
+  if (others == 0) return;
   // corner j of the samples' area: a + u_j e1 + v_j e2, counter-clockwise in (u, v)
   const V3 a = mk(L.pos), e1 = mk(L.d1), e2 = mk(L.d2);
   const V3 kc = a + 0.5f * e1 + 0.5f * e2;
@@ -789,8 +796,6 @@ __device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight&
   const float m = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z))));
   const float l1 = sqrtf(dot(e1, e1)), l2 = sqrtf(dot(e2, e2));
   const float padk = 1e-5f * (1.0f + m + dmax + l1 + l2);
-  need_o = cand;
-  need_s = self < CAND_BITS ? cand & ~(1ull << self) : cand;
   // pyramid faces one at a time (few registers held across the shape loop):
   // sides j = 0..3 through p and corners j, j+1; the base (j = 4) through a.
   // Outward normals; each pad is scaled by |x||y| of its cross product x X y,
@@ -814,14 +819,14 @@ __device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight&
       q = a;
       pad = padk * l1 * l2;
     }
-    for (uint64_t w = wcand; w != 0; w &= w - 1) {
+    for (uint64_t w = others; w != 0; w &= w - 1) {
       const uint32_t i = (uint32_t)__builtin_ctzll(w);
       const Node bx = lin_box(sc, i);
       if (bx.lo.w == 0.0f && box_beyond(bx, f, q) > pad) out |= 1ull << i;
     }
   }
   need_s &= ~out;
-  for (uint64_t w = wcand; w != 0; w &= w - 1) {
+  for (uint64_t w = others; w != 0; w &= w - 1) {
     const uint32_t i = (uint32_t)__builtin_ctzll(w);
     if (((cand >> i) & 1ull) == 0) continue;
     const DevShape s = lin_shape(sc, i);
