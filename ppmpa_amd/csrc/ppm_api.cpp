@@ -242,6 +242,7 @@ uint64_t photons_needed(uint64_t n_photon, int max_depth, int n_cu) {
 hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) {
   const uint32_t tiles = (uint32_t)cam.tiles_x * (uint32_t)((cam.yreso + 7) / 8);
   eb.q_primary = tiles * 64u;
+  eb.q_fine = eb.q_primary >= QUERY_FINE_PIXELS ? 1u : 0u;
   eb.n_tiles = tiles;
   // deeper queries kept per pixel: a glass sphere yields about one per depth
   eb.sec_max = (uint32_t)std::max(1, std::min(2 * max_depth, 32));

@@ -1732,7 +1732,8 @@ __device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const Photo
 // overflow queries, which follow all in-place ones.  Key = Morton code of
 // the position.
 __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
-                                                    float fq, uint32_t* keys, uint32_t* vals, Blk blk) {
+                                                    float fq, uint32_t lshift, uint32_t* keys, uint32_t* vals,
+                                                    Blk blk) {
   const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   const uint32_t g = t >> 6, lane = lane_id();
   const uint64_t ovf_base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
@@ -1742,7 +1743,7 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     if (i < n_ovf) {
       const uint32_t pos = st->n_qsort - n_ovf + i;
       const float4 a = eb.q[ovf_base + i].a;
-      keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
+      keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
       vals[pos] = (uint32_t)(ovf_base + i);
     }
     return;
@@ -1759,22 +1760,22 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
   uint32_t pos = eb.tile_off[g] + incl - c;
   if (primary) {
     const float4 a = eb.q[t].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
     vals[pos] = t;
     ++pos;
   }
   for (uint32_t k = 0; k < nsec; ++k) {
     const uint32_t slot = eb.q_primary * (1u + k) + t;
     const float4 a = eb.q[slot].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, QUERY_LEVEL_SHIFT);
+    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
     vals[pos] = slot;
     ++pos;
   }
 }
 
 __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
-                                                    float fq, uint32_t* keys, uint32_t* vals) {
-  k_query_keys_body(eb, st, bmin, bscale, fq, keys, vals, Blk{blockIdx.x, gridDim.x});
+                                                    float fq, uint32_t lshift, uint32_t* keys, uint32_t* vals) {
+  k_query_keys_body(eb, st, bmin, bscale, fq, lshift, keys, vals, Blk{blockIdx.x, gridDim.x});
 }
 
 // Shading of the queries in Morton order, one query per lane, 64 spatially
@@ -2110,16 +2111,19 @@ __global__ __launch_bounds__(256) void k_accumulate(EyeBuf eb, DevCamera cam, do
 hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb) {
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, eb.tile_count, eb.tile_off, eb.n_tiles,
                      &st->n_qsort, (const uint32_t*)&st->n_queries, eb.ovf_cap);
-  const uint32_t qmax = QUERY_MORTON_MAX;
-  const int first = (int)QUERY_KEYS_FIRST;
+  const bool fine = eb.q_fine != 0u;
+  const uint32_t qmax = fine ? QUERY_FINE_MORTON_MAX : QUERY_MORTON_MAX;
+  const uint32_t lshift = fine ? QUERY_FINE_LEVEL_SHIFT : QUERY_LEVEL_SHIFT;
+  const uint32_t kbits = fine ? QUERY_FINE_KEY_BITS : QUERY_KEY_BITS;
+  const int first = (int)((kbits / 8u) & 1u);
   hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, st, sc.bmin,
-                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, eb.qkeys[first], eb.qvals[first]);
+                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, lshift, eb.qkeys[first], eb.qvals[first]);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint32_t* qk[2] = {eb.qkeys[first], eb.qkeys[first ^ 1]};
   uint32_t* qv[2] = {eb.qvals[first], eb.qvals[first ^ 1]};
   int cur = 0;
-  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, QUERY_KEY_BITS, &cur);
+  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, kbits, &cur);
   if (e == hipSuccess && (cur ^ first) != 0) e = hipErrorInvalidValue;  // the sorted order is read from qvals[0]
   return e;
 }
@@ -2463,7 +2467,8 @@ __global__ __launch_bounds__(256) void kb_tile_count(const BatchItem* items, con
 __global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                      uint32_t n_all) {
   BATCH_PROLOGUE(BK_QKEYS)
-  k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, (float)QUERY_MORTON_MAX,
+  // batches sort on the coarse keys whatever their rooms' sizes
+  k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, (float)QUERY_MORTON_MAX, QUERY_LEVEL_SHIFT,
                     QUERY_KEYS_FIRST != 0u ? it.eb.qkeys[1] : it.eb.qkeys[0],
                     QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], blk);
 }

@@ -96,3 +96,27 @@ def test_batch_mixed_bvh_and_tuning():
         assert _same(m, plain[i][0]), i
     b.close()
     b2.close()
+
+
+def test_fine_query_keys_match_batch():
+    """A session of at least 2^20 pixels sorts its queries on fine (10-bit per
+    axis) Morton keys, a batch on coarse (7-bit) ones whatever the size; the
+    order is only the gather's, so the images agree bit for bit (outside
+    overflowed pixels)."""
+    say("fine query keys: session == batch")
+    c = ctx()
+    sc = P.caustic_scene(1024, 1024)
+    cfg = dict(seed=5, n_photon=50000, radius0=0.02, alpha=2 / 3, max_depth=6)
+    ds = P.DeviceScene(c, sc)
+    s = P.Session(c, ds, _cfg(cfg))
+    s.run(2)
+    m0, v0, k0 = s.image()
+    s.close()
+    b = P.Batch(c, [ds], [_cfg(cfg)], [2])
+    b.run()
+    m1, v1, k1 = b.image(0)
+    b.close()
+    assert k0 == k1 == 2
+    assert m0.mean() > 0
+    assert _same(m1, m0)
+    assert np.allclose(v1, v0, rtol=1e-5, atol=1e-12)
