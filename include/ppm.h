@@ -275,6 +275,21 @@ void        ppm_map_destroy(ppm_map* map);
 ppm_status  ppm_map_build(ppm_ctx* ctx, const ppm_scene* scene,
                           const ppm_photons* ph, double power, ppm_map* map);
 
+/* build_photonmap (photonmap.rs:23-29) in the form the PPM-PA passes build
+ * it every pass (ppm_session_run): every query of a pass has one radius, so
+ * the photons are binned into cubic cells of side >= 2.1 sqrt(r2), hashed
+ * into a power-of-two bucket table and sorted by bucket.  ppm_radius_query
+ * and ppm_gather on such a map run the passes' own code: the cell walk of
+ * the shade kernel for the index sets, and the shade kernel itself (with
+ * the direct light off, scale power / (pi r2) applied in float) for the
+ * radiance; ppm_trace_eye shades through it as a pass does.  Their r2 must
+ * not exceed the r2 given here, and ppm_gather on a grid takes refine64 = 0
+ * (the passes decide membership in float).  scene may be NULL, as for
+ * ppm_map_build. */
+ppm_status  ppm_map_build_grid(ppm_ctx* ctx, const ppm_scene* scene,
+                               const ppm_photons* ph, double power, double r2,
+                               ppm_map* map);
+
 /* KdTree::within(q, r2, squared_euclidean) (tracer.rs:180): for each of the
  * nq query points (nq x 3 doubles) the indices (into the photon set as it
  * was given to ppm_map_build) of the photons with squared distance <= r2.

@@ -638,21 +638,42 @@ static Flt filter_gauss(Flt d, Flt rmax) {
   return e_r > E_BETA ? 0.0 : ALPHA * (1.0 - e_r / E_BETA) + CORR;
 }
 
+// One photon's term of estimate_radiance (tracer.rs:185-192) with
+// photon_to_radiance (optics.rs:224-233), added into its channel.
+static inline void radiance_term(const Kd& kd, const Photon& p, Flt d2, Flt r2, int pf, V n, C& rad) {
+  Flt wt = pf == PPM_FILTER_CONE ? filter_cone(d2, r2) : (pf == PPM_FILTER_GAUSS ? filter_gauss(d2, r2) : 1.0);
+  Flt pw = wt * kd.power;
+  Flt cos0 = dot(n, p.dir);
+  Flt pw2 = cos0 < 0.0 ? pw * -cos0 * p.pw.w : 0.0;
+  if (p.wl == 0) rad.r = rad.r + pw2;
+  else if (p.wl == 1) rad.g = rad.g + pw2;
+  else rad.b = rad.b + pw2;
+}
+
 // estimate_radiance (tracer.rs:179-195) with photon_to_radiance (optics.rs:224-233)
 static C estimate_radiance(const Kd& kd, Flt r2, int pf, V pos, V n, std::vector<std::pair<Flt, int>>& buf) {
   kd_within(kd, pos, r2, buf);
   if (buf.empty()) return C{0, 0, 0};
   C rad{0, 0, 0};
-  for (auto& e : buf) {
-    Flt wt = pf == PPM_FILTER_CONE ? filter_cone(e.first, r2) : (pf == PPM_FILTER_GAUSS ? filter_gauss(e.first, r2) : 1.0);
-    const Photon& p = kd.ph[e.second];
-    Flt pw = wt * kd.power;
-    Flt cos0 = dot(n, p.dir);
-    Flt pw2 = cos0 < 0.0 ? pw * -cos0 * p.pw.w : 0.0;
-    if (p.wl == 0) rad.r = rad.r + pw2;
-    else if (p.wl == 1) rad.g = rad.g + pw2;
-    else rad.b = rad.b + pw2;
+  for (auto& e : buf) radiance_term(kd, kd.ph[e.second], e.first, r2, pf, n, rad);
+  return rad * (ONE_PI / r2);
+}
+
+// The same estimate over the photons that the float predicate of the GPU
+// passes keeps, fl32((dx^2 + dy^2) + dz^2) <= fl32(r2) on float coordinates
+// (every photon tested; ascending distance, then index, as kd_within
+// orders its result), each term still formed in double.
+static C estimate_radiance_f32set(const Kd& kd, Flt r2, int pf, V pos, V n, std::vector<std::pair<Flt, int>>& buf) {
+  buf.clear();
+  const float qx = (float)pos.x, qy = (float)pos.y, qz = (float)pos.z, r2f = (float)r2;
+  for (size_t j = 0; j < kd.ph.size(); ++j) {
+    const Photon& p = kd.ph[j];
+    const float d = sq_euclid_f(qx, qy, qz, (float)p.pos.x, (float)p.pos.y, (float)p.pos.z);
+    if (d <= r2f) buf.push_back({sq_euclid(pos, p.pos), (int)j});
   }
+  std::sort(buf.begin(), buf.end());
+  C rad{0, 0, 0};
+  for (auto& e : buf) radiance_term(kd, kd.ph[e.second], e.first, r2, pf, n, rad);
   return rad * (ONE_PI / r2);
 }
 
@@ -1112,6 +1133,21 @@ void orc_gather(const orc_map* m, uint64_t nq, const double* q, const double* nr
 #pragma omp for schedule(dynamic, 64)
     for (int64_t k = 0; k < (int64_t)nq; ++k) {
       C c = estimate_radiance(m->kd, r2, pf, vd(q + 3 * k), vd(nrm + 3 * k), buf);
+      rgb[3 * k] = c.r; rgb[3 * k + 1] = c.g; rgb[3 * k + 2] = c.b;
+    }
+  }
+}
+
+// estimate_radiance for nq (position, normal) pairs over the float-decided
+// ball of the GPU passes (estimate_radiance_f32set).
+void orc_gather_f32set(const orc_map* m, uint64_t nq, const double* q, const double* nrm, double r2, int pf,
+                       double* rgb) {
+#pragma omp parallel
+  {
+    std::vector<std::pair<Flt, int>> buf;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t k = 0; k < (int64_t)nq; ++k) {
+      C c = estimate_radiance_f32set(m->kd, r2, pf, vd(q + 3 * k), vd(nrm + 3 * k), buf);
       rgb[3 * k] = c.r; rgb[3 * k + 1] = c.g; rgb[3 * k + 2] = c.b;
     }
   }

@@ -59,6 +59,12 @@ struct ppm_map {
   double power = 1.0;
   bool has64 = false;
   bool built = false;
+  // built by ppm_map_build_grid: the passes' hashed grid for radius^2
+  // grid_r2 over the box [gbmin, gbmax] (cells allocated on first use)
+  bool grid = false;
+  double grid_r2 = 0.0;
+  float4 gbmin{}, gbmax{};
+  DevMaterial* hook_mat = nullptr;  // M_AIR alone: the scene of the shade kernel run by ppm_gather
 };
 
 // One pass pipeline of a session: the buffers, pass state, FP64 sums and
@@ -295,6 +301,24 @@ PassState host_state() {
 }
 
 }  // namespace
+
+// The map's buffers as a box tree (no grid cells, so the gather launch picks
+// the tree kernels) or as the passes' grid (no tree nodes).
+static MapBuf tree_view(const ppm_map* m) {
+  MapBuf b = m->m;
+  b.cells = nullptr;
+  b.cells_max = 0;
+  return b;
+}
+static MapBuf grid_view_of(const ppm_map* m) {
+  MapBuf b = m->m;
+  b.nodes = nullptr;
+  b.lpad_max = 0;
+  return b;
+}
+static MapBuf map_view(const ppm_map* m) { return m->grid ? grid_view_of(m) : tree_view(m); }
+
+namespace {}  // namespace
 
 extern "C" {
 
@@ -1011,6 +1035,7 @@ void ppm_map_destroy(ppm_map* m) {
   if (m->ctx) (void)hipStreamSynchronize(m->ctx->stream);
   map_free(m->m);
   dfree(m->st);
+  dfree(m->hook_mat);
   delete m;
 }
 
@@ -1024,12 +1049,63 @@ ppm_status ppm_map_build(ppm_ctx* c, const ppm_scene* sc, const ppm_photons* ph,
   HIP_TRY(c, hipMemcpyAsync(m->st, ph->st, sizeof(PassState), hipMemcpyDeviceToDevice, c->stream));
   PhotonBuf in = ph->pb;
   if (!ph->has64) in.rec64 = nullptr;
-  HIP_TRY(c, launch_map_build(c->stream, m->m, m->st, in, f4(lo), f4(hi)));
+  m->built = false;
+  m->grid = false;
+  MapBuf tv = tree_view(m);
+  HIP_TRY(c, launch_map_build(c->stream, tv, m->st, in, f4(lo), f4(hi)));
   PassState h;
   HIP_TRY(c, hipMemcpyAsync(&h, m->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   m->power = power;
   m->has64 = ph->has64;
+  m->built = true;
+  if (h.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
+  return PPM_OK;
+}
+
+ppm_status ppm_map_build_grid(ppm_ctx* c, const ppm_scene* sc, const ppm_photons* ph, double power, double r2,
+                              ppm_map* m) {
+  if (!c || !ph || !m) return fail(c, PPM_E_INVALID, "null argument");
+  if (ph->cap > m->m.cap) return fail(c, PPM_E_RANGE, "map capacity smaller than the photon set's");
+  if (!(r2 > 0.0) || !std::isfinite(r2) || !std::isfinite((float)r2)) return fail(c, PPM_E_INVALID, "bad radius");
+  const double* lo = sc ? sc->bmin : ph->bmin;
+  const double* hi = sc ? sc->bmax : ph->bmax;
+  if (!sc && !ph->has_bounds) return fail(c, PPM_E_INVALID, "no bounds for the grid");
+  m->built = false;
+  if (!m->m.cells) {
+    m->m.cells_max = (uint32_t)std::min<uint64_t>(GRID_MAX_BUCKETS, next_pow2(std::max<uint64_t>(m->m.cap, 1024)));
+    HIP_TRY(c, dalloc(&m->m.cells, m->m.cells_max));
+  }
+  if (!m->hook_mat) {
+    DevMaterial air;
+    std::memset(&air, 0, sizeof(air));
+    HIP_TRY(c, dalloc(&m->hook_mat, 1));
+    HIP_TRY(c, hipMemcpy(m->hook_mat, &air, sizeof(air), hipMemcpyHostToDevice));
+  }
+  if (ph->has64 && !m->m.sorted64) HIP_TRY(c, dalloc(&m->m.sorted64, 3 * m->m.cap));
+  // the pass state of a pass with this radius (k_grid_setup sizes the cells from r2f)
+  PassState hs;
+  HIP_TRY(c, hipMemcpyAsync(&hs, ph->st, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  hs.r2 = r2;
+  hs.r2f = (float)r2;
+  hs.power = power;
+  hs.overflow = 0;
+  hs.nonfinite = 0;
+  HIP_TRY(c, hipMemcpy(m->st, &hs, sizeof(hs), hipMemcpyHostToDevice));
+  PhotonBuf in = ph->pb;
+  if (!ph->has64) in.rec64 = nullptr;
+  m->gbmin = f4(lo);
+  m->gbmax = f4(hi);
+  MapBuf g = grid_view_of(m);
+  HIP_TRY(c, launch_grid_build(c->stream, g, m->st, in, m->gbmin, m->gbmax));
+  PassState h;
+  HIP_TRY(c, hipMemcpyAsync(&h, m->st, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  m->power = power;
+  m->has64 = ph->has64;
+  m->grid = true;
+  m->grid_r2 = r2;
   m->built = true;
   if (h.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
   return PPM_OK;
@@ -1051,11 +1127,11 @@ static ppm_status run_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, c
   }
   float4 *dqf = nullptr, *dqn = nullptr, *drgb = nullptr;
   double* dq64 = nullptr;
-  uint32_t *dcnt = nullptr, *didx = nullptr;
+  uint32_t *dcnt = nullptr, *didx = nullptr, *dflag = nullptr, hflag = 0;
   uint64_t* doff = nullptr;
   hipError_t e = hipSuccess;
   auto cleanup = [&]() {
-    dfree(dqf); dfree(dqn); dfree(drgb); dfree(dq64); dfree(dcnt); dfree(didx); dfree(doff);
+    dfree(dqf); dfree(dqn); dfree(drgb); dfree(dq64); dfree(dcnt); dfree(didx); dfree(doff); dfree(dflag);
   };
   do {
     if ((e = dalloc(&dqf, nq)) != hipSuccess) break;
@@ -1075,9 +1151,17 @@ static ppm_status run_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, c
       if ((e = dalloc(&didx, idx_total)) != hipSuccess) break;
     }
     if (mode == 2 && (e = dalloc(&drgb, nq)) != hipSuccess) break;
-    if ((e = launch_query(c->stream, m->m, m->st, dqf, dq64, dqn, (uint32_t)nq, r2, refine64, mode, pfilter, dcnt,
-                          doff, didx, drgb, c->n_cu)) != hipSuccess)
+    if (m->grid) {
+      if ((e = dalloc(&dflag, 1)) != hipSuccess) break;
+      if ((e = hipMemsetAsync(dflag, 0, sizeof(uint32_t), c->stream)) != hipSuccess) break;
+      if ((e = launch_grid_query(c->stream, grid_view_of(m), m->st, m->gbmin, dqf, dq64, (uint32_t)nq, r2, refine64,
+                                 mode, dcnt, doff, didx, dflag)) != hipSuccess)
+        break;
+      if ((e = hipMemcpyAsync(&hflag, dflag, sizeof(hflag), hipMemcpyDeviceToHost, c->stream)) != hipSuccess) break;
+    } else if ((e = launch_query(c->stream, tree_view(m), m->st, dqf, dq64, dqn, (uint32_t)nq, r2, refine64, mode,
+                                 pfilter, dcnt, doff, didx, drgb, c->n_cu)) != hipSuccess) {
       break;
+    }
     if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
     if (mode == 0) {
       counts->resize(nq);
@@ -1091,6 +1175,72 @@ static ppm_status run_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, c
   } while (0);
   cleanup();
   if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  if (hflag) return fail(c, PPM_E_RANGE, "a query's cells exceed the grid's span");
+  return PPM_OK;
+}
+
+// ppm_gather on a grid map: the queries run through the passes' shade
+// kernel (direct light off), in the given order, with the pass scale
+// power / (pi r2) applied in float as in a pass.
+static ppm_status run_grid_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm,
+                                  uint64_t nq, double r2, int pfilter, double* rgb_out) {
+  std::vector<Query> hq(nq);
+  std::vector<uint32_t> order(nq);
+  for (uint64_t i = 0; i < nq; ++i) {
+    if (!finite3(q_xyz + 3 * i) || !finite3(q_nrm + 3 * i)) return fail(c, PPM_E_NONFINITE, "non-finite query");
+    hq[i].a = make_float4((float)q_xyz[3 * i], (float)q_xyz[3 * i + 1], (float)q_xyz[3 * i + 2],
+                          __builtin_bit_cast(float, (uint32_t)i));
+    hq[i].b = make_float4((float)q_nrm[3 * i], (float)q_nrm[3 * i + 1], (float)q_nrm[3 * i + 2],
+                          __builtin_bit_cast(float, 0u));
+    hq[i].w = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    order[i] = (uint32_t)i;
+  }
+  PassState hs;
+  HIP_TRY(c, hipMemcpyAsync(&hs, m->st, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  hs.r2 = r2;
+  hs.r2f = (float)r2;
+  hs.scale = (float)(m->power / (M_PI * r2));
+  hs.n_qsort = (uint32_t)nq;
+  hs.overflow = 0;
+  std::memset(hs.work, 0, sizeof(hs.work));
+  EyeBuf eb;
+  PassState* st = nullptr;
+  uint32_t* dorder = nullptr;
+  std::vector<float4> res(nq);
+  hipError_t e = hipSuccess;
+  do {
+    if ((e = dalloc(&st, 1)) != hipSuccess) break;
+    if ((e = dalloc(&eb.q, nq)) != hipSuccess) break;
+    if ((e = dalloc(&eb.res, nq)) != hipSuccess) break;
+    if ((e = dalloc(&dorder, nq)) != hipSuccess) break;
+    if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((e = hipMemcpy(eb.q, hq.data(), sizeof(Query) * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+    if ((e = hipMemcpy(dorder, order.data(), sizeof(uint32_t) * nq, hipMemcpyHostToDevice)) != hipSuccess) break;
+    SceneBuf sb;
+    sb.mats = m->hook_mat;
+    sb.n_mats = 0;
+    sb.bmin = m->gbmin;
+    sb.bmax = m->gbmax;
+    if ((e = launch_grid_shade(c->stream, sb, grid_view_of(m), st, eb, dorder, pfilter, c->n_cu)) != hipSuccess)
+      break;
+    if ((e = hipMemcpyAsync(res.data(), eb.res, sizeof(float4) * nq, hipMemcpyDeviceToHost, c->stream)) !=
+        hipSuccess)
+      break;
+    if ((e = hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, c->stream)) != hipSuccess) break;
+    e = hipStreamSynchronize(c->stream);
+  } while (0);
+  dfree(st);
+  dfree(eb.q);
+  dfree(eb.res);
+  dfree(dorder);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  if (hs.overflow) return fail(c, PPM_E_RANGE, "a query's cells exceed the grid's span");
+  for (uint64_t i = 0; i < nq; ++i) {
+    rgb_out[3 * i] = res[i].x;
+    rgb_out[3 * i + 1] = res[i].y;
+    rgb_out[3 * i + 2] = res[i].z;
+  }
   return PPM_OK;
 }
 
@@ -1100,6 +1250,7 @@ ppm_status ppm_radius_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, u
   if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
   if (refine64 && !m->has64) return fail(c, PPM_E_INVALID, "refine64 needs a set loaded by ppm_photons_upload");
   if (!(r2 >= 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
+  if (m->grid && r2 > m->grid_r2) return fail(c, PPM_E_INVALID, "r2 exceeds the radius the grid was built for");
   if (nq > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "too many queries");
   offsets_out[0] = 0;
   if (nq == 0) return PPM_OK;
@@ -1130,7 +1281,12 @@ ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const d
   if (!(r2 > 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
   if (pfilter < 0 || pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
   if (nq > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "too many queries");
+  if (m->grid) {
+    if (refine64) return fail(c, PPM_E_INVALID, "the grid's gather decides membership in float (refine64 = 0)");
+    if (r2 > m->grid_r2) return fail(c, PPM_E_INVALID, "r2 exceeds the radius the grid was built for");
+  }
   if (nq == 0) return PPM_OK;
+  if (m->grid) return run_grid_gather(c, m, q_xyz, q_nrm, nq, r2, pfilter, rgb_out);
   std::vector<float4> rgb;
   if (ppm_status s = run_query(c, m, q_xyz, q_nrm, nq, r2, refine64, 2, pfilter, nullptr, nullptr, nullptr, 0, &rgb))
     return s;
@@ -1151,6 +1307,7 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
   if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
   if (!(r2 > 0.0) || !std::isfinite(r2)) return fail(c, PPM_E_INVALID, "bad radius");
   if (pfilter < 0 || pfilter > 2) return fail(c, PPM_E_INVALID, "bad photon filter");
+  if (m->grid && r2 > m->grid_r2) return fail(c, PPM_E_INVALID, "r2 exceeds the radius the grid was built for");
   EyeBuf eb;
   PassState* st = nullptr;
   hipError_t e = hipSuccess;
@@ -1172,7 +1329,13 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
     std::memset(hs.work, 0, sizeof(hs.work));
     if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
     if ((e = launch_eye(c->stream, sc->buf, st, eb, seed, use_classic, max_depth, c->n_cu)) != hipSuccess) break;
-    if ((e = launch_gather(c->stream, sc->buf, m->m, st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=
+    // a grid's cells are indexed from the box it was built over
+    SceneBuf sb = sc->buf;
+    if (m->grid) {
+      sb.bmin = m->gbmin;
+      sb.bmax = m->gbmax;
+    }
+    if ((e = launch_gather(c->stream, sb, map_view(m), st, eb, pfilter, use_classic, c->n_cu, nullptr, nullptr)) !=
         hipSuccess)
       break;
     h.resize(eb.n_pix);

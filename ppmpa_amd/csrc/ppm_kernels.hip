@@ -1115,7 +1115,7 @@ __device__ __forceinline__ void grid_range_scan(const PhotonRec* ph, uint2 se, f
   }
 }
 
-// The same scan calling on_photon(pm, dm, d2, inside) for every photon of
+// The same scan calling on_photon(j, pm, dm, d2, inside) for every photon of
 // the range, inside = (d2 <= r2): for a callback whose work is a select
 // (the shade's sum adds 0 for photons outside), with no branch per photon
 // and so none of the exec-mask work that kept the scalar units half busy.
@@ -1129,15 +1129,15 @@ __device__ __forceinline__ void grid_range_scan_all(const PhotonRec* ph, uint2 s
     const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
     const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
     const float d3 = dist2_nc(qx, qy, qz, p3.pm.x, p3.pm.y, p3.pm.z);
-    on_photon(p0.pm, p0.dm, d0, d0 <= r2);
-    on_photon(p1.pm, p1.dm, d1, d1 <= r2);
-    on_photon(p2.pm, p2.dm, d2, d2 <= r2);
-    on_photon(p3.pm, p3.dm, d3, d3 <= r2);
+    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2);
+    on_photon(j + 1, p1.pm, p1.dm, d1, d1 <= r2);
+    on_photon(j + 2, p2.pm, p2.dm, d2, d2 <= r2);
+    on_photon(j + 3, p3.pm, p3.dm, d3, d3 <= r2);
   }
   for (; j < se.y; ++j) {
     const PhotonRec p0 = ph[j];
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
-    on_photon(p0.pm, p0.dm, d0, d0 <= r2);
+    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2);
   }
 }
 
@@ -1150,7 +1150,7 @@ __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const Photo
 // Returns false (and scans only the first three cells of an axis) if the
 // query's cells span more than three per axis, which the cell side rules out
 // for every finite point (the caller raises a flag).
-// ALL = true: on_photon(pm, dm, d2, inside) for every photon of the cells
+// ALL = true: on_photon(j, pm, dm, d2, inside) for every photon of the cells
 // (grid_range_scan_all), in the same order.
 template <bool ALL = false, typename OnPhoton>
 __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells, const PhotonRec* ph, float qx,
@@ -1796,7 +1796,8 @@ __device__ __forceinline__ bool shade_gather(const GridView& g, const uint2* cel
                                              float r2, float inv_r2, int pf, float& ar, float& ag, float& ab) {
   const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
   const int f = PF >= 0 ? PF : pf;
-  return grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2, [&](float4 pm, float4 dm, float d2, bool in) {
+  return grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
+                          [&](uint32_t, float4 pm, float4 dm, float d2, bool in) {
     const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
     const float v = (in && c < 0.0f) ? -c * filter_weight(f, d2 * inv_r2) * dm.w : 0.0f;
     const uint32_t wl = __float_as_uint(pm.w) & 3u;
@@ -2270,6 +2271,71 @@ hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const flo
   if (blocks == 0) blocks = 1;
   hipLaunchKernelGGL(k_query, dim3(blocks), dim3(256), 0, s, m.nodes, m.sorted, m.sorted64, m.src_idx, st, qf, q64,
                      qn, nq, r2, refine64, mode, pfilter, counts, offsets, idx_out, rgb_out);
+  return hipGetLastError();
+}
+
+// The same index sets over a pass's photon grid (a map built by
+// launch_grid_build for a radius >= sqrt(r2)): one query per thread through
+// grid_query, the cell walk of the shade kernel's gather, which calls back
+// for every photon of the query's cells in the shade's order.  Membership is
+// the shade's float predicate, or, when refining, decided again in double
+// near the sphere (the cells cover the double ball as well: the grid's reach
+// exceeds r by r/1024 plus the rounding of float coordinates).  flags bit 0:
+// a query whose cells exceeded the grid's span rule.
+__global__ __launch_bounds__(256) void k_grid_query(const uint2* cells, const PhotonRec* ph, const double* ph64,
+                                                    const uint32_t* src_idx, const PassState* st, float4 bmin,
+                                                    const float4* qf, const double* q64, uint32_t nq, double r2,
+                                                    int refine, int mode, uint32_t* counts, const uint64_t* offsets,
+                                                    uint32_t* idx_out, uint32_t* flags) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const GridView g = grid_view(st, bmin);
+  const float4 qp = qf[i];
+  const float r2f = (float)r2;
+  double qx = 0, qy = 0, qz = 0;
+  float band = 0.0f;
+  if (refine) {
+    qx = q64[3 * (uint64_t)i];
+    qy = q64[3 * (uint64_t)i + 1];
+    qz = q64[3 * (uint64_t)i + 2];
+    const double sc = fmax(fmax(fabs(qx), fabs(qy)), fabs(qz)) + sqrt(r2);
+    band = (float)(64.0 * 5.9604644775390625e-8 * (sc * sqrt(r2) + r2) + 1e-30);
+  }
+  uint32_t cnt = 0;
+  const uint64_t off = mode == 1 ? offsets[i] : 0;
+  const bool ok = grid_query<true>(g, cells, ph, qp.x, qp.y, qp.z, r2f,
+                                   [&](uint32_t j, float4, float4, float d2, bool in) {
+                                     float t;
+                                     const bool keep =
+                                         refine ? in_ball(d2, r2f, band, true, ph64, j, qx, qy, qz, r2, &t) : in;
+                                     if (keep) {
+                                       if (mode == 1) idx_out[off + cnt] = src_idx[j];
+                                       ++cnt;
+                                     }
+                                   });
+  if (!ok) atomicOr(flags, 1u);
+  if (mode == 0) counts[i] = cnt;
+}
+
+hipError_t launch_grid_query(hipStream_t s, const MapBuf& m, const PassState* st, float4 bmin, const float4* qf,
+                             const double* q64, uint32_t nq, double r2, int refine64, int mode, uint32_t* counts,
+                             const uint64_t* offsets, uint32_t* idx_out, uint32_t* flags) {
+  if (nq == 0) return hipSuccess;
+  if (!m.cells || (refine64 && !m.sorted64)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_grid_query, dim3((nq + 255) / 256), dim3(256), 0, s, m.cells, m.sorted, m.sorted64,
+                     m.src_idx, st, bmin, qf, q64, nq, r2, refine64, mode, counts, offsets, idx_out, flags);
+  return hipGetLastError();
+}
+
+// The gather of host-given queries through the shade kernel itself
+// (k_shade, direct light off): eb.q holds the queries (position, normal,
+// weight), order their order, st the pass state of the grid with r2f,
+// scale and n_qsort set and the work counters zeroed; eb.res receives
+// w * scale * Σ filter * max(0, -n.dir) * weight per channel.
+hipError_t launch_grid_shade(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
+                             const uint32_t* order, int pfilter, int n_cu) {
+  if (!m.cells || !sc.mats || sc.bvh) return hipErrorInvalidValue;
+  launch_shade<0>(s, (uint32_t)n_cu * 8u, 0, make_view(sc), m.cells, m.sorted, st, eb, order, pfilter, 0);
   return hipGetLastError();
 }
 

@@ -189,6 +189,19 @@ hipError_t launch_query(hipStream_t s, const MapBuf& m, PassState* st, const flo
                         const float4* qn, uint32_t nq, double r2, int refine64, int mode, int pfilter,
                         uint32_t* counts, const uint64_t* offsets, uint32_t* idx_out, float4* rgb_out, int n_cu);
 
+// The same hooks over a pass's photon grid (launch_grid_build for a radius
+// >= sqrt(r2)): index sets (mode 0 count, 1 list) through the shade
+// kernel's cell walk, flags bit 0 set for a query beyond the grid's span;
+// and the gather through the shade kernel itself with the direct light off
+// (sc: a scene with no BVH whose bmin is the grid's; eb.q/eb.res the
+// queries and results, order their order, st with r2f, scale, n_qsort set
+// and the work counters zeroed).
+hipError_t launch_grid_query(hipStream_t s, const MapBuf& m, const PassState* st, float4 bmin, const float4* qf,
+                             const double* q64, uint32_t nq, double r2, int refine64, int mode, uint32_t* counts,
+                             const uint64_t* offsets, uint32_t* idx_out, uint32_t* flags);
+hipError_t launch_grid_shade(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
+                             const uint32_t* order, int pfilter, int n_cu);
+
 // Scene BVH over the n shapes shapes[ids[i]]: Morton keys of the centroids
 // (cmin, cscale = 1023 / extent), a radix sort (count *np on the device),
 // leaves of `leaf` shapes copied into bprims, levels folded up.

@@ -207,6 +207,16 @@ class PhotonMap:
                                                   self.power, self.h))
         return self
 
+    def build_grid(self, photons, power, r2, dscene=None):
+        """The map as the PPM-PA passes build it for one radius (hashed cells
+        of side >= 2.1 sqrt(r2)): within() and estimate_radiance() then run
+        the passes' own cell walk and shade kernel, for any r2 up to this
+        one (ppm_map_build_grid)."""
+        self.power = float(power)
+        self.ctx.check(self.ctx.lib.ppm_map_build_grid(self.ctx.h, dscene.h if dscene is not None else None,
+                                                       photons.h, self.power, float(r2), self.h))
+        return self
+
     def within(self, q, r2, refine64=False):
         """KdTree::within(q, r2, squared_euclidean) for each query point:
         a list of ascending index arrays."""

@@ -47,6 +47,7 @@ _PROTOS = {
     "orc_map_destroy": (None, [vp]),
     "orc_map_within": (C.c_int64, [vp, dp, C.c_double, C.c_uint64, u32p, dp]),
     "orc_gather": (None, [vp, C.c_uint64, dp, dp, C.c_double, C.c_int, dp]),
+    "orc_gather_f32set": (None, [vp, C.c_uint64, dp, dp, C.c_double, C.c_int, dp]),
     "orc_trace_eye": (None, [vp, vp, C.c_uint64, C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
     "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, dp]),
     "orc_run_passes": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, C.c_int, dp, dp]),
@@ -159,6 +160,7 @@ class OMap:
         self.pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
         self.dir = np.ascontiguousarray(dir, dtype=np.float64).reshape(-1, 3)
         self.wl = np.ascontiguousarray(wl, dtype=np.uint8)
+        self.power = float(power)
         self.h = lib().orc_map_build(self.pos.shape[0], _d(self.pos), _d(self.dir), self.wl.ctypes.data_as(u8p),
                                      float(power))
 
@@ -175,11 +177,14 @@ class OMap:
         n = lib().orc_map_within(self.h, _d(arr3(q)), float(r2), cap, idx.ctypes.data_as(u32p), _d(d2))
         return idx[:n], d2[:n]
 
-    def gather(self, q, nrm, r2, pf=0):
+    def gather(self, q, nrm, r2, pf=0, f32set=False):
+        """estimate_radiance in FP64 over the FP64 ball, or (f32set) over the
+        ball that the GPU passes' float predicate decides."""
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
         nrm = np.ascontiguousarray(nrm, dtype=np.float64).reshape(-1, 3)
         out = np.zeros_like(q)
-        lib().orc_gather(self.h, q.shape[0], _d(q), _d(nrm), float(r2), int(pf), _d(out))
+        f = lib().orc_gather_f32set if f32set else lib().orc_gather
+        f(self.h, q.shape[0], _d(q), _d(nrm), float(r2), int(pf), _d(out))
         return out
 
 
