@@ -311,6 +311,20 @@ ppm_status  ppm_gather(ppm_ctx* ctx, const ppm_map* map, const double* q_xyz,
                        const double* q_nrm, uint64_t nq, double r2,
                        int32_t pfilter, int32_t refine64, double* rgb_out);
 
+/* ppm_gather for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid allocation).
+ * No counterpart in the reference.  Same arguments, checks and results as
+ * ppm_gather on either map form (bit for bit), for nq <= the bound below
+ * (PPM_E_RANGE beyond it).  It allocates nothing: queries, results and the
+ * gather's state live in a fixed area of the library in device memory
+ * (under 9 MiB, one per GPU, held until the process ends), so calls that
+ * overlap on one GPU, from any contexts or threads, race.  Synchronises. */
+#define PPM_NOALLOC_MAX_QUERIES 65536
+ppm_status  ppm_gather_noalloc(ppm_ctx* ctx, const ppm_map* map,
+                               const double* q_xyz, const double* q_nrm,
+                               uint64_t nq, double r2, int32_t pfilter,
+                               int32_t refine64, double* rgb_out);
+
 /* ---- eye pass (H7) ------------------------------------------------------ */
 
 /* One pass of per-pixel trace_ray (tracer.rs:129-177) over the camera's

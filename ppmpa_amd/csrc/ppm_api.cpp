@@ -346,6 +346,8 @@ ppm_status ppm_create(ppm_ctx** out, int device, void* hip_stream) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return PPM_E_NODEVICE;
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return PPM_E_NODEVICE;
+  // the device address of this GPU's area for ppm_gather_noalloc (once per device)
+  if (noalloc_init(device) != hipSuccess) return PPM_E_HIP;
   ppm_ctx* c = new ppm_ctx();
   c->device = device;
   c->n_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
@@ -1273,8 +1275,9 @@ ppm_status ppm_radius_query(ppm_ctx* c, const ppm_map* m, const double* q_xyz, u
   return PPM_OK;
 }
 
-ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq, double r2,
-                      int32_t pfilter, int32_t refine64, double* rgb_out) {
+// The argument checks of ppm_gather (and ppm_gather_noalloc).
+static ppm_status check_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq,
+                               double r2, int32_t pfilter, int32_t refine64, double* rgb_out) {
   if (!c || !m || (nq && (!q_xyz || !q_nrm || !rgb_out))) return fail(c, PPM_E_INVALID, "null argument");
   if (!m->built) return fail(c, PPM_E_INVALID, "map not built");
   if (refine64 && !m->has64) return fail(c, PPM_E_INVALID, "refine64 needs a set loaded by ppm_photons_upload");
@@ -1285,6 +1288,12 @@ ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const d
     if (refine64) return fail(c, PPM_E_INVALID, "the grid's gather decides membership in float (refine64 = 0)");
     if (r2 > m->grid_r2) return fail(c, PPM_E_INVALID, "r2 exceeds the radius the grid was built for");
   }
+  return PPM_OK;
+}
+
+ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq, double r2,
+                      int32_t pfilter, int32_t refine64, double* rgb_out) {
+  if (ppm_status s = check_gather(c, m, q_xyz, q_nrm, nq, r2, pfilter, refine64, rgb_out)) return s;
   if (nq == 0) return PPM_OK;
   if (m->grid) return run_grid_gather(c, m, q_xyz, q_nrm, nq, r2, pfilter, rgb_out);
   std::vector<float4> rgb;
@@ -1297,6 +1306,35 @@ ppm_status ppm_gather(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const d
     rgb_out[3 * i + 1] = (double)rgb[i].y * k;
     rgb_out[3 * i + 2] = (double)rgb[i].z * k;
   }
+  return PPM_OK;
+}
+
+static_assert(PPM_NOALLOC_MAX_QUERIES == NOALLOC_MAX_QUERIES, "ppm.h and the kernels disagree on the bound");
+
+ppm_status ppm_gather_noalloc(ppm_ctx* c, const ppm_map* m, const double* q_xyz, const double* q_nrm, uint64_t nq,
+                              double r2, int32_t pfilter, int32_t refine64, double* rgb_out) {
+  if (ppm_status s = check_gather(c, m, q_xyz, q_nrm, nq, r2, pfilter, refine64, rgb_out)) return s;
+  if (nq > PPM_NOALLOC_MAX_QUERIES)
+    return fail(c, PPM_E_RANGE, "nq exceeds PPM_NOALLOC_MAX_QUERIES (" + std::to_string(PPM_NOALLOC_MAX_QUERIES) + ")");
+  if (nq == 0) return PPM_OK;
+  for (uint64_t i = 0; i < nq; ++i) {
+    if (!finite3(q_xyz + 3 * i)) return fail(c, PPM_E_NONFINITE, "non-finite query point");
+    if (m->grid && !finite3(q_nrm + 3 * i)) return fail(c, PPM_E_NONFINITE, "non-finite query");
+  }
+  // the same scale factors as ppm_gather: the tree's sums times power/pi/r2
+  // in double, the grid's scaled in float by the shade kernel
+  const double k = m->power * (1.0 / M_PI) / r2;
+  const float scale = (float)(m->power / (M_PI * r2));
+  SceneBuf sb;
+  sb.mats = m->hook_mat;
+  sb.n_mats = 0;
+  sb.bmin = m->gbmin;
+  sb.bmax = m->gbmax;
+  uint32_t overflow = 0;
+  HIP_TRY(c, launch_gather_noalloc(c->stream, c->device, map_view(m), m->st, m->grid, sb, q_xyz, q_nrm,
+                                   (uint32_t)nq, r2, k, scale, refine64, pfilter, c->n_cu, rgb_out, &overflow));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (overflow) return fail(c, PPM_E_RANGE, "a query's cells exceed the grid's span");
   return PPM_OK;
 }
 

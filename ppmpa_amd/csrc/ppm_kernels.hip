@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "ppm_dev.hpp"
 #include "ppm_launch.hpp"
@@ -2630,6 +2631,178 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   hipLaunchKernelGGL(kb_accumulate, dim3(blocks[BK_PIX]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_pass_end, dim3(n), T64, 0, s, items, n);
   return hipGetLastError();
+}
+
+}  // namespace ppm
+
+// ===========================================================================
+// The gather without allocation (ppm_gather_noalloc), for callers that cannot
+// allocate device memory.  Its buffers live in `scratch`, a fixed area of the
+// library's code object: one copy per GPU, held until the process ends,
+// whose device address noalloc_init() takes once per device (the array's
+// name is only a host-side stand-in; kernels get the address).  Every call
+// on a GPU reuses that GPU's area, so two calls that overlap on one GPU
+// race; whatever must start at zero (the shade's work counters, its flags)
+// is cleared by the call's own kernels, never assumed from the last call.
+namespace ppm {
+
+// Layout: positions, normals and results as doubles (NA_D3 each), then
+// either the tree query's float records (NA_F4 each) or the shade's queries,
+// order, results and pass state.
+constexpr size_t na_align(size_t v) { return (v + 255) & ~(size_t)255; }
+constexpr size_t NA_D3 = na_align(3ull * NOALLOC_MAX_QUERIES * sizeof(double));
+constexpr size_t NA_F4 = na_align((size_t)NOALLOC_MAX_QUERIES * sizeof(float4));
+constexpr size_t NA_TREE = 3 * NA_F4;
+constexpr size_t NA_GRID = na_align((size_t)NOALLOC_MAX_QUERIES * sizeof(Query)) +
+                           na_align((size_t)NOALLOC_MAX_QUERIES * sizeof(uint32_t)) + NA_F4 +
+                           na_align(sizeof(PassState));
+constexpr size_t NA_BYTES = 3 * NA_D3 + (NA_TREE > NA_GRID ? NA_TREE : NA_GRID);
+static_assert(NA_BYTES <= NOALLOC_SCRATCH_MAX_BYTES, "the no-allocation area outgrew its stated bound");
+
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_dev[NOALLOC_MAX_DEVICES] = {};
+
+hipError_t noalloc_init(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= (int)NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  if (g_scratch_dev[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch_dev[device] = static_cast<unsigned char*>(p);
+  return hipSuccess;
+}
+
+namespace {
+
+struct NaLayout {
+  double* q64;    // query positions (host doubles)
+  double* n64;    // query normals
+  double* out64;  // results (doubles, copied back)
+  float4* qf;     // tree: float positions
+  float4* qn;     // tree: float normals
+  float4* rgb;    // tree: float sums
+  Query* q;       // grid: shade queries
+  uint32_t* order;
+  float4* res;    // grid: shade results
+  PassState* st;  // grid: the shade's pass state
+};
+
+NaLayout na_layout(unsigned char* base) {
+  NaLayout L;
+  L.q64 = reinterpret_cast<double*>(base);
+  L.n64 = reinterpret_cast<double*>(base + NA_D3);
+  L.out64 = reinterpret_cast<double*>(base + 2 * NA_D3);
+  unsigned char* r = base + 3 * NA_D3;
+  L.qf = reinterpret_cast<float4*>(r);
+  L.qn = reinterpret_cast<float4*>(r + NA_F4);
+  L.rgb = reinterpret_cast<float4*>(r + 2 * NA_F4);
+  L.q = reinterpret_cast<Query*>(r);
+  unsigned char* o = r + na_align((size_t)NOALLOC_MAX_QUERIES * sizeof(Query));
+  L.order = reinterpret_cast<uint32_t*>(o);
+  o += na_align((size_t)NOALLOC_MAX_QUERIES * sizeof(uint32_t));
+  L.res = reinterpret_cast<float4*>(o);
+  L.st = reinterpret_cast<PassState*>(o + NA_F4);
+  return L;
+}
+
+}  // namespace
+
+// Host doubles -> the float records of the tree query (as run_query forms
+// them: each coordinate rounded to float).
+__global__ __launch_bounds__(256) void k_na_tree_prep(const double* q64, const double* n64, float4* qf, float4* qn,
+                                                      uint32_t nq, int with_nrm) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  qf[i] = make_float4((float)q64[3 * i], (float)q64[3 * i + 1], (float)q64[3 * i + 2], 0.0f);
+  if (with_nrm) qn[i] = make_float4((float)n64[3 * i], (float)n64[3 * i + 1], (float)n64[3 * i + 2], 0.0f);
+}
+
+// Host doubles -> shade queries (weight 1, identity order), as run_grid_gather forms them.
+__global__ __launch_bounds__(256) void k_na_grid_prep(const double* q64, const double* n64, Query* q,
+                                                      uint32_t* order, uint32_t nq) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  Query r;
+  r.a = make_float4((float)q64[3 * i], (float)q64[3 * i + 1], (float)q64[3 * i + 2], __uint_as_float(i));
+  r.b = make_float4((float)n64[3 * i], (float)n64[3 * i + 1], (float)n64[3 * i + 2], __uint_as_float(0u));
+  r.w = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+  q[i] = r;
+  order[i] = i;
+}
+
+// The shade's pass state: the map's, with this call's radius, scale and
+// query count, the flags and work counters cleared.
+__global__ __launch_bounds__(256) void k_na_grid_state(const PassState* src, PassState* dst, double r2, float scale,
+                                                       uint32_t nq) {
+  constexpr uint32_t words = sizeof(PassState) / 4u;
+  const uint32_t* s = reinterpret_cast<const uint32_t*>(src);
+  uint32_t* d = reinterpret_cast<uint32_t*>(dst);
+  for (uint32_t w = threadIdx.x; w < words; w += blockDim.x) d[w] = s[w];
+  __syncthreads();
+  uint32_t* work = &dst->work[0][0][0];
+  for (uint32_t w = threadIdx.x; w < 2u * 8u * 32u; w += blockDim.x) work[w] = 0u;
+  if (threadIdx.x == 0) {
+    dst->r2 = r2;
+    dst->r2f = (float)r2;
+    dst->scale = scale;
+    dst->n_qsort = nq;
+    dst->overflow = 0u;
+  }
+}
+
+// Float sums -> doubles times k (the tree's power / pi / r2, formed on the
+// host exactly as ppm_gather forms it; 1 for the grid, already scaled).
+__global__ __launch_bounds__(256) void k_na_finish(const float4* rgb, double* out, uint32_t nq, double k) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nq) return;
+  const float4 v = rgb[i];
+  out[3 * i] = (double)v.x * k;
+  out[3 * i + 1] = (double)v.y * k;
+  out[3 * i + 2] = (double)v.z * k;
+}
+
+hipError_t launch_gather_noalloc(hipStream_t s, int device, const MapBuf& m, const PassState* map_st, bool grid,
+                                 const SceneBuf& grid_scene, const double* q_xyz, const double* q_nrm, uint32_t nq,
+                                 double r2, double k, float scale, int refine64, int pfilter, int n_cu,
+                                 double* rgb_out, uint32_t* overflow_out) {
+  if (device < 0 || device >= (int)NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch_dev[device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  if (nq == 0 || nq > NOALLOC_MAX_QUERIES) return hipErrorInvalidValue;
+  const NaLayout L = na_layout(base);
+  const dim3 blocks((nq + 255) / 256);
+  hipError_t e;
+  if ((e = hipMemcpyAsync(L.q64, q_xyz, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(L.n64, q_nrm, sizeof(double) * 3 * nq, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+  *overflow_out = 0u;
+  if (grid) {
+    hipLaunchKernelGGL(k_na_grid_prep, blocks, dim3(256), 0, s, L.q64, L.n64, L.q, L.order, nq);
+    hipLaunchKernelGGL(k_na_grid_state, dim3(1), dim3(256), 0, s, map_st, L.st, r2, scale, nq);
+    EyeBuf eb;
+    eb.q = L.q;
+    eb.res = L.res;
+    if ((e = launch_grid_shade(s, grid_scene, m, L.st, eb, L.order, pfilter, n_cu)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_na_finish, blocks, dim3(256), 0, s, L.res, L.out64, nq, 1.0);
+    if ((e = hipMemcpyAsync(overflow_out, &L.st->overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+      return e;
+  } else {
+    hipLaunchKernelGGL(k_na_tree_prep, blocks, dim3(256), 0, s, L.q64, L.n64, L.qf, L.qn, nq, 1);
+    if ((e = launch_query(s, m, const_cast<PassState*>(map_st), L.qf, refine64 ? L.q64 : nullptr, L.qn, nq, r2,
+                          refine64, 2, pfilter, nullptr, nullptr, nullptr, L.rgb, n_cu)) != hipSuccess)
+      return e;
+    hipLaunchKernelGGL(k_na_finish, blocks, dim3(256), 0, s, L.rgb, L.out64, nq, k);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipMemcpyAsync(rgb_out, L.out64, sizeof(double) * 3 * nq, hipMemcpyDeviceToHost, s);
 }
 
 }  // namespace ppm

@@ -202,6 +202,24 @@ hipError_t launch_grid_query(hipStream_t s, const MapBuf& m, const PassState* st
 hipError_t launch_grid_shade(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
                              const uint32_t* order, int pfilter, int n_cu);
 
+// The gather without allocation: at most NOALLOC_MAX_QUERIES queries per
+// call (= PPM_NOALLOC_MAX_QUERIES of ppm.h), in a fixed device area of at
+// most NOALLOC_SCRATCH_MAX_BYTES per GPU, for devices 0..NOALLOC_MAX_DEVICES-1.
+// noalloc_init(device), with that device current, takes the area's address
+// (once per device); launch_gather_noalloc then runs ppm_gather's kernels on
+// a tree map (grid = false: launch_query mode 2, results * k) or a grid map
+// (launch_grid_shade over grid_scene, results as the shade scaled them),
+// copying q_xyz/q_nrm in and rgb_out (nq x 3 doubles) out on stream s;
+// *overflow_out receives the grid shade's flags once s has run.
+constexpr uint32_t NOALLOC_MAX_QUERIES = 65536;
+constexpr size_t NOALLOC_SCRATCH_MAX_BYTES = 9u << 20;
+constexpr uint32_t NOALLOC_MAX_DEVICES = 64;
+hipError_t noalloc_init(int device);
+hipError_t launch_gather_noalloc(hipStream_t s, int device, const MapBuf& m, const PassState* map_st, bool grid,
+                                 const SceneBuf& grid_scene, const double* q_xyz, const double* q_nrm, uint32_t nq,
+                                 double r2, double k, float scale, int refine64, int pfilter, int n_cu,
+                                 double* rgb_out, uint32_t* overflow_out);
+
 // Scene BVH over the n shapes shapes[ids[i]]: Morton keys of the centroids
 // (cmin, cscale = 1023 / extent), a radix sort (count *np on the device),
 // leaves of `leaf` shapes copied into bprims, levels folded up.
