@@ -974,6 +974,47 @@ void orc_diffuse_samples(const double n[3], uint64_t seed, int count, double* ou
 }
 
 // ---- scene ---------------------------------------------------------------------
+static Material material_from_desc(const ppm_material_desc& m) {
+  Material mm;
+  mm.em = cd(m.emittance);
+  mm.ior = cd(m.ior);
+  mm.c1 = cd(m.color1);
+  mm.c2 = cd(m.color2);
+  mm.surface = m.surface;
+  mm.diff = m.diffuseness;
+  mm.metal = m.metalness;
+  mm.rough = m.roughness;
+  mm.density_pow = 1.0 / (std::pow(10.0, 5.0 * (1.0 - std::sqrt(m.roughness))) + 1.0);  // surface.rs:52
+  mm.q = mm.c1;
+  return mm;
+}
+
+// reflection_glossy (physics.rs:232-261) about the mirror direction r of a
+// surface with normal n: `count` samples, sample i from stream (seed, i).
+void orc_reflection_glossy(const double n[3], const double r[3], double pw, uint64_t seed, uint32_t count,
+                           double* out) {
+  for (uint32_t i = 0; i < count; ++i) {
+    Rng rng(seed, i, 0, 0x30000000u);
+    V o = reflection_glossy(vd(n), vd(r), pw, rng);
+    out[3 * i] = o.x; out[3 * i + 1] = o.y; out[3 * i + 2] = o.z;
+  }
+}
+
+// Surface::next_direction (surface.rs:211-258) of a photon of channel wl
+// arriving along v at a surface of material md (normal n facing the photon,
+// relative IOR eta): `count` draws, draw i from stream (seed, i); kind_out
+// 0 absorbed, 1 reflected, 2 transmitted, with the direction in dirs_out.
+void orc_next_direction(const ppm_material_desc* md, double eta, const double n[3], const double v[3], int wl,
+                        uint64_t seed, uint32_t count, int32_t* kind_out, double* dirs_out) {
+  const Material m = material_from_desc(*md);
+  for (uint32_t i = 0; i < count; ++i) {
+    Rng rng(seed, i, 0, 0x40000000u);
+    V o{0, 0, 0};
+    kind_out[i] = s_next_direction(m, eta, vd(n), vd(v), wl, rng, o);
+    dirs_out[3 * i] = o.x; dirs_out[3 * i + 1] = o.y; dirs_out[3 * i + 2] = o.z;
+  }
+}
+
 orc_scene* orc_scene_create(const ppm_scene_desc* d) {
   orc_scene* o = new orc_scene();
   Scene& sc = o->sc;
@@ -992,21 +1033,7 @@ orc_scene* orc_scene_create(const ppm_scene_desc* d) {
       normalize(cross(sh.d1, sh.d2), sh.n);
     sc.shapes.push_back(sh);
   }
-  for (int i = 0; i < d->n_materials; ++i) {
-    const ppm_material_desc& m = d->materials[i];
-    Material mm;
-    mm.em = cd(m.emittance);
-    mm.ior = cd(m.ior);
-    mm.c1 = cd(m.color1);
-    mm.c2 = cd(m.color2);
-    mm.surface = m.surface;
-    mm.diff = m.diffuseness;
-    mm.metal = m.metalness;
-    mm.rough = m.roughness;
-    mm.density_pow = 1.0 / (std::pow(10.0, 5.0 * (1.0 - std::sqrt(m.roughness))) + 1.0);  // surface.rs:52
-    mm.q = mm.c1;
-    sc.mats.push_back(mm);
-  }
+  for (int i = 0; i < d->n_materials; ++i) sc.mats.push_back(material_from_desc(d->materials[i]));
   Material air;  // M_AIR (scene.rs:13-18)
   air.em = C{0, 0, 0};
   air.ior = C{1, 1, 1};

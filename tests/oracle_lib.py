@@ -13,7 +13,7 @@ from ppmpa_amd import _abi as A
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+ORACLE_SO = os.environ.get("PPM_ORACLE_SO") or os.path.join(ORACLE_DIR, "liboracle.so")
 
 dp = C.POINTER(C.c_double)
 u32p = C.POINTER(C.c_uint32)
@@ -52,6 +52,9 @@ _PROTOS = {
     "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, dp]),
     "orc_run_passes": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, C.c_int, dp, dp]),
     "orc_max_threads": (C.c_int, []),
+    "orc_reflection_glossy": (None, [dp, dp, C.c_double, C.c_uint64, C.c_uint32, dp]),
+    "orc_next_direction": (None, [C.POINTER(A.MaterialDesc), C.c_double, dp, dp, C.c_int, C.c_uint64, C.c_uint32,
+                                  C.POINTER(C.c_int32), dp]),
     "orc_scene_set_params": (None, [vp, C.c_int, dp, dp, C.c_double]),
     "orc_run_passes_grad": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, dp, C.c_int]),
 }
@@ -204,6 +207,29 @@ def within_brute(pos, q, r2, mode=0):
         n = lib().orc_within_brute(pos.shape[0], _d(pos), nq, _d(q), float(r2), mode, cap, idx.ctypes.data_as(u32p),
                                    off.ctypes.data_as(u64p))
     return [idx[int(off[i]):int(off[i + 1])] for i in range(nq)]
+
+
+def reflection_glossy(n, r, pw, seed, count):
+    """count samples of reflection_glossy (physics.rs:232-261) about r."""
+    out = np.zeros((count, 3))
+    lib().orc_reflection_glossy(_d(arr3(n)), _d(arr3(r)), float(pw), int(seed), int(count), _d(out))
+    return out
+
+
+def next_direction(material, eta, n, v, wl, seed, count):
+    """count draws of Surface::next_direction (surface.rs:211-258): (kinds,
+    dirs), kind 0 absorbed, 1 reflected, 2 transmitted."""
+    m = material
+    d = A.MaterialDesc()
+    d.emittance, d.transmittance, d.ior = A.D3(*m.emittance), A.D3(*m.transmittance), A.D3(*m.ior)
+    d.surface = int(m.surface)
+    d.color1, d.color2 = A.D3(*m.color1), A.D3(*m.color2)
+    d.diffuseness, d.metalness, d.roughness = float(m.diffuseness), float(m.metalness), float(m.roughness)
+    kinds = np.zeros(count, np.int32)
+    dirs = np.zeros((count, 3))
+    lib().orc_next_direction(C.byref(d), float(eta), _d(arr3(n)), _d(arr3(v)), int(wl), int(seed), int(count),
+                             kinds.ctypes.data_as(C.POINTER(C.c_int32)), _d(dirs))
+    return kinds, dirs
 
 
 def radius_schedule(r1, alpha, first, n):
