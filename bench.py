@@ -30,6 +30,9 @@ DESCRIPTION = {
     "mesh": "mesh: Cornell box + icosphere of 327,680 triangles (radial noise 2%) + 64 spheres "
             "(diffuse/mirror/glass), 1920x1080, 8M photons/pass, max 16 bounces, r1=0.005 m, alpha=2/3, seed 1, "
             "scene BVH",
+    "caustic_plane": "caustic_plane: the caustic workload with the five walls as infinite planes (the reference's "
+                     "Plain shape) instead of 1x1 m quads, 1024x768, 2M photons/pass, max 16 bounces, r1=0.005 m, "
+                     "alpha=2/3, seed 1",
     "two_spheres": "two_spheres: Cornell box + diffuse and mirror spheres, 256x256, 200k photons/pass, "
                    "max 8 bounces, r1=0.01 m, alpha=2/3, seed 1",
     "inverse": "inverse: caustic box with 32 Lambertian materials (3 wall + 29 spheres r=0.05) as unknowns, 512x512, "
@@ -57,7 +60,10 @@ def main():
     ap.add_argument("--workload", choices=sorted(DESCRIPTION), default="caustic")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-3sigma", action="store_true", help="skip the time-to-3-sigma comparison")
-    ap.add_argument("--cpu-sample-passes", type=int, default=8)
+    ap.add_argument("--cpu-sample-passes", type=int, default=32,
+                    help="passes of the oracle's sample on all allowed cores (also the agreement's pass count)")
+    ap.add_argument("--no-plane-walls", action="store_true",
+                    help="caustic workload: skip the 64-pass run of the same box with infinite plane walls")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batch workload: skip the one-room-after-another comparison")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -136,6 +142,9 @@ def main():
 
     if args.copy_probe_mb > 0:
         out["copy_probe_bytes"] = copy_probe(args.copy_probe_mb << 20)
+
+    if args.workload == "caustic" and not args.no_plane_walls:
+        out["walls_plane"] = plane_walls(P, ctx, args.warmup)
 
     if not args.no_cpu_baseline:
         orc = {}
@@ -367,7 +376,28 @@ def _oracle():
 
 
 def _threads():
-    return max(1, min(len(os.sched_getaffinity(0)), 16))
+    """Every core this process may run on (its affinity mask)."""
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def host_info():
+    return {"nproc": os.cpu_count(), "affinity_cores": _threads(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu": cpu_model()}
+
+
+def plane_walls(P, ctx, warmup):
+    """64 passes of the caustic box with infinite Plain walls (the
+    reference's wall shape) timed like the main run: the photons that leave
+    through the open front land on the planes' far parts, so a pass stores
+    more of them."""
+    cfgd = P.CONFIGS["caustic_plane"]
+    sc = cfgd["scene"]()
+    ds = P.DeviceScene(ctx, sc)
+    r = run_workload(P, ctx, ds, cfgd, 64, warmup)
+    return {"workload": DESCRIPTION["caustic_plane"], "passes": 64,
+            "value": round(cfgd["n_photon"] * 64 / r["T"] / 1e6, 3), "unit": "Mphotons/s",
+            "ms_per_step": round(r["T"] * 1e3 / 64, 4), "stored_photons_per_pass": round(r["stored_per_pass"], 1),
+            "stage_ms_per_pass": {k[3:]: round(v, 4) for k, v in r["ms"].items()}}
 
 
 def cpu_model():
@@ -406,10 +436,18 @@ def cpu_baseline(sc, cfgd, workload, n_sample, keep=None):
     T = time.perf_counter() - t0
     if keep is not None:
         keep.update(mean=mean, var=var, passes=n_sample)
+    # the same code on one core, over pass 1 alone
+    t0 = time.perf_counter()
+    osc.run_passes(c, 1, threads=1)
+    T1 = time.perf_counter() - t0
+    h = host_info()
     return {"value": round(cfgd["n_photon"] * n_sample / T / 1e6, 4), "unit": "Mphotons/s", "cores": threads,
-            "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle, {T:.1f} s; "
-                                      f"trace+kd {ms_trace / 1e3:.1f} s, eye+gather {ms_eye / 1e3:.1f} s; "
-                                      f"CPU {cpu_model()}"}
+            "kind": "port", "sample": f"passes 1-{n_sample} of the workload, FP64 oracle on {threads} threads "
+                                      f"(all cores the process may use), {T:.1f} s; trace+kd {ms_trace / 1e3:.1f} s, "
+                                      f"eye+gather {ms_eye / 1e3:.1f} s; CPU {h['cpu']}",
+            "one_core": {"value": round(cfgd["n_photon"] / T1 / 1e6, 4), "unit": "Mphotons/s",
+                         "sample": f"pass 1 on 1 thread, {T1:.1f} s"},
+            "host": h}
 
 
 def oracle_agreement(P, ctx, ds, cfgd, orc):
@@ -426,12 +464,37 @@ def oracle_agreement(P, ctx, ds, cfgd, orc):
     mg, vg, n = s.image()
     s.close()
     mo, vo = orc["mean"], orc["var"]
-    se = np.sqrt(vg / K + vo / K)
-    ok = se > 0
-    frac = float((np.abs(mg - mo)[ok] / se[ok] <= 3).mean())
+    frac, expected, sigma, ncomp = band_fraction(mg, vg, mo, vo, K, K)
     rel = float(np.abs(mg.mean() - mo.mean()) / mo.mean())
-    return {"passes": K, "frac_pixels_within_3sigma": round(frac, 4), "pixels_compared": int(ok.sum()),
-            "image_mean_rel_diff": round(rel, 6)}
+    return {"passes": K, "frac_pixels_within_3sigma": round(frac, 4),
+            "expected_frac_student_t": round(expected, 4), "binomial_sigma": round(sigma, 5),
+            "consistent_within_3_binomial_sigma": bool(abs(frac - expected) <= 3 * sigma),
+            "pixels_compared": ncomp, "image_mean_rel_diff": round(rel, 6)}
+
+
+def band_fraction(ma, va, mb, vb, ka, kb):
+    """Pixel channels whose means agree within 3 standard errors,
+    |ma - mb| <= 3 sqrt(va/ka + vb/kb), with the variances estimated from
+    ka and kb passes; and the fraction expected if both sides estimate the
+    same image: with estimated variances the ratio follows Student's t
+    whose degrees of freedom are Welch-Satterthwaite's,
+    nu = (a + b)^2 / (a^2/(ka-1) + b^2/(kb-1)), a = va/ka, b = vb/kb, so each
+    channel lies within the band with probability P(|T_nu| <= 3) (0.9904 at
+    ka = kb = 8, 0.9961 at 32; 0.9973 as nu grows).  Returns (observed,
+    expected, binomial sigma of the observed fraction for independent
+    channels, channels compared).  Neighbouring pixels share photons, so the
+    binomial sigma understates the spread."""
+    import numpy as np
+    from scipy import stats
+    a, b = va / ka, vb / kb
+    se = np.sqrt(a + b)
+    ok = se > 0
+    nu = (a + b)[ok] ** 2 / (a[ok] ** 2 / (ka - 1) + b[ok] ** 2 / (kb - 1))
+    frac = float((np.abs(ma - mb)[ok] / se[ok] <= 3).mean())
+    expected = float((1.0 - 2.0 * stats.t.sf(3.0, nu)).mean())
+    n = int(ok.sum())
+    sigma = float(np.sqrt(expected * (1.0 - expected) / max(n, 1)))
+    return frac, expected, sigma, n
 
 
 def _blocks(img, b=4):
@@ -487,12 +550,12 @@ def time_to_3sigma(P, ctx):
     ok = se > 0
     fb = float((np.abs(G.mean(0) - Ob.mean(0))[ok] / se[ok] <= 3).mean())
     pg, po = np.stack(imgs_g), np.stack(imgs_o)
-    sp = np.sqrt(pg.var(0, ddof=1) / K + po.var(0, ddof=1) / K)
-    okp = sp > 0
-    fp = float((np.abs(pg.mean(0) - po.mean(0))[okp] / sp[okp] <= 3).mean())
+    fp, ep, sgp, _ = band_fraction(pg.mean(0), pg.var(0, ddof=1), po.mean(0), po.var(0, ddof=1), K, K)
     return {"workload": DESCRIPTION["two_spheres"], "passes": K, "frac_blocks_within_3sigma": round(fb, 4),
-            "frac_pixels_within_3sigma": round(fp, 4), "gpu_s": round(gpu_s, 4), "cpu_s": round(cpu_s, 3),
-            "cpu_cores": threads, "speedup": round(cpu_s / gpu_s, 1)}
+            "frac_pixels_within_3sigma": round(fp, 4), "expected_frac_student_t": round(ep, 4),
+            "consistent_within_3_binomial_sigma": bool(abs(fp - ep) <= 3 * sgp),
+            "gpu_s": round(gpu_s, 4), "cpu_s": round(cpu_s, 3), "cpu_cores": threads,
+            "speedup": round(cpu_s / gpu_s, 1)}
 
 
 if __name__ == "__main__":

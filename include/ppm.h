@@ -15,7 +15,9 @@
  *    non-finite kd coordinates; algebra.rs:220, camera.rs:74 on zero-length
  *    normalisation; photonmap.rs:62-67 on malformed map text.)
  *  - The caller owns host buffers.  Device buffers are owned by the object
- *    that allocated them and are freed by the matching *_destroy.
+ *    that allocated them and are freed by the matching *_destroy.  Destroy
+ *    every object made on a context before ppm_destroy on it: their
+ *    *_destroy calls wait on the context's streams, which ppm_destroy frees.
  *  - All calls on one context are ordered on that context's HIP stream and
  *    are not thread-safe; use one context per host thread.
  *  - Geometry and colours cross the boundary as double; the device keeps
@@ -257,7 +259,9 @@ ppm_status  ppm_photons_upload(ppm_ctx* ctx, ppm_photons* ph, uint64_t n,
 
 /* Copy a photon set back to host memory (float values widened to double).
  * meta may be NULL; otherwise it receives the packed record tag:
- * bits 0-1 channel, bits 2-6 bounce depth, bits 7-31 photon index. */
+ * bits 0-1 channel, bits 2-6 bounce depth, bits 7-31 photon index (the
+ * photon's number in its pass, or its position in an uploaded set; hence
+ * at most 2^25 photons per pass and per uploaded set, PPM_E_RANGE beyond). */
 ppm_status  ppm_photons_download(ppm_ctx* ctx, const ppm_photons* ph,
                                  uint64_t cap, double* pos, double* dir,
                                  uint8_t* wl, uint32_t* meta, uint64_t* n);

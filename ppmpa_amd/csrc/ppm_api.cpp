@@ -19,12 +19,23 @@
 
 using namespace ppm;
 
+// Streams a context makes for its sessions and batches on first use and
+// keeps until ppm_destroy, so that creating and destroying sessions creates
+// and destroys no streams: AUX_PIPE + j - 1 runs pass pipeline j >= 1 (its
+// graph launches), AUX_EYE + j the eye branch of pipeline j (forked into a
+// pass graph's capture, or used directly without graphs), AUX_CAPTURE and
+// AUX_CAPTURE2 the streams pass graphs are captured on.  Calls on a context
+// never overlap (each returns once its work is done), so its sessions and
+// batches may share them.
+constexpr int AUX_PIPE = 0, AUX_EYE = 3, AUX_CAPTURE = 7, AUX_CAPTURE2 = 8, AUX_N = 9;
+
 struct ppm_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   int n_cu = 256;
   std::string err;
+  hipStream_t aux[AUX_N] = {};
 };
 
 struct ppm_scene {
@@ -99,6 +110,29 @@ struct ppm_session {
 };
 
 namespace {
+
+// The context's stream i (made on first use).
+hipError_t ctx_stream(ppm_ctx* c, int i, hipStream_t* out) {
+  if (!c->aux[i]) {
+    const hipError_t e = hipStreamCreateWithFlags(&c->aux[i], hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      c->aux[i] = nullptr;
+      return e;
+    }
+  }
+  *out = c->aux[i];
+  return hipSuccess;
+}
+
+// Replace the context's stream i (after a capture that failed part-way, which
+// may leave a stream forked into it in an unknown capture state).
+void ctx_stream_renew(ppm_ctx* c, int i) {
+  if (c->aux[i]) {
+    (void)hipStreamSynchronize(c->aux[i]);
+    (void)hipStreamDestroy(c->aux[i]);
+  }
+  c->aux[i] = nullptr;
+}
 
 ppm_status fail(ppm_ctx* c, ppm_status s, const std::string& msg) {
   if (c) c->err = msg;
@@ -366,6 +400,12 @@ ppm_status ppm_create(ppm_ctx** out, int device, void* hip_stream) {
 
 void ppm_destroy(ppm_ctx* c) {
   if (!c) return;
+  for (hipStream_t& a : c->aux)
+    if (a) {
+      (void)hipStreamSynchronize(a);
+      (void)hipStreamDestroy(a);
+      a = nullptr;
+    }
   if (c->own_stream && c->stream) {
     (void)hipStreamSynchronize(c->stream);
     (void)hipStreamDestroy(c->stream);
@@ -897,7 +937,8 @@ ppm_status ppm_trace_photons(ppm_ctx* c, const ppm_scene* sc, uint64_t seed, uin
   ppm_scene* s = const_cast<ppm_scene*>(sc);
   double power = 0;
   uint64_t total = assign_photons(s->lights, s->total_flux, n_photon, &power);
-  if (total == 0 || total > 0x7FFFFFFFull) return fail(c, PPM_E_RANGE, "per-light photon counts out of range");
+  if (total == 0 || total > PHOTON_INDEX_MAX)
+    return fail(c, PPM_E_RANGE, "photons per pass must be in [1, 2^25] (the record's photon index)");
   if (!photons_layout(out->pb, total, max_depth, c->n_cu))
     return fail(c, PPM_E_RANGE, "photon set capacity too small: need " +
                                     std::to_string(photons_needed(total, max_depth, c->n_cu)));
@@ -936,6 +977,7 @@ ppm_status ppm_photons_upload(ppm_ctx* c, ppm_photons* p, uint64_t n, const doub
                               const uint8_t* wl) {
   if (!c || !p || (n && (!pos || !dir || !wl))) return fail(c, PPM_E_INVALID, "null argument");
   if (n > p->cap) return fail(c, PPM_E_RANGE, "photon set capacity exceeded");
+  if (n > PHOTON_INDEX_MAX) return fail(c, PPM_E_RANGE, "at most 2^25 photons per set (the record's photon index)");
   std::vector<PhotonRec> recs(n);
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint64_t i = 0; i < n; ++i) {
@@ -1409,11 +1451,10 @@ void ppm_session_destroy(ppm_session* s) {
   if (!s) return;
   if (s->ctx) (void)hipStreamSynchronize(s->ctx->stream);
   for (Pipe& p : s->pipes) {
+    // the streams are the context's (ctx_stream): waited for, not destroyed
     if (p.stream) (void)hipStreamSynchronize(p.stream);
     if (p.s2) (void)hipStreamSynchronize(p.s2);
     pipe_drop_graph(p);
-    if (p.stream && p.stream != s->ctx->stream) (void)hipStreamDestroy(p.stream);
-    if (p.s2) (void)hipStreamDestroy(p.s2);
     for (hipEvent_t e : {p.evf, p.evj, p.done})
       if (e) (void)hipEventDestroy(e);
     photons_free(p.pb);
@@ -1457,9 +1498,9 @@ static ppm_status session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass
   if (s->cfg.first_pass == 0) s->cfg.first_pass = 1;
   std::vector<DevLight> L = sc->lights;
   uint64_t total = assign_photons(L, sc->total_flux, cfg->n_photon, &s->power);
-  if (total == 0 || total > 0x7FFFFFFFull) {
+  if (total == 0 || total > PHOTON_INDEX_MAX) {
     delete s;
-    return fail(c, PPM_E_RANGE, "per-light photon counts out of range");
+    return fail(c, PPM_E_RANGE, "photons per pass must be in [1, 2^25] (the record's photon index)");
   }
   const uint64_t cap = photons_needed(total, cfg->max_depth, c->n_cu);
   if (cap > 0x7FFFFFFFull) {
@@ -1492,12 +1533,13 @@ static ppm_status session_create(ppm_ctx* c, const ppm_scene* sc, const ppm_pass
       // pipeline 0 runs on the context's stream: K pipelines then use 2K
       // streams in all, which HIP spreads over that many hardware queues
       // (GPU_MAX_HW_QUEUES, 4 by default) when K <= 2
-      if (&p == &s->pipes[0] || !own_streams)
+      const int j = (int)(&p - &s->pipes[0]);
+      if (j == 0 || !own_streams)
         p.stream = c->stream;
-      else if ((e = hipStreamCreateWithFlags(&p.stream, hipStreamNonBlocking)) != hipSuccess)
+      else if ((e = ctx_stream(c, AUX_PIPE + j - 1, &p.stream)) != hipSuccess)
         break;
       if (!own_streams) continue;
-      if ((e = hipStreamCreateWithFlags(&p.s2, hipStreamNonBlocking)) != hipSuccess) break;
+      if ((e = ctx_stream(c, AUX_EYE + j, &p.s2)) != hipSuccess) break;
       if ((e = hipEventCreateWithFlags(&p.evf, hipEventDisableTiming)) != hipSuccess) break;
       if ((e = hipEventCreateWithFlags(&p.evj, hipEventDisableTiming)) != hipSuccess) break;
       if ((e = hipEventCreateWithFlags(&p.done, hipEventDisableTiming)) != hipSuccess) break;
@@ -1565,21 +1607,31 @@ static hipError_t enqueue_pass(ppm_session* s, uint32_t j, hipStream_t st, hipEv
   return hipSuccess;
 }
 
-// The pass graph of pipeline j, captured once.
+// The pass graph of pipeline j, captured once.  A capture that fails
+// part-way leaves no graph behind, and the capture stream and the eye
+// stream forked into it are replaced by fresh ones.
 static hipError_t pipe_graph(ppm_session* s, uint32_t j) {
   Pipe& p = s->pipes[j];
   if (p.exec) return hipSuccess;
+  ppm_ctx* c = s->ctx;
   hipStream_t cap;
-  hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+  hipError_t e = ctx_stream(c, AUX_CAPTURE, &cap);
   if (e != hipSuccess) return e;
   e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
     const hipError_t e2 = enqueue_pass(s, j, cap, nullptr);
-    e = hipStreamEndCapture(cap, &p.graph);
-    if (e2 != hipSuccess) e = e2;
+    const hipError_t e3 = hipStreamEndCapture(cap, &p.graph);
+    e = e2 != hipSuccess ? e2 : e3;
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0);
-  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess) {
+    pipe_drop_graph(p);
+    ctx_stream_renew(c, AUX_CAPTURE);
+    ctx_stream_renew(c, AUX_EYE + (int)j);
+    p.s2 = nullptr;
+    const hipError_t e4 = ctx_stream(c, AUX_EYE + (int)j, &p.s2);
+    (void)e4;  // a missing eye stream fails the next capture, not this report
+  }
   return e;
 }
 
@@ -1946,27 +1998,34 @@ hipError_t batch_round(ppm_batch* b, uint32_t act, uint32_t q) {
   if (!b->exec[gi]) {
     uint32_t blocks[BK_KINDS];
     for (int k = 0; k < BK_KINDS; ++k) blocks[k] = b->htab[(size_t)k * (n + 1) + act];
-    // the capture stream, the eye branch's stream and the fork/join events
-    // live only while the graph is captured
+    // captured on the context's capture streams; the fork/join events live
+    // only while the graph is captured.  A capture that fails part-way
+    // leaves no graph behind and replaces both capture streams.
     hipStream_t cap = nullptr, cap2 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&cap2, hipStreamNonBlocking);
+    hipError_t e = ctx_stream(b->ctx, AUX_CAPTURE, &cap);
+    if (e == hipSuccess) e = ctx_stream(b->ctx, AUX_CAPTURE2, &cap2);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
       const hipError_t e2 =
           launch_batch_pass(cap, cap2, fork, join, b->items + (size_t)q * n, b->tab, blocks, act, n, b->bvh, b->dyn);
-      e = hipStreamEndCapture(cap, &b->graph[gi]);
-      if (e2 != hipSuccess) e = e2;
+      const hipError_t e3 = hipStreamEndCapture(cap, &b->graph[gi]);
+      e = e2 != hipSuccess ? e2 : e3;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&b->exec[gi], b->graph[gi], nullptr, nullptr, 0);
     if (join) (void)hipEventDestroy(join);
     if (fork) (void)hipEventDestroy(fork);
-    if (cap2) (void)hipStreamDestroy(cap2);
-    if (cap) (void)hipStreamDestroy(cap);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+      if (b->exec[gi]) (void)hipGraphExecDestroy(b->exec[gi]);
+      if (b->graph[gi]) (void)hipGraphDestroy(b->graph[gi]);
+      b->exec[gi] = nullptr;
+      b->graph[gi] = nullptr;
+      ctx_stream_renew(b->ctx, AUX_CAPTURE);
+      ctx_stream_renew(b->ctx, AUX_CAPTURE2);
+      return e;
+    }
   }
   return hipGraphLaunch(b->exec[gi], b->ps[q]);
 }
@@ -1995,8 +2054,8 @@ extern "C" {
 void ppm_batch_destroy(ppm_batch* b) {
   if (!b) return;
   if (b->ctx) (void)hipStreamSynchronize(b->ctx->stream);
-  for (size_t q = 1; q < b->ps.size(); ++q)
-    if (b->ps[q]) (void)hipStreamSynchronize(b->ps[q]), (void)hipStreamDestroy(b->ps[q]);
+  for (size_t q = 1; q < b->ps.size(); ++q)  // the context's streams: waited for, not destroyed
+    if (b->ps[q]) (void)hipStreamSynchronize(b->ps[q]);
   for (hipEvent_t e : b->pdone)
     if (e) (void)hipEventDestroy(e);
   batch_drop_graphs(b);
@@ -2047,7 +2106,7 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
   b->ps[0] = c->stream;
   e = hipSuccess;
   for (uint32_t q = 1; q < b->K && e == hipSuccess; ++q) {
-    e = hipStreamCreateWithFlags(&b->ps[q], hipStreamNonBlocking);
+    e = ctx_stream(c, AUX_PIPE + (int)q - 1, &b->ps[q]);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->pdone[q], hipEventDisableTiming);
   }
   if (e != hipSuccess || (e = dalloc(&b->items, (uint64_t)b->K * n)) != hipSuccess ||

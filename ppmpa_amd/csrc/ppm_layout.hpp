@@ -116,6 +116,9 @@ struct alignas(16) PhotonRec {
   float4 pm;
   float4 dm;
 };
+// The photon index has 25 bits: photons per pass (and per uploaded set) are
+// limited to PHOTON_INDEX_MAX, checked where sets are made.
+constexpr uint64_t PHOTON_INDEX_MAX = 1ull << 25;
 __host__ __device__ inline uint32_t pack_meta(uint32_t wl, uint32_t depth, uint32_t pid) {
   return (wl & 3u) | ((depth & 31u) << 2) | (pid << 7);
 }

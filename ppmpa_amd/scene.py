@@ -587,6 +587,11 @@ CONFIGS = {
                     seed=1),
     "caustic": dict(scene=caustic_scene, n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0,
                     max_depth=16, seed=1),
+    # the same box with infinite Plain walls, the reference's own wall shape
+    # (geometry.rs:170-177): the open front lets photons escape into the
+    # planes' far reaches, where they are stored too
+    "caustic_plane": dict(scene=lambda xreso=1024, yreso=768: caustic_scene(xreso, yreso, walls="plane"),
+                          n_photon=2_000_000, passes=512, radius0=0.005, alpha=2.0 / 3.0, max_depth=16, seed=1),
     "mesh": dict(scene=mesh_scene, n_photon=8_000_000, passes=256, radius0=0.005, alpha=2.0 / 3.0, max_depth=16,
                  seed=1),
     "inverse": dict(scene=inverse_scene, n_photon=1_000_000, passes=64, radius0=0.005, alpha=2.0 / 3.0, max_depth=16,

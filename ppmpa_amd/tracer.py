@@ -15,11 +15,28 @@ gfx950 C ABI (libppm.so).
 Every call goes to the HIP library; there is no CPU fallback.  Errors raise
 PpmError carrying the library's status and message.
 """
+import atexit
 import ctypes as C
+import weakref
 
 import numpy as np
 
 from . import _abi as A
+
+# Contexts alive in this process.  At exit they are closed while the
+# interpreter is still whole, each after every object made on it (the
+# objects' destroy calls wait on the context's stream, which ppm_destroy
+# frees); left to the interpreter's shutdown the order would be arbitrary.
+_CONTEXTS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all():
+    for c in list(_CONTEXTS):
+        try:
+            c.close()
+        except Exception:
+            pass
 
 
 class PpmError(RuntimeError):
@@ -37,11 +54,18 @@ class Context:
 
     def __init__(self, device=0):
         self.lib = A.load()
+        self.h = None
+        self._objs = weakref.WeakSet()  # everything made on this context
         h = C.c_void_p()
         s = self.lib.ppm_create(C.byref(h), int(device), None)
         if s != 0:
             raise PpmError(s, self.lib.ppm_status_string(s).decode())
         self.h = h
+        _CONTEXTS.add(self)
+
+    def _own(self, obj):
+        """Record an object made on this context (closed before it)."""
+        self._objs.add(obj)
 
     def check(self, s):
         if s != 0:
@@ -52,7 +76,12 @@ class Context:
         self.check(self.lib.ppm_sync(self.h))
 
     def close(self):
+        """ppm_destroy, after closing every object still open on the context
+        (batches and sessions first, scenes last)."""
         if self.h:
+            rank = {Batch: 0, Session: 1, PhotonMap: 2, PhotonSet: 3, DeviceScene: 4}
+            for o in sorted(list(self._objs), key=lambda o: rank.get(type(o), 0)):
+                o.close()
             self.lib.ppm_destroy(self.h)
             self.h = None
 
@@ -73,6 +102,7 @@ class DeviceScene:
         ctx.check(ctx.lib.ppm_scene_create(ctx.h, C.byref(desc), C.byref(h)))
         self.h = h
         self._keep = keep
+        ctx._own(self)
 
     @property
     def pixels(self):
@@ -131,6 +161,7 @@ class PhotonSet:
         ctx.check(ctx.lib.ppm_photons_create(ctx.h, self.capacity, C.byref(h)))
         self.h = h
         self.power = 1.0
+        ctx._own(self)
 
     def count(self):
         n = C.c_uint64()
@@ -200,6 +231,7 @@ class PhotonMap:
         ctx.check(ctx.lib.ppm_map_create(ctx.h, self.capacity, C.byref(h)))
         self.h = h
         self.power = 1.0
+        ctx._own(self)
 
     def build(self, photons, power, dscene=None):
         self.power = float(power)
@@ -302,6 +334,7 @@ class Session:
         h = C.c_void_p()
         ctx.check(ctx.lib.ppm_session_create(ctx.h, dscene.h, C.byref(cfg), C.byref(h)))
         self.h = h
+        ctx._own(self)
 
     def run(self, n_pass):
         self.ctx.check(self.ctx.lib.ppm_session_run(self.h, int(n_pass)))
@@ -377,6 +410,7 @@ class Batch:
         h = C.c_void_p()
         ctx.check(ctx.lib.ppm_batch_create(ctx.h, n, hs, cf, npass, C.byref(h)))
         self.h = h
+        ctx._own(self)
         self.n_passes = [int(k) for k in n_passes]
 
     def run(self, max_rounds=0):

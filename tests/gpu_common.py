@@ -82,15 +82,77 @@ def check_photon_trace_crn(c, sc, seed, pass_, n, uc, maxd, match=0.99):
     return ph
 
 
-def eye_agreement(got, want, tol=1e-3):
-    """Fraction of pixels whose channels all agree within tol relative
-    (floor: tol of the image maximum)."""
+def pixel_rel(got, want):
+    """Per pixel, the largest channel difference relative to the oracle
+    (floor: 1e-3 of the image maximum)."""
     rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-3 * np.abs(want).max())
-    return float((rel.max(axis=2) < tol).mean())
+    return rel.max(axis=2).ravel()
 
 
-def check_eye_crn(c, sc, seed, n, r2, uc, maxd, pfilter=0, frac_min=0.98):
-    """One eye pass over the oracle's photons: device vs FP64 oracle."""
+def eye_agreement(got, want, tol=1e-3):
+    """Fraction of pixels whose channels all agree within tol relative."""
+    return float((pixel_rel(got, want) < tol).mean())
+
+
+# Agreement of a GPU image with the FP64 oracle's on common random numbers.
+# Per pixel (pixel_rel) the bulk agrees to float rounding, ~1e-7.  Three
+# things widen the distribution, each by a known mechanism:
+#  - photon filters (cone, Gauss): the weight of d^2/r^2 formed from float
+#    coordinates, where p - q cancels, carries ~1e-6 (|q| + r)/r per photon,
+#    so pixel medians sit at 1e-6 to 1e-5;
+#  - float decisions that come out against the double ones: a photon on a
+#    ball's edge (eye passes over shared photons), a photon's whole path
+#    (sessions, whose photons each side traces: a roulette draw at its
+#    threshold, a grazing hit), which moves a pixel by one photon's share;
+#  - glossy TS surfaces (roughness 0.5): their reflection_glossy draws turn
+#    float differences into direction differences, so more paths part;
+#  - rooms crowded with mirror and glass spheres (the batched workload's
+#    rooms of 100 spheres): long specular chains amplify the float
+#    differences of every hit (the same room with its glass made diffuse
+#    agrees as loosely: its mirrors remain).
+# Bands (tolerance, least fraction of pixels within it) and the image
+# means' relative tolerance, set from the MI355X measurements (left):
+#   eye, no filter, smooth        99.2-99.6% < 1e-5, 99.97-100% < 1e-3, means 3e-6
+#   eye, cone/Gauss filter        97.8-99.0% < 1e-4, 99.8-99.9% < 1e-3, means 6e-6
+#   eye, glossy TS room           97.7-99.4% < 1e-4, 99.4-99.8% < 1e-3, means 2.4e-4
+#   session, no filter, smooth    97.8-99.0% < 1e-5, 99.5-99.9% < 1e-4, means 1.2e-4
+#   session, cone filter          92.0% < 1e-4, 99.5% < 1e-3, mean 3e-7
+#   session, glossy TS room       95.8% < 1e-5, 99.0% < 1e-4, mean 3.2e-4
+#   session, 101 specular spheres 92.0% < 1e-4, 97.7% < 1e-3, mean 2.0e-4
+# The per-pixel median, 1e-7 to 2e-7 without a filter (3e-6 to 1e-5 with
+# one), is bounded too: a systematic error moves every pixel, the median
+# first.  Entries: (bands, image mean tolerance, median bound).
+CRN = {
+    "eye": (((1e-5, 0.985), (1e-3, 0.998)), 1e-5, 1e-6),
+    "eye_filter": (((1e-4, 0.96), (1e-3, 0.995)), 1e-4, 3e-5),
+    "eye_glossy": (((1e-4, 0.96), (1e-3, 0.99)), 1e-3, 3e-6),
+    "session": (((1e-5, 0.97), (1e-4, 0.99)), 5e-4, 1e-6),
+    "session_filter": (((1e-4, 0.88), (1e-3, 0.99)), 1e-4, 5e-5),
+    "session_glossy": (((1e-5, 0.94), (1e-4, 0.98)), 1e-3, 1e-6),
+    "session_specular": (((1e-4, 0.88), (1e-3, 0.96)), 1e-3, 1e-6),
+}
+
+
+def assert_crn(got, want, kind, label=""):
+    """got agrees with want within every band of CRN[kind], the image means
+    within its tolerance and the per-pixel median under its bound."""
+    bands, mean_tol, median_max = CRN[kind]
+    rel = pixel_rel(got, want)
+    fr = {t: float((rel < t).mean()) for t in (1e-6, 1e-5, 1e-4, 1e-3, 1e-2)}
+    fr.update({t: float((rel < t).mean()) for t, _ in bands})
+    mrel = abs(got.mean() - want.mean()) / abs(want.mean())
+    print(f"[crn] {label} " + " ".join(f"<{t:g}: {f:.4f}" for t, f in sorted(fr.items())) +
+          f" median {np.median(rel):.2e} mean rel {mrel:.2e}", flush=True)
+    for t, need in bands:
+        assert fr[t] >= need, (label, t, fr[t], need)
+    assert mrel <= mean_tol, (label, mrel, mean_tol)
+    assert np.median(rel) <= median_max, (label, np.median(rel), median_max)
+
+
+def check_eye_crn(c, sc, seed, n, r2, uc, maxd, pfilter=0, frac_min=None, kind=None):
+    """One eye pass over the oracle's photons: device vs FP64 oracle, within
+    the CRN bands of `kind` (default: by filter), or, given frac_min, that
+    fraction of pixels within 1e-3."""
     import oracle_lib as O
     osc = O.OScene(sc)
     pos, dir, wl, meta, pw = osc.trace_photons(seed, 1, n, uc, maxd)
@@ -103,7 +165,11 @@ def check_eye_crn(c, sc, seed, n, r2, uc, maxd, pfilter=0, frac_min=0.98):
     want = osc.trace_eye(om, seed + 1, 1, r2, uc, maxd, pfilter)
     assert np.isfinite(got).all()
     assert want.max() > 0
-    frac = eye_agreement(got, want)
-    assert frac > frac_min, frac
-    assert abs(got.mean() - want.mean()) < 1e-3 * want.mean()
+    if frac_min is None:
+        assert_crn(got, want, kind or ("eye" if pfilter == 0 else "eye_filter"), "eye")
+    else:
+        frac = eye_agreement(got, want)
+        print(f"[crn] eye {frac:.4f} within 1e-3", flush=True)
+        assert frac > frac_min, frac
+        assert abs(got.mean() - want.mean()) < 1e-3 * want.mean()
     return got, want

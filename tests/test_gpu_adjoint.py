@@ -122,8 +122,14 @@ def test_adjoint_against_fp64_oracle():
     ograd, omean = osc.run_passes_grad(_cfg(), N_PASS, g_img)
     a, b = grad[:32].ravel(), ograd[:32].ravel()
     rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-    print("gpu vs oracle gradient: rel L2", rel, "image rel", np.abs(mean - omean).sum() / np.abs(omean).sum())
-    assert rel < 2e-2
+    img_rel = np.abs(mean - omean).sum() / np.abs(omean).sum()
+    print("gpu vs oracle gradient: rel L2", rel, "image rel", img_rel)
+    # Measured on the MI355X: 1.9e-5 for both.  The per-pixel agreement is
+    # float rounding (~1e-7) except where a photon's float path flipped
+    # against the double one (a few of the 40k photons of the two passes),
+    # which moves the image and its gradient by the same ~2e-5.
+    assert img_rel < 1e-4
+    assert rel < 1e-4
     big = np.abs(b) > 0.05 * np.abs(b).max()
     assert np.all(np.sign(a[big]) == np.sign(b[big]))
 

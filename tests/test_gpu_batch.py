@@ -120,3 +120,52 @@ def test_fine_query_keys_match_batch():
     assert m0.mean() > 0
     assert _same(m1, m0)
     assert np.allclose(v1, v0, rtol=1e-5, atol=1e-12)
+
+
+def test_batch_rooms_match_oracle():
+    """Three rooms of the batched workload at reduced size (their 32-38 px
+    images, 12k photons per pass, 3 passes), rendered through ppm_batch_*:
+    each room's image against the FP64 oracle's passes on the same Philox
+    streams, within the session bands.  Rooms 0 (6 spheres, 4 of glass),
+    6 (101 spheres, a BVH) and 10 (34 spheres)."""
+    import oracle_lib as O
+    from gpu_common import assert_crn
+    say("batch rooms vs oracle")
+    c = ctx()
+    all_items = P.batch_scenes(n=64, seed=1, max_side=40, min_side=32)
+    items = [(all_items[i][0], dict(all_items[i][1], n_photon=12000), 3) for i in (0, 6, 10)]
+    dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
+    assert dss[1].stats()[1] > 0 and dss[0].stats()[1] == 0
+    b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    b.run()
+    for i, (sc, cfg, passes) in enumerate(items):
+        m, _, k = b.image(i, with_var=False)
+        assert k == passes
+        omean, _, _, _, _ = O.OScene(sc).run_passes(_cfg(cfg), passes)
+        assert omean.max() > 0
+        # the rooms' spheres are a third mirrors and a third glass
+        assert_crn(m, omean, "session_specular", f"batch room {(0, 6, 10)[i]}")
+    b.close()
+
+
+def test_batch_full_size_rooms_equal_sessions():
+    """All 64 rooms of the batched workload at their full sizes (images of
+    128-512 px, 50k-2M photons per pass; passes capped at 24 per room): the
+    autotuned batch gives every room its own session's image."""
+    say("batch full size == sessions")
+    c = ctx()
+    items = [(sc, cfg, min(p, 24)) for sc, cfg, p in P.batch_scenes(n=64, seed=1)]
+    dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
+    b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+    b.tune(rounds=1)
+    b.run()
+    for i, ((sc, cfg, passes), ds) in enumerate(zip(items, dss)):
+        m, _, k = b.image(i, with_var=False)
+        assert k == passes
+        s = P.Session(c, ds, _cfg(cfg))
+        s.run(passes)
+        m0, _, _ = s.image(with_var=False)
+        s.close()
+        assert m0.mean() > 0
+        assert _same(m, m0), i
+    b.close()

@@ -67,3 +67,37 @@ def test_reset_and_default_pipelines():
     assert n == 5 and np.array_equal(m0, m1)
     with pytest.raises(P.PpmError):
         P.Session(c, ds, _cfg(5))
+
+
+@pytest.mark.parametrize("n_pass,pipelines", [(512, 1), (512, 2), (1000, 3)])
+def test_radius_schedule_closed_form(n_pass, pipelines):
+    """r_i^2 = r_1^2 prod_{k=1}^{i-1} (k + alpha)/(k + 1) (util/iterator.rb:33-38),
+    kept as an FP64 product on the device: after 512 (1000) passes it equals
+    the exact rational product rounded once to 1e-12 relative (measured
+    8e-15 and 1.8e-14), and the closed form Gamma(i + alpha) /
+    (Gamma(1 + alpha) Gamma(i + 1)) to 1e-10 (lgamma's own precision).  For
+    comparison a float32 product would have drifted by ~2e-7 (5e-7); the
+    kernels' float r2 is the double rounded once (6e-8 at most)."""
+    import math
+    from fractions import Fraction
+    say("radius schedule", n_pass, pipelines)
+    c = ctx()
+    ds = P.DeviceScene(c, P.two_spheres_scene(16, 16))
+    r1 = 0.01
+    cfg = P.pass_cfg(seed=1, n_photon=2000, radius0=r1, alpha=2 / 3, max_depth=2, pipelines=pipelines)
+    s = P.Session(c, ds, cfg)
+    s.run(n_pass)
+    r2 = s.radius2()
+    s.close()
+    exact = Fraction(1, 10000)  # r1^2, alpha = 2/3: (k + 2/3)/(k + 1) = (3k + 2)/(3k + 3)
+    for k in range(1, n_pass):
+        exact *= Fraction(3 * k + 2, 3 * k + 3)
+    assert abs(r2 - float(exact)) <= 1e-12 * float(exact), (r2, float(exact))
+    a = 2.0 / 3.0
+    gam = r1 * r1 * math.exp(math.lgamma(n_pass + a) - math.lgamma(1 + a) - math.lgamma(n_pass + 1))
+    assert abs(r2 - gam) <= 1e-10 * gam
+    f = np.float32(r1 * r1)
+    for k in range(1, n_pass):
+        f = np.float32(f * np.float32((k + a) / (k + 1)))
+    print(f"radius^2 after {n_pass} passes: device rel {abs(r2 - float(exact)) / float(exact):.1e}, "
+          f"float32 product rel {abs(float(f) - float(exact)) / float(exact):.1e}")
