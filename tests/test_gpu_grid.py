@@ -180,20 +180,27 @@ def test_grid_golden_map():
 
 
 def test_grid_scene_bounds_clamp():
-    """Photons beyond the scene's bounds fall into the clamped outer cells:
-    the sets stay exact (a scene's bounds are what a pass's grid uses)."""
+    """Photons beyond the scene's bounds keep cells of their own (hashed like
+    any other), out to about 2^20 cells, where the coordinates clamp: near
+    and kilometres-away photons and queries, sets exact (a scene's bounds
+    are what a pass's grid uses)."""
     say("grid clamp")
     c = ctx()
     ds = P.DeviceScene(c, P.two_spheres_scene(16, 16))
     rng = np.random.default_rng(41)
     n = 6000
     pos = rng.uniform(-0.6, 1.6, (n, 3))
+    # a plane far out, as infinite walls store them: 1 km and 200 km away
+    far = rng.uniform(-1.0, 1.0, (600, 3)) * np.array([40.0, 0.0, 40.0]) + np.array([0.5, 0.0, -1000.0])
+    farther = rng.uniform(-1.0, 1.0, (300, 3)) * np.array([300.0, 0.0, 300.0]) + np.array([0.5, 0.0, -2e5])
+    pos = np.concatenate([pos, far, farther])
+    n = len(pos)
     d = rng.normal(size=(n, 3))
     d /= np.linalg.norm(d, axis=1, keepdims=True)
     wl = rng.integers(0, 3, n).astype(np.uint8)
     r2 = 0.08 ** 2
     m, _ = _grid(pos, d, wl, 1.0 / n, r2, ds)
-    q = np.concatenate([rng.uniform(-0.7, 1.7, (300, 3)), pos[:100]])
+    q = np.concatenate([rng.uniform(-0.7, 1.7, (300, 3)), pos[:100], far[:50], farther[:50]])
     nrm = rng.normal(size=q.shape)
     nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
     _check_sets(m, pos, q, r2)
