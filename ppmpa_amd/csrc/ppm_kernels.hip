@@ -1126,8 +1126,10 @@ __device__ __forceinline__ void grid_range_scan(const PhotonRec* ph, uint2 se, f
   }
 }
 
-// The same scan calling on_photon(j, pm, dm, d2, inside) for every photon of
-// the range, inside = (d2 <= r2): for a callback whose work is a select
+// The same scan calling on_photon(j, pm, dm, d2, inside, valid) for every
+// photon of the range, inside = (d2 <= r2), valid = true (the walk below also
+// calls it for slots past a range's end, with both false): for a callback
+// whose work is a select
 // (the shade's sum adds 0 for photons outside), with no branch per photon
 // and so none of the exec-mask work that kept the scalar units half busy.
 template <typename OnPhoton>
@@ -1140,15 +1142,65 @@ __device__ __forceinline__ void grid_range_scan_all(const PhotonRec* ph, uint2 s
     const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
     const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
     const float d3 = dist2_nc(qx, qy, qz, p3.pm.x, p3.pm.y, p3.pm.z);
-    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2);
-    on_photon(j + 1, p1.pm, p1.dm, d1, d1 <= r2);
-    on_photon(j + 2, p2.pm, p2.dm, d2, d2 <= r2);
-    on_photon(j + 3, p3.pm, p3.dm, d3, d3 <= r2);
+    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2, true);
+    on_photon(j + 1, p1.pm, p1.dm, d1, d1 <= r2, true);
+    on_photon(j + 2, p2.pm, p2.dm, d2, d2 <= r2, true);
+    on_photon(j + 3, p3.pm, p3.dm, d3, d3 <= r2, true);
   }
   for (; j < se.y; ++j) {
     const PhotonRec p0 = ph[j];
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
-    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2);
+    on_photon(j, p0.pm, p0.dm, d0, d0 <= r2, true);
+  }
+}
+
+// The photons of a query's (at most) eight cell ranges se[0..7] walked in
+// one loop, as one sequence: a lane moves on to its next range when it has
+// used up the current one.  Walked range by range, a wave ran each cell's
+// loop as long as that cell's longest lane, and the sum of those exceeds
+// the lane with the most photons over all its cells: on the first pass of
+// the caustic workload (a model of its photons and Morton-ordered queries)
+// 361 against 275 four-photon steps per wave of 64 queries.  Photons are
+// visited in the same order as range by range, so every sum is formed as
+// before.  Four photons per step (their loads in flight together); a step
+// that runs past its range's end reads the range's last photon again for
+// the missing slots and marks them not valid.  ALL: on_photon(j, pm, dm,
+// d2, inside, valid) for every slot, inside only when valid; otherwise
+// on_photon(j, pm, dm, d2) for the valid photons inside the ball.
+template <bool ALL, typename OnPhoton>
+__device__ __forceinline__ void grid_walk8(const PhotonRec* ph, const uint2 (&se)[8], float qx, float qy, float qz,
+                                           float r2, OnPhoton& on_photon) {
+  uint32_t k = 0, j = se[0].x, e = se[0].y;
+  while (true) {
+    while (j >= e) {
+      if (++k >= 8u) return;
+      // range k (k differs between lanes: a chain of selects, not an index)
+      uint2 n = se[1];
+#pragma unroll
+      for (uint32_t t = 2; t < 8; ++t) n = k == t ? se[t] : n;
+      j = n.x;
+      e = n.y;
+    }
+    const uint32_t last = e - 1u;
+    const uint32_t i1 = min(j + 1u, last), i2 = min(j + 2u, last), i3 = min(j + 3u, last);
+    const PhotonRec p0 = ph[j], p1 = ph[i1], p2 = ph[i2], p3 = ph[i3];
+    const bool v1 = j + 1u < e, v2 = j + 2u < e, v3 = j + 3u < e;
+    const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
+    const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
+    const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
+    const float d3 = dist2_nc(qx, qy, qz, p3.pm.x, p3.pm.y, p3.pm.z);
+    if constexpr (ALL) {
+      on_photon(j, p0.pm, p0.dm, d0, d0 <= r2, true);
+      on_photon(i1, p1.pm, p1.dm, d1, v1 && d1 <= r2, v1);
+      on_photon(i2, p2.pm, p2.dm, d2, v2 && d2 <= r2, v2);
+      on_photon(i3, p3.pm, p3.dm, d3, v3 && d3 <= r2, v3);
+    } else {
+      if (d0 <= r2) on_photon(j, p0.pm, p0.dm, d0);
+      if (v1 && d1 <= r2) on_photon(i1, p1.pm, p1.dm, d1);
+      if (v2 && d2 <= r2) on_photon(i2, p2.pm, p2.dm, d2);
+      if (v3 && d3 <= r2) on_photon(i3, p3.pm, p3.dm, d3);
+    }
+    j += 4u;
   }
 }
 
@@ -1161,8 +1213,8 @@ __device__ __forceinline__ void grid_bucket_scan(const uint2* cells, const Photo
 // Returns false (and scans only the first three cells of an axis) if the
 // query's cells span more than three per axis, which the cell side rules out
 // for every finite point (the caller raises a flag).
-// ALL = true: on_photon(j, pm, dm, d2, inside) for every photon of the cells
-// (grid_range_scan_all), in the same order.
+// ALL = true: on_photon(j, pm, dm, d2, inside, valid) for every photon of the
+// cells, in the same order, and for padding slots with valid = false.
 template <bool ALL = false, typename OnPhoton>
 __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells, const PhotonRec* ph, float qx,
                                            float qy, float qz, float r2, OnPhoton&& on_photon) {
@@ -1185,18 +1237,12 @@ __device__ __forceinline__ bool grid_query(const GridView& g, const uint2* cells
 #pragma unroll
       for (int j = 0; j < k; ++j) v[k] = v[k] && !(v[j] && b[j] == b[k]);
     }
-    // the cells' ranges are loaded together before any is scanned
+    // the cells' ranges are loaded together before any is scanned (an
+    // unused or repeated cell's range is empty)
     uint2 se[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) se[k] = v[k] ? cells[b[k]] : make_uint2(0u, 0u);
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      if constexpr (ALL) {
-        grid_range_scan_all(ph, se[k], qx, qy, qz, r2, on_photon);  // an unused cell's range is empty
-      } else {
-        if (v[k]) grid_range_scan(ph, se[k], qx, qy, qz, r2, on_photon);
-      }
-    }
+    grid_walk8<ALL>(ph, se, qx, qy, qz, r2, on_photon);
     return true;
   }
   // three cells on some axis (float rounding at the edge of the bound): the
@@ -1808,7 +1854,7 @@ __device__ __forceinline__ bool shade_gather(const GridView& g, const uint2* cel
   const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
   const int f = PF >= 0 ? PF : pf;
   return grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
-                          [&](uint32_t, float4 pm, float4 dm, float d2, bool in) {
+                          [&](uint32_t, float4 pm, float4 dm, float d2, bool in, bool) {
     const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
     const float v = (in && c < 0.0f) ? -c * filter_weight(f, d2 * inv_r2) * dm.w : 0.0f;
     const uint32_t wl = __float_as_uint(pm.w) & 3u;
@@ -2315,10 +2361,11 @@ __global__ __launch_bounds__(256) void k_grid_query(const uint2* cells, const Ph
   uint32_t cnt = 0;
   const uint64_t off = mode == 1 ? offsets[i] : 0;
   const bool ok = grid_query<true>(g, cells, ph, qp.x, qp.y, qp.z, r2f,
-                                   [&](uint32_t j, float4, float4, float d2, bool in) {
+                                   [&](uint32_t j, float4, float4, float d2, bool in, bool valid) {
                                      float t;
                                      const bool keep =
-                                         refine ? in_ball(d2, r2f, band, true, ph64, j, qx, qy, qz, r2, &t) : in;
+                                         refine ? valid && in_ball(d2, r2f, band, true, ph64, j, qx, qy, qz, r2, &t)
+                                                : in;
                                      if (keep) {
                                        if (mode == 1) idx_out[off + cnt] = src_idx[j];
                                        ++cnt;
