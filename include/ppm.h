@@ -397,7 +397,7 @@ ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
  * own session would produce.  Rounds (one pass of every running scene) are
  * spread over K pipelines as a session's passes are: round r runs on
  * pipeline r mod K, with its own buffers in every scene, so K rounds are in
- * flight at once.  K = the smallest cfgs[i].pipelines, 2 where that is 0. */
+ * flight at once.  K = the smallest cfgs[i].pipelines, 3 where that is 0. */
 typedef struct ppm_batch ppm_batch;
 
 /* Launch sizes of a batch (0 = the defaults 16 and 8). */

@@ -256,7 +256,7 @@ enum BatchKind : int {
   BK_TRACE = 0,  // photon tracing waves / 4
   BK_MORTON,     // photon grid keys
   BK_PSORT,      // photon sort blocks (hist, scatter)
-  BK_SCAN,       // 256 digit blocks of a sort's scan
+  BK_SCAN,       // one block per scene for a sort's scan of its 256 digit rows
   BK_LEAVES,     // grid cell ranges and record permutation (photons / 256)
   BK_CLEAR,      // grid bucket clear
   BK_EYE,        // eye waves / 4
