@@ -212,7 +212,8 @@ hipError_t map_alloc(MapBuf& m, uint64_t cap, int n_cu, bool grid) {
   m.sort_blocks = (uint32_t)std::min<uint64_t>(4096, std::max<uint64_t>(1, std::min<uint64_t>((uint64_t)n_cu * 4,
                                                                                                 (cap + 2047) / 2048)));
   hipError_t e;
-  if ((e = dalloc(&m.sorted, cap)) != hipSuccess) return e;
+  // the grid's gathers read up to GRID_READ_SLACK records past the last photon
+  if ((e = dalloc(&m.sorted, cap + GRID_READ_SLACK)) != hipSuccess) return e;
   if ((e = dalloc(&m.src_idx, cap)) != hipSuccess) return e;
   if (grid) {
     m.cells_max = (uint32_t)std::min<uint64_t>(GRID_MAX_BUCKETS, next_pow2(std::max<uint64_t>(cap, 1024)));

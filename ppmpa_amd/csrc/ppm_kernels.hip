@@ -1077,14 +1077,18 @@ __global__ __launch_bounds__(256) void k_grid_cells(const uint32_t* keys, const 
   k_grid_cells_body(keys, st, cells, Blk{blockIdx.x, gridDim.x});
 }
 
-// Photon records (and double positions, if any) in bucket order.
+// Photon records (and double positions, if any) in bucket order, in the
+// grid's gather form (GridRec), and the GRID_READ_SLACK records after the
+// last one zeroed.
 __device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const double* in64, const uint32_t* order,
                                                     PhotonRec* out, double* out64, uint32_t* src_idx,
                                                     const PassState* st, Blk blk) {
   const uint32_t n = st->n_stored;
-  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
+  if (t < GRID_READ_SLACK) out[n + t] = PhotonRec{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+  for (uint32_t i = t; i < n; i += blk.n * blockDim.x) {
     const uint32_t src = order[i];
-    out[i] = in[src];
+    out[i] = grid_rec(in[src]);
     src_idx[i] = src;
     if (in64) {
       out64[3 * (uint64_t)i + 0] = in64[3 * (uint64_t)src + 0];
@@ -1188,11 +1192,12 @@ __device__ __forceinline__ void grid_range_scan_all(const PhotonRec* ph, uint2 s
 // the caustic workload (a model of its photons and Morton-ordered queries)
 // 361 against 275 four-photon steps per wave of 64 queries.  Photons are
 // visited in the same order as range by range, so every sum is formed as
-// before.  Four photons per step (their loads in flight together); a step
-// that runs past its range's end reads the range's last photon again for
-// the missing slots and marks them not valid.  ALL: on_photon(j, pm, dm,
-// d2, inside, valid) for every slot, inside only when valid; otherwise
-// on_photon(j, pm, dm, d2) for the valid photons inside the ball.
+// before.  Four consecutive records per step, loaded together from one
+// address; the slots of a step that runs past its range's end (into the
+// next bucket's records, or the GRID_READ_SLACK zeroed ones after the last)
+// are marked not valid.  ALL: on_photon(j, pm, dm, d2, inside, valid) for
+// every slot, inside only when valid; otherwise on_photon(j, pm, dm, d2) for
+// the valid photons inside the ball.
 template <bool ALL, typename OnPhoton>
 __device__ __forceinline__ void grid_walk8(const PhotonRec* ph, const uint2 (&se)[8], float qx, float qy, float qz,
                                            float r2, OnPhoton& on_photon) {
@@ -1207,10 +1212,10 @@ __device__ __forceinline__ void grid_walk8(const PhotonRec* ph, const uint2 (&se
       j = n.x;
       e = n.y;
     }
-    const uint32_t last = e - 1u;
-    const uint32_t i1 = min(j + 1u, last), i2 = min(j + 2u, last), i3 = min(j + 3u, last);
-    const PhotonRec p0 = ph[j], p1 = ph[i1], p2 = ph[i2], p3 = ph[i3];
-    const bool v1 = j + 1u < e, v2 = j + 2u < e, v3 = j + 3u < e;
+    const PhotonRec* p = ph + j;
+    const PhotonRec p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3];
+    const uint32_t i1 = j + 1u, i2 = j + 2u, i3 = j + 3u;
+    const bool v1 = i1 < e, v2 = i2 < e, v3 = i3 < e;
     const float d0 = dist2_nc(qx, qy, qz, p0.pm.x, p0.pm.y, p0.pm.z);
     const float d1 = dist2_nc(qx, qy, qz, p1.pm.x, p1.pm.y, p1.pm.z);
     const float d2 = dist2_nc(qx, qy, qz, p2.pm.x, p2.pm.y, p2.pm.z);
@@ -1879,14 +1884,16 @@ __device__ __forceinline__ bool shade_gather(const GridView& g, const uint2* cel
                                              float r2, float inv_r2, int pf, float& ar, float& ag, float& ab) {
   const float nx = qq.b.x, ny = qq.b.y, nz = qq.b.z;
   const int f = PF >= 0 ? PF : pf;
+  // grid records (grid_rec): the direction carries the weight, pm.w and dm.w
+  // flag channels 0 and 1; v * 1 and v * 0 are exact, so each channel's sum
+  // is formed as with a select
   return grid_query<true>(g, cells, ph, qq.a.x, qq.a.y, qq.a.z, r2,
                           [&](uint32_t, float4 pm, float4 dm, float d2, bool in, bool) {
     const float c = nx * dm.x + ny * dm.y + nz * dm.z;  // optics.rs:225
-    const float v = (in && c < 0.0f) ? -c * filter_weight(f, d2 * inv_r2) * dm.w : 0.0f;
-    const uint32_t wl = __float_as_uint(pm.w) & 3u;
-    ar += wl == 0 ? v : 0.0f;
-    ag += wl == 1 ? v : 0.0f;
-    ab += wl == 2 ? v : 0.0f;
+    const float v = (in && c < 0.0f) ? -c * filter_weight(f, d2 * inv_r2) : 0.0f;
+    ar = fmaf(v, pm.w, ar);
+    ag = fmaf(v, dm.w, ag);
+    ab = fmaf(v, 1.0f - pm.w - dm.w, ab);
   });
 }
 
@@ -2071,11 +2078,11 @@ __global__ __launch_bounds__(256) void k_adj_gather(SceneView sc, const uint2* c
                              // the channel factors by value: a conditional over referenced
                              // floats selects an address, which puts the closure in scratch
                              [&, fr, fg, fb](uint32_t j, float4 pm, float4 dm, float d2) {
+                               // grid records (grid_rec): weighted direction, channel flags
                                const float c = nx * dm.x + ny * dm.y + nz * dm.z;
                                if (c < 0.0f) {
-                                 const uint32_t wl = __float_as_uint(pm.w) & 3u;
-                                 const float f = wl == 0 ? fr : (wl == 1 ? fg : fb);
-                                 const float a = f * (-c * filter_weight(pf, d2 * inv_r2) * dm.w);
+                                 const float f = pm.w != 0.0f ? fr : (dm.w != 0.0f ? fg : fb);
+                                 const float a = f * (-c * filter_weight(pf, d2 * inv_r2));
                                  if (a != 0.0f) atomicAdd(&adj[j], a);
                                }
                              });

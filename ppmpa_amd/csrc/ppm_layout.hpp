@@ -123,6 +123,23 @@ __host__ __device__ inline uint32_t pack_meta(uint32_t wl, uint32_t depth, uint3
   return (wl & 3u) | ((depth & 31u) << 2) | (pid << 7);
 }
 
+// The per-pass grid keeps its sorted copy in a gather form of the same 32 B:
+// pm = (x, y, z, [channel 0]), dm = (w dx, w dy, w dz, [channel 1]), the
+// flags 1.0f or 0.0f.  A gather then adds v * flag per channel (channel 2:
+// 1 - both flags), exact for flags of 0 and 1, in place of unpacking the
+// meta bits and selecting, and takes the weight with the direction.  The
+// index and depth stay in the map's src_idx and the trace's own records.
+// Gathers read four records per step from any start inside a cell's range:
+// up to GRID_READ_SLACK records past the last photon, which the grid build
+// keeps allocated and zeroed.
+constexpr uint32_t GRID_READ_SLACK = 3;
+__device__ inline PhotonRec grid_rec(const PhotonRec& r) {
+  const uint32_t wl = __float_as_uint(r.pm.w) & 3u;
+  const float w = r.dm.w;
+  return PhotonRec{make_float4(r.pm.x, r.pm.y, r.pm.z, wl == 0u ? 1.0f : 0.0f),
+                   make_float4(r.dm.x * w, r.dm.y * w, r.dm.z * w, wl == 1u ? 1.0f : 0.0f)};
+}
+
 // --- tree node (32 B) ------------------------------------------------------
 struct alignas(16) Node {
   float4 lo;  // xyz min
