@@ -232,7 +232,7 @@ __device__ __forceinline__ void sphere_hit(V3 c, float rr, int32_t idx, V3 o, V3
   float t0 = dot(oc, d);
   float t1 = rr - (dot(oc, oc) - t0 * t0);
   if (t1 > 0.0f) {
-    float t2 = sqrtf(t1);
+    float t2 = __builtin_amdgcn_sqrtf(t1);  // v_sqrt_f32, ~1 ulp
     if (t2 == 0.0f) {
       take(h, t0, idx);
     } else {
@@ -493,7 +493,7 @@ __device__ __forceinline__ float schlick(float f0, float c) { return f0 + (1.0f 
 __device__ __forceinline__ V3 diffuse_reflection(V3 n, Rng& rng) {
   float phi = TWO_PI_F * rng.next();
   float xi = 2.0f * rng.next() - 1.0f;
-  float xi2 = sqrtf(fmaxf(1.0f - xi * xi, 0.0f));
+  float xi2 = __builtin_amdgcn_sqrtf(fmaxf(1.0f - xi * xi, 0.0f));
   const float sp = __sinf(phi), cp = __cosf(phi);  // = __sincosf, without its pointer outputs
   V3 d = normalize_or(mk(xi2 * cp, xi, xi2 * sp), mk(0.0f, 1.0f, 0.0f));
   return sel(dot(n, d) > 0.0f, d, -d);
@@ -1015,15 +1015,17 @@ __device__ __forceinline__ float box_dist2_nc(float qx, float qy, float qz, floa
 
 // filter_cone (tracer.rs:197-204) and filter_gauss (tracer.rs:206-216) as
 // functions of t = d2 / r2 (both reference filters depend on d2 only through
-// that ratio).
+// that ratio).  Hardware square root and exponential (a few ulp) and
+// products by the constants' reciprocals: the weight is formed per photon
+// in the gather, and these keep it within 1e-6 of the exact one.
 __device__ __forceinline__ float filter_weight(int pf, float t) {
   if (pf == FILTER_CONE) {
-    const float d = sqrtf(t) / K_CONE_F;
-    return d > 1.0f ? 0.0f : (1.0f - d) / FAC_K_F;
+    const float d = __builtin_amdgcn_sqrtf(t) * (1.0f / K_CONE_F);
+    return d > 1.0f ? 0.0f : (1.0f - d) * (1.0f / FAC_K_F);
   }
   if (pf == FILTER_GAUSS) {
-    const float e_r = 1.0f - expf(-GAUSS_BETA_F * t * 0.5f);
-    return e_r > GAUSS_E_BETA_F ? 0.0f : GAUSS_ALPHA_F * (1.0f - e_r / GAUSS_E_BETA_F) + GAUSS_CORR_F;
+    const float e_r = 1.0f - __expf(-GAUSS_BETA_F * t * 0.5f);
+    return e_r > GAUSS_E_BETA_F ? 0.0f : GAUSS_ALPHA_F * (1.0f - e_r * (1.0f / GAUSS_E_BETA_F)) + GAUSS_CORR_F;
   }
   return 1.0f;
 }
