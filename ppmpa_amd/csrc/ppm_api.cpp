@@ -717,6 +717,15 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
     o.d2 = f4(l.dir2);
     double dn[3];
     o.dir = norm3(l.dir, dn) ? f4(dn) : f4(l.dir);
+    if (o.kind == LIGHT_PARALLELOGRAM) {
+      static const float uv[5] = {0.1f, 0.3f, 0.5f, 0.7f, 0.9f};
+      for (int k = 0; k < 25; ++k) {
+        const float u = uv[k / 5], v = uv[k % 5];
+        o.tss[k] = make_float4(std::fmaf(v, o.d2.x, std::fmaf(u, o.d1.x, o.pos.x)),
+                               std::fmaf(v, o.d2.y, std::fmaf(u, o.d1.y, o.pos.y)),
+                               std::fmaf(v, o.d2.z, std::fmaf(u, o.d1.z, o.pos.z)), 0.0f);
+      }
+    }
     sc->total_flux += l.flux;
   }
   SceneBuf& b = sc->buf;

@@ -920,12 +920,11 @@ __device__ __forceinline__ C3 light_radiance(const DevLight& L, float sqd) {
 // cast for them.
 // Sample s of the TSS grid (light.rs:164-170), row-major in the first
 // coordinate: (0.1 + 0.2 (s / 5), 0.1 + 0.2 (s % 5)) as the float literals
-// 0.1, 0.3, 0.5, 0.7, 0.9.
-__device__ __forceinline__ float tss(int i) {
-  return i == 0 ? 0.1f : (i == 1 ? 0.3f : (i == 2 ? 0.5f : (i == 3 ? 0.7f : 0.9f)));
-}
-__device__ __forceinline__ V3 tss_point(const DevLight& L, int s) {
-  return mk(L.pos) + tss(s / 5) * mk(L.d1) + tss(s % 5) * mk(L.d2);
+// 0.1, 0.3, 0.5, 0.7, 0.9; the light record holds the 25 points, formed once
+// when the scene is made (a wave-uniform scalar load per sample).
+__device__ __forceinline__ V3 tss_point(const SceneView& sc, uint32_t li, int s) {
+  const float4 t = ldc(&sc.lights[li].tss[0], (uint32_t)s);
+  return mk(t.x, t.y, t.z);
 }
 
 template <int BV, bool BVH = (BV != 0)>
@@ -957,7 +956,7 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
     }
     C3 prev{0.0f, 0.0f, 0.0f};
     for (int s = 0; s < 25; ++s) {
-      const V3 ld = tss_point(L, s) - p;
+      const V3 ld = tss_point(sc, li, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
       if (!sample_lit<BV>(sc, p, n, ld, cand, need_o, need_s, wu_o, wu_s, ls, csq, sqd)) continue;

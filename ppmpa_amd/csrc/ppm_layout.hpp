@@ -98,6 +98,10 @@ struct alignas(16) DevLight {
   float4 d1;
   float4 d2;
   float4 dir;
+  // a ParallelogramLight's 25 sample points (light.rs:164-170): sample s at
+  // (u, v) = (0.1 + 0.2 (s / 5), 0.1 + 0.2 (s % 5)) is pos + u d1 + v d2,
+  // formed as fma(v, d2, fma(u, d1, pos)) in float
+  float4 tss[25];
 };
 
 struct alignas(16) DevCamera {
