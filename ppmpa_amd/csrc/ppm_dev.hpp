@@ -757,6 +757,54 @@ __device__ __forceinline__ float box_beyond(const Node& bx, V3 nv, V3 q) {
   return dot(nv, mk(x - q.x, y - q.y, z - q.z));
 }
 
+// The pyramid with apex p over the samples' area of light L, the
+// parallelogram of (u, v) in [0.1, 0.9]^2 that holds the TSS points (corner
+// j: a + u_j e1 + v_j e2, counter-clockwise in (u, v)).  Face j: the sides
+// j = 0..3 through p and corners j, j + 1, the base (j = 4) through a, each
+// with its outward normal f through q and a pad on dot(f, x - q) that
+// bounds its rounding: padk times |x||y| of the cross product x X y that
+// forms f, so it holds also where a face is nearly degenerate.
+struct PyrBase {
+  V3 a, e1, e2, kc;
+  float dmax, padk, l1, l2;
+};
+__device__ __forceinline__ PyrBase pyr_base(const DevLight& L, V3 p) {
+  PyrBase b;
+  b.a = mk(L.pos);
+  b.e1 = mk(L.d1);
+  b.e2 = mk(L.d2);
+  b.kc = b.a + 0.5f * b.e1 + 0.5f * b.e2;
+  float dmax = 0.0f;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const V3 r = b.a + (j == 1 || j == 2 ? 0.9f : 0.1f) * b.e1 + (j >= 2 ? 0.9f : 0.1f) * b.e2 - p;
+    dmax = fmaxf(dmax, dot(r, r));
+  }
+  b.dmax = sqrtf(dmax);
+  const float m =
+      fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), fmaxf(fmaxf(fabsf(b.a.x), fabsf(b.a.y)), fabsf(b.a.z))));
+  b.l1 = sqrtf(dot(b.e1, b.e1));
+  b.l2 = sqrtf(dot(b.e2, b.e2));
+  b.padk = 1e-5f * (1.0f + m + b.dmax + b.l1 + b.l2);
+  return b;
+}
+__device__ __forceinline__ void pyr_face(const PyrBase& b, V3 p, int j, V3& f, V3& q, float& pad) {
+  if (j < 4) {
+    const int j1 = (j + 1) & 3;
+    const V3 ra = b.a + (j == 1 || j == 2 ? 0.9f : 0.1f) * b.e1 + (j >= 2 ? 0.9f : 0.1f) * b.e2 - p;
+    const V3 rb = b.a + (j1 == 1 || j1 == 2 ? 0.9f : 0.1f) * b.e1 + (j1 >= 2 ? 0.9f : 0.1f) * b.e2 - p;
+    f = cross(ra, rb);
+    f = sel(dot(f, b.kc - p) > 0.0f, -1.0f * f, f);
+    q = p;
+    pad = b.padk * sqrtf(dot(ra, ra)) * sqrtf(dot(rb, rb));
+  } else {
+    f = cross(b.e1, b.e2);
+    f = sel(dot(f, p - b.a) > 0.0f, -1.0f * f, f);
+    q = b.a;
+    pad = b.padk * b.l1 * b.l2;
+  }
+}
+
 // The candidates (cand, a bit per shape of lin) that a shadow ray from p to a
 // sample of L still has to test.  Both cuts are exact:
 // - a parallelogram, polygon or plane whose plane holds p while the samples'
@@ -783,42 +831,16 @@ __device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight&
   // nothing to drop
   const uint64_t others = self < CAND_BITS ? wcand & ~(1ull << self) : wcand;
   if (others == 0) return;
-  // corner j of the samples' area: a + u_j e1 + v_j e2, counter-clockwise in (u, v)
-  const V3 a = mk(L.pos), e1 = mk(L.d1), e2 = mk(L.d2);
-  const V3 kc = a + 0.5f * e1 + 0.5f * e2;
-  float dmax = 0.0f;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const V3 r = a + (j == 1 || j == 2 ? 0.9f : 0.1f) * e1 + (j >= 2 ? 0.9f : 0.1f) * e2 - p;
-    dmax = fmaxf(dmax, dot(r, r));
-  }
-  dmax = sqrtf(dmax);
-  const float m = fmaxf(fmaxf(fabsf(p.x), fabsf(p.y)), fmaxf(fabsf(p.z), fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z))));
-  const float l1 = sqrtf(dot(e1, e1)), l2 = sqrtf(dot(e2, e2));
-  const float padk = 1e-5f * (1.0f + m + dmax + l1 + l2);
-  // pyramid faces one at a time (few registers held across the shape loop):
-  // sides j = 0..3 through p and corners j, j+1; the base (j = 4) through a.
-  // Outward normals; each pad is scaled by |x||y| of its cross product x X y,
-  // which bounds its rounding error also where the face is nearly degenerate.
+  const PyrBase pb = pyr_base(L, p);
+  const V3 a = pb.a, e1 = pb.e1, e2 = pb.e2;
+  const float dmax = pb.dmax;
+  // pyramid faces one at a time (few registers held across the shape loop)
   uint64_t out = 0;
 #pragma unroll 1
   for (int j = 0; j < 5; ++j) {
     V3 f, q;
     float pad;
-    if (j < 4) {
-      const int j1 = (j + 1) & 3;
-      const V3 ra = a + (j == 1 || j == 2 ? 0.9f : 0.1f) * e1 + (j >= 2 ? 0.9f : 0.1f) * e2 - p;
-      const V3 rb = a + (j1 == 1 || j1 == 2 ? 0.9f : 0.1f) * e1 + (j1 >= 2 ? 0.9f : 0.1f) * e2 - p;
-      f = cross(ra, rb);
-      f = sel(dot(f, kc - p) > 0.0f, -1.0f * f, f);
-      q = p;
-      pad = padk * sqrtf(dot(ra, ra)) * sqrtf(dot(rb, rb));
-    } else {
-      f = cross(e1, e2);
-      f = sel(dot(f, p - a) > 0.0f, -1.0f * f, f);
-      q = a;
-      pad = padk * l1 * l2;
-    }
+    pyr_face(pb, p, j, f, q, pad);
     for (uint64_t w = others; w != 0; w &= w - 1) {
       const uint32_t i = (uint32_t)__builtin_ctzll(w);
       const Node bx = lin_box(sc, i);
