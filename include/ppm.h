@@ -357,6 +357,14 @@ ppm_status  ppm_session_image(ppm_session* s, double* mean_out,
 /* r_i^2 of the most recent pass, read back from the device (synchronises). */
 ppm_status  ppm_session_radius2(ppm_session* s, double* r2_next);
 ppm_status  ppm_session_timings(ppm_session* s, ppm_timings* t);
+/* The eye rays' tiles: width x (64 / width) pixels, one wavefront each
+ * (width 4, 8, 16 or 32; 8 when never set).  No counterpart in the
+ * reference, which traces pixel by pixel (ppmpa.rs:70-80).  A pixel's
+ * radiance is summed over its eye-tree nodes in the order the tile's wave
+ * reaches them, so images at different widths agree to float rounding and
+ * at equal widths bit for bit.  Takes effect from the next pass
+ * (synchronises). */
+ppm_status  ppm_session_set_tile_width(ppm_session* s, uint32_t width);
 
 /* ---- differentiable form (no counterpart in the reference) ---------------- */
 
@@ -400,11 +408,13 @@ ppm_status  ppm_run_passes(ppm_ctx* ctx, const ppm_scene* scene,
  * flight at once.  K = the smallest cfgs[i].pipelines, 3 where that is 0. */
 typedef struct ppm_batch ppm_batch;
 
-/* Launch sizes of a batch (0 = the defaults 16 and 8). */
+/* Launch sizes of a batch (0 = the defaults 16, 8 and 8). */
 typedef struct {
   uint32_t eye_waves_per_cu;     /* persistent eye waves per CU per scene */
   uint32_t shade_blocks_per_cu;  /* gather/shade blocks per CU per scene */
-  uint32_t reserved[2];
+  uint32_t tile_width;           /* eye tiles of width x (64 / width) pixels:
+                                    4, 8, 16 or 32 (ppm_session_set_tile_width) */
+  uint32_t reserved;
 } ppm_launch_cfg;
 
 ppm_status  ppm_batch_create(ppm_ctx* ctx, uint32_t n,
@@ -418,8 +428,9 @@ ppm_status  ppm_batch_run(ppm_batch* b, uint32_t max_rounds);
 ppm_status  ppm_batch_set_launch(ppm_batch* b, const ppm_launch_cfg* lc);
 ppm_status  ppm_batch_get_launch(const ppm_batch* b, ppm_launch_cfg* lc);
 /* Autotuner: time `rounds` passes of the batch under each candidate launch
- * configuration (eye waves 4/8/16/32 x shade blocks 2/4/8/16 per CU), keep
- * the fastest, and reset every scene to before its first pass. */
+ * configuration (eye waves 4/8/16/32 x shade blocks 2/4/8/16 per CU at
+ * 8-pixel-wide tiles, then tile widths 4/16/32 at the fastest of those),
+ * keep the fastest, and reset every scene to before its first pass. */
 ppm_status  ppm_batch_tune(ppm_batch* b, uint32_t rounds, ppm_launch_cfg* best);
 /* Scene i's per-pixel mean and variance, as ppm_session_image. */
 ppm_status  ppm_batch_image(ppm_batch* b, uint32_t i, double* mean_out,

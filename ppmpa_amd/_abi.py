@@ -142,7 +142,8 @@ class LaunchCfg(C.Structure):
     _fields_ = [
         ("eye_waves_per_cu", C.c_uint32),
         ("shade_blocks_per_cu", C.c_uint32),
-        ("reserved", C.c_uint32 * 2),
+        ("tile_width", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 
@@ -187,6 +188,7 @@ PROTOTYPES = {
     "ppm_session_image": (st, [vp, dp, dp, u32p]),
     "ppm_session_radius2": (st, [vp, dp]),
     "ppm_session_timings": (st, [vp, C.POINTER(Timings)]),
+    "ppm_session_set_tile_width": (st, [vp, C.c_uint32]),
     "ppm_run_passes": (st, [vp, vp, C.POINTER(PassCfg), C.c_uint32, dp, dp, C.POINTER(Timings)]),
     "ppm_scene_set_params": (st, [vp, vp, C.c_uint32, dp, dp, C.c_double]),
     "ppm_session_backward": (st, [vp, dp, dp, dp]),

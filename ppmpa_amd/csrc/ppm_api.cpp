@@ -280,11 +280,42 @@ uint64_t photons_needed(uint64_t n_photon, int max_depth, int n_cu) {
   return (uint64_t)w * per * (uint64_t)max_depth;
 }
 
-hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) {
-  const uint32_t tiles = (uint32_t)cam.tiles_x * (uint32_t)((cam.yreso + 7) / 8);
+// Eye tiles of camera cam at tile shape `shift` ((1 << shift) x (64 >> shift)
+// pixels); tiles per row in *tx.
+uint32_t cam_tiles(const DevCamera& cam, uint32_t shift, uint32_t* tx = nullptr) {
+  const uint32_t w = 1u << shift, h = 64u >> shift;
+  const uint32_t x = ((uint32_t)cam.xreso + w - 1u) / w, y = ((uint32_t)cam.yreso + h - 1u) / h;
+  if (tx) *tx = x;
+  return x * y;
+}
+constexpr uint32_t TILE_SHIFT_MIN = 2, TILE_SHIFT_MAX = 5, TILE_SHIFT_DEFAULT = 3;
+uint32_t cam_tiles_max(const DevCamera& cam) {
+  uint32_t m = 0;
+  for (uint32_t k = TILE_SHIFT_MIN; k <= TILE_SHIFT_MAX; ++k) m = std::max(m, cam_tiles(cam, k));
+  return m;
+}
+
+// The eye buffers at tile shape `shift` (sized for any shape by eye_alloc);
+// the tiles' query counts of earlier passes, which only order the eye's
+// work, start again from zero.
+hipError_t eye_set_tiles(EyeBuf& eb, DevCamera& cam, uint32_t shift) {
+  uint32_t tx = 0;
+  const uint32_t tiles = cam_tiles(cam, shift, &tx);
+  if (tiles > eb.tiles_cap) return hipErrorInvalidValue;
+  cam.tile_shift = (int32_t)shift;
+  cam.tiles_x = (int32_t)tx;
+  eb.n_tiles = tiles;
   eb.q_primary = tiles * 64u;
   eb.q_fine = eb.q_primary >= QUERY_FINE_PIXELS ? 1u : 0u;
-  eb.n_tiles = tiles;
+  return hipMemset(eb.tile_count, 0, sizeof(uint32_t) * eb.tiles_cap);
+}
+
+hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) {
+  // buffers for the most tiles any tile shape needs, so that a batch can
+  // tune the shape without reallocating
+  const uint32_t tiles = cam_tiles_max(cam);
+  eb.tiles_cap = tiles;
+  eb.q_primary = tiles * 64u;
   // deeper queries kept per pixel: a glass sphere yields about one per depth
   eb.sec_max = (uint32_t)std::max(1, std::min(2 * max_depth, 32));
   eb.n_pix = (uint32_t)cam.xreso * (uint32_t)cam.yreso;
@@ -309,7 +340,8 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   eb.estack_lanes = 64u * std::max<uint32_t>(1, std::min<uint32_t>(tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU_MAX));
   eb.estack_depth = (uint32_t)std::max(1, std::min(max_depth, MAX_DEPTH_LIMIT));
   if ((e = dalloc(&eb.estack, 3ull * eb.estack_depth * eb.estack_lanes)) != hipSuccess) return e;
-  return hipSuccess;
+  DevCamera c = cam;
+  return eye_set_tiles(eb, c, (uint32_t)cam.tile_shift);
 }
 
 void eye_free(EyeBuf& eb) {
@@ -742,6 +774,7 @@ ppm_status ppm_scene_create(ppm_ctx* c, const ppm_scene_desc* d, ppm_scene** out
   cam.blur = cd.blur;
   cam.pfilter = cd.pfilter;
   cam.tiles_x = (cd.xreso + 7) / 8;
+  cam.tile_shift = (int32_t)TILE_SHIFT_DEFAULT;
   cam.eye = f4(cd.eye_pos);
   cam.origin = f4(cd.origin);
   cam.esx = f4(cd.esx);
@@ -1770,6 +1803,41 @@ ppm_status ppm_session_reset(ppm_session* s) {
   return PPM_OK;
 }
 
+// The tile shift of a tile width (4, 8, 16 or 32 pixels; 0 = 8).
+static bool tile_shift_of(uint32_t width, uint32_t* shift) {
+  if (width == 0) width = 1u << TILE_SHIFT_DEFAULT;
+  for (uint32_t k = TILE_SHIFT_MIN; k <= TILE_SHIFT_MAX; ++k)
+    if (width == (1u << k)) {
+      *shift = k;
+      return true;
+    }
+  return false;
+}
+
+// Every pipeline's eye buffers and the session's camera at tile shape shift
+// (the pipelines idle; their captured passes are captured again).
+static hipError_t session_set_tiles(ppm_session* s, uint32_t shift) {
+  for (Pipe& p : s->pipes) {
+    hipError_t e;
+    if (p.stream && (e = hipStreamSynchronize(p.stream)) != hipSuccess) return e;
+    if (p.s2 && (e = hipStreamSynchronize(p.s2)) != hipSuccess) return e;
+  }
+  for (Pipe& p : s->pipes) {
+    const hipError_t e = eye_set_tiles(p.eb, s->sbuf.cam, shift);
+    if (e != hipSuccess) return e;
+    pipe_drop_graph(p);
+  }
+  return hipSuccess;
+}
+
+ppm_status ppm_session_set_tile_width(ppm_session* s, uint32_t width) {
+  if (!s) return PPM_E_INVALID;
+  uint32_t shift = 0;
+  if (!tile_shift_of(width, &shift)) return fail(s->ctx, PPM_E_INVALID, "tile width must be 4, 8, 16 or 32");
+  HIP_TRY(s->ctx, session_set_tiles(s, shift));
+  return PPM_OK;
+}
+
 ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double* grad_albedo, double* grad_light) {
   if (!s || !grad_image || !grad_albedo) return PPM_E_INVALID;
   ppm_ctx* c = s->ctx;
@@ -1927,6 +1995,13 @@ hipError_t batch_layout(ppm_batch* b) {
   const uint32_t n_cu = (uint32_t)b->ctx->n_cu;
   const uint32_t ew = b->lc.eye_waves_per_cu ? b->lc.eye_waves_per_cu : 16u;
   const uint32_t sb = b->lc.shade_blocks_per_cu ? b->lc.shade_blocks_per_cu : 8u;
+  uint32_t shift = TILE_SHIFT_DEFAULT;
+  if (!tile_shift_of(b->lc.tile_width, &shift)) return hipErrorInvalidValue;
+  for (ppm_session* s : b->sess)
+    if (s->sbuf.cam.tile_shift != (int32_t)shift) {
+      const hipError_t e = session_set_tiles(s, shift);
+      if (e != hipSuccess) return e;
+    }
   std::vector<BatchItem> items((size_t)b->K * n);
   b->htab.assign((size_t)BK_KINDS * (n + 1), 0u);
   for (uint32_t j = 0; j < n; ++j) {
@@ -2138,6 +2213,8 @@ ppm_status ppm_batch_set_launch(ppm_batch* b, const ppm_launch_cfg* lc) {
   if (!b || !lc) return PPM_E_INVALID;
   if (lc->eye_waves_per_cu > 64 || lc->shade_blocks_per_cu > 64)
     return fail(b->ctx, PPM_E_RANGE, "at most 64 eye waves / shade blocks per CU");
+  uint32_t shift = 0;
+  if (!tile_shift_of(lc->tile_width, &shift)) return fail(b->ctx, PPM_E_INVALID, "tile width must be 4, 8, 16 or 32");
   b->lc = *lc;
   HIP_TRY(b->ctx, batch_layout(b));
   return PPM_OK;
@@ -2148,6 +2225,7 @@ ppm_status ppm_batch_get_launch(const ppm_batch* b, ppm_launch_cfg* lc) {
   *lc = b->lc;
   if (!lc->eye_waves_per_cu) lc->eye_waves_per_cu = 16;
   if (!lc->shade_blocks_per_cu) lc->shade_blocks_per_cu = 8;
+  if (!lc->tile_width) lc->tile_width = 8;
   return PPM_OK;
 }
 
@@ -2187,26 +2265,39 @@ ppm_status ppm_batch_tune(ppm_batch* b, uint32_t rounds, ppm_launch_cfg* best) {
   ppm_ctx* c = b->ctx;
   static const uint32_t eye_c[] = {4, 8, 16, 32};
   static const uint32_t shade_c[] = {2, 4, 8, 16};
+  static const uint32_t tile_c[] = {4, 16, 32};
   ppm_launch_cfg pick = b->lc;
   double best_ms = INFINITY;
+  auto trial = [&](const ppm_launch_cfg& lc) -> ppm_status {
+    b->lc = lc;
+    HIP_TRY(c, batch_layout(b));
+    HIP_TRY(c, batch_reset(b));
+    // the first round captures the graph; the next `rounds` are timed
+    if (ppm_status st = ppm_batch_run(b, 1)) return st;
+    const double t0 = b->ms_total;
+    if (ppm_status st = ppm_batch_run(b, rounds)) return st;
+    const double ms = b->ms_total - t0;
+    if (ms < best_ms) {
+      best_ms = ms;
+      pick = lc;
+    }
+    return PPM_OK;
+  };
+  // launch sizes at 8-pixel-wide tiles, then the tile shape at the best sizes
   for (uint32_t ew : eye_c)
     for (uint32_t sb : shade_c) {
       ppm_launch_cfg lc{};
       lc.eye_waves_per_cu = ew;
       lc.shade_blocks_per_cu = sb;
-      b->lc = lc;
-      HIP_TRY(c, batch_layout(b));
-      HIP_TRY(c, batch_reset(b));
-      // the first round captures the graph; the next `rounds` are timed
-      if (ppm_status st = ppm_batch_run(b, 1)) return st;
-      const double t0 = b->ms_total;
-      if (ppm_status st = ppm_batch_run(b, rounds)) return st;
-      const double ms = b->ms_total - t0;
-      if (ms < best_ms) {
-        best_ms = ms;
-        pick = lc;
-      }
+      lc.tile_width = 8;
+      if (ppm_status st = trial(lc)) return st;
     }
+  const ppm_launch_cfg sizes = pick;
+  for (uint32_t tw : tile_c) {
+    ppm_launch_cfg lc = sizes;
+    lc.tile_width = tw;
+    if (ppm_status st = trial(lc)) return st;
+  }
   b->lc = pick;
   HIP_TRY(c, batch_layout(b));
   HIP_TRY(c, batch_reset(b));

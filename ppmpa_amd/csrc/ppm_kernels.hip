@@ -1518,13 +1518,14 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
   const uint32_t cap = 64u * eb.estack_depth;  // pool entries of this wave
   float4* const pool = eb.estack + (uint64_t)w * cap * 3u;
   const uint32_t tx = (uint32_t)sc.cam.tiles_x;
+  const uint32_t ts = (uint32_t)sc.cam.tile_shift, tm = (1u << ts) - 1u;  // tile width 1 << ts
   uint32_t wc = w & 7u, wtries = 0;
   for (uint32_t k = wave_next8(st->work[WORK_EYE], wc, wtries, eb.n_tiles); k < eb.n_tiles;
        k = wave_next8(st->work[WORK_EYE], wc, wtries, eb.n_tiles)) {
     const uint32_t tile = eb.tile_order[k];
-    const uint32_t x0 = (tile % tx) * 8u, y0 = (tile / tx) * 8u;
+    const uint32_t x0 = (tile % tx) << ts, y0 = (tile / tx) << (6u - ts);
     const uint32_t t = tile * 64u + lane;
-    const int x = (int)(x0 + (lane & 7u)), y = (int)(y0 + (lane >> 3));
+    const int x = (int)(x0 + (lane & tm)), y = (int)(y0 + (lane >> ts));
     const bool inside = x < sc.cam.xreso && y < sc.cam.yreso;
     const uint32_t my_pixel = inside ? (uint32_t)y * (uint32_t)sc.cam.xreso + (uint32_t)x : 0u;
     nsec[lane] = 0u;
@@ -1565,7 +1566,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
         top -= take;
       }
       if (!__any(alive)) break;
-      const uint32_t px = (uint32_t)y0 + (own >> 3), qx = x0 + (own & 7u);
+      const uint32_t px = (uint32_t)y0 + (own >> ts), qx = x0 + (own & tm);
       const uint32_t pixel = px * (uint32_t)sc.cam.xreso + qx;
       const uint32_t to = tile * 64u + own;  // the pixel's primary slot
       bool next_set = false, push = false, want_q = false;
@@ -2185,9 +2186,9 @@ __global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState
 __device__ __forceinline__ void k_accumulate_body(EyeBuf eb, DevCamera cam, double* sum, double* sumsq, Blk blk) {
   const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   if (t >= eb.q_primary) return;
-  const uint32_t g = t >> 6, wi = t & 63u;
-  const int x = (int)((g % (uint32_t)cam.tiles_x) * 8u + (wi & 7u));
-  const int y = (int)((g / (uint32_t)cam.tiles_x) * 8u + (wi >> 3));
+  const uint32_t g = t >> 6, wi = t & 63u, ts = (uint32_t)cam.tile_shift;
+  const int x = (int)(((g % (uint32_t)cam.tiles_x) << ts) + (wi & ((1u << ts) - 1u)));
+  const int y = (int)(((g / (uint32_t)cam.tiles_x) << (6u - ts)) + (wi >> ts));
   if (x >= cam.xreso || y >= cam.yreso) return;
   const uint32_t p = (uint32_t)y * (uint32_t)cam.xreso + (uint32_t)x;
   float4 c = eb.pix[p];

@@ -93,6 +93,7 @@ struct EyeBuf {
   float4* pix = nullptr;  // per-pixel float radiance of the pass (row-major)
   uint32_t n_pix = 0;
   uint32_t n_tiles = 0;
+  uint32_t tiles_cap = 0;  // the buffers hold the tiles of any tile shape
   uint32_t* tile_count = nullptr;   // queries per tile (of the last pass until k_tile_count)
   uint32_t* tile_off = nullptr;
   uint32_t* tile_order = nullptr;   // the eye kernel's tile order (k_tile_order)

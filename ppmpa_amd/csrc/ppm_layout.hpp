@@ -108,7 +108,9 @@ struct alignas(16) DevCamera {
   int32_t xreso, yreso;
   int32_t progressive, antialias;
   int32_t blur, pfilter;
-  int32_t tiles_x, pad;
+  // the eye's tiles: (1 << tile_shift) x (64 >> tile_shift) pixels, one per
+  // wave, tiles_x of them per row (tile_shift 2..5: 4x16, 8x8, 16x4, 32x2)
+  int32_t tiles_x, tile_shift;
   float4 eye, origin, esx, esy, eex, eey;
 };
 

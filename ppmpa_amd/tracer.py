@@ -353,6 +353,11 @@ class Session:
         self.ctx.check(self.ctx.lib.ppm_session_radius2(self.h, C.byref(r)))
         return r.value
 
+    def set_tile_width(self, width):
+        """Eye tiles of width x (64 / width) pixels, width 4, 8, 16 or 32
+        (ppm_session_set_tile_width)."""
+        self.ctx.check(self.ctx.lib.ppm_session_set_tile_width(self.h, int(width)))
+
     def reset(self):
         """Back to before the first pass, with the scene's current parameters
         (ppm_session_reset)."""
@@ -421,17 +426,26 @@ class Batch:
         self.ctx.check(self.ctx.lib.ppm_batch_get_launch(self.h, C.byref(lc)))
         return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu}
 
-    def set_launch(self, eye_waves_per_cu=0, shade_blocks_per_cu=0):
+    def set_launch(self, eye_waves_per_cu=0, shade_blocks_per_cu=0, tile_width=0):
         lc = A.LaunchCfg()
         lc.eye_waves_per_cu, lc.shade_blocks_per_cu = int(eye_waves_per_cu), int(shade_blocks_per_cu)
+        lc.tile_width = int(tile_width)
         self.ctx.check(self.ctx.lib.ppm_batch_set_launch(self.h, C.byref(lc)))
 
+    def launch(self):
+        lc = A.LaunchCfg()
+        self.ctx.check(self.ctx.lib.ppm_batch_get_launch(self.h, C.byref(lc)))
+        return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu,
+                "tile_width": lc.tile_width}
+
     def tune(self, rounds=2):
-        """Autotune the launch sizes on `rounds` timed passes per candidate;
-        every scene is reset to before its first pass afterwards."""
+        """Autotune the launch sizes and the eye's tile shape on `rounds`
+        timed passes per candidate; every scene is reset to before its first
+        pass afterwards."""
         lc = A.LaunchCfg()
         self.ctx.check(self.ctx.lib.ppm_batch_tune(self.h, int(rounds), C.byref(lc)))
-        return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu}
+        return {"eye_waves_per_cu": lc.eye_waves_per_cu, "shade_blocks_per_cu": lc.shade_blocks_per_cu,
+                "tile_width": lc.tile_width}
 
     def image(self, i, with_var=True):
         cam = self.dscenes[i].scene.camera

@@ -33,10 +33,11 @@ def _cfg(c, pipelines=2):
                       max_depth=c["max_depth"], pipelines=pipelines)
 
 
-def _plain_images(c, items, dss, pipelines=2):
+def _plain_images(c, items, dss, pipelines=2, tile_width=8):
     imgs = []
     for (sc, cfg, passes), ds in zip(items, dss):
         s = P.Session(c, ds, _cfg(cfg, pipelines))
+        s.set_tile_width(tile_width)
         s.run(passes)
         m, v, k = s.image()
         s.close()
@@ -78,17 +79,19 @@ def test_batch_mixed_bvh_and_tuning():
     items.append((many[0], dict(many[1], n_photon=20000), 3))
     dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
     assert any(ds.stats()[1] > 0 for ds in dss) and any(ds.stats()[1] == 0 for ds in dss)
-    plain = _plain_images(c, items, dss)
     b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
     best = b.tune(rounds=1)
     assert best["eye_waves_per_cu"] in (4, 8, 16, 32) and best["shade_blocks_per_cu"] in (2, 4, 8, 16)
+    assert best["tile_width"] in (4, 8, 16, 32) and b.launch() == best
+    # sessions with the tuned tile shape give the batch's images bit for bit
+    plain = _plain_images(c, items, dss, tile_width=best["tile_width"])
     b.run()
     for i, (_, _, passes) in enumerate(items):
         m, _, k = b.image(i, with_var=False)
         assert k == passes and _same(m, plain[i][0]), i
     # partial runs: rounds continue where they stopped
     b2 = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
-    b2.set_launch(eye_waves_per_cu=4, shade_blocks_per_cu=2)
+    b2.set_launch(eye_waves_per_cu=4, shade_blocks_per_cu=2, tile_width=best["tile_width"])
     b2.run(max_rounds=1)
     b2.run()
     for i in range(len(items)):
@@ -157,15 +160,56 @@ def test_batch_full_size_rooms_equal_sessions():
     items = [(sc, cfg, min(p, 24)) for sc, cfg, p in P.batch_scenes(n=64, seed=1)]
     dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
     b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
-    b.tune(rounds=1)
+    tw = b.tune(rounds=1)["tile_width"]
     b.run()
     for i, ((sc, cfg, passes), ds) in enumerate(zip(items, dss)):
         m, _, k = b.image(i, with_var=False)
         assert k == passes
         s = P.Session(c, ds, _cfg(cfg))
+        s.set_tile_width(tw)
         s.run(passes)
         m0, _, _ = s.image(with_var=False)
         s.close()
         assert m0.mean() > 0
         assert _same(m, m0), i
     b.close()
+
+
+def test_tile_widths():
+    """The eye's tile shape (4x16, 8x8, 16x4, 32x2 pixels per wave) changes
+    the order in which a pixel's eye-tree terms are summed, nothing else:
+    images agree with the 8x8 one to float rounding (the per-pixel sums of a
+    few terms, 1e-6 relative), a session and a batch at the same width bit
+    for bit, and switching the width between passes keeps the run going."""
+    say("tile widths")
+    c = ctx()
+    items = _small_batch(6, 21)
+    dss = [P.DeviceScene(c, sc) for sc, _, _ in items]
+    ref = _plain_images(c, items, dss, tile_width=8)
+    for tw in (4, 16, 32):
+        got = _plain_images(c, items, dss, tile_width=tw)
+        for i in range(len(items)):
+            assert np.allclose(got[i][0], ref[i][0], rtol=1e-6, atol=1e-12), (tw, i)
+        b = P.Batch(c, dss, [_cfg(cfg) for _, cfg, _ in items], [p for _, _, p in items])
+        b.set_launch(tile_width=tw)
+        assert b.launch()["tile_width"] == tw
+        b.run()
+        for i in range(len(items)):
+            m, _, _ = b.image(i, with_var=False)
+            assert _same(m, got[i][0]), (tw, i)
+        b.close()
+    # a width change part-way: the same passes as at 8 throughout, to rounding
+    sc, cfg, passes = items[0]
+    s = P.Session(c, dss[0], _cfg(cfg))
+    s.run(1)
+    s.set_tile_width(32)
+    s.run(passes - 1)
+    m, _, k = s.image(with_var=False)
+    s.close()
+    assert k == passes and np.allclose(m, ref[0][0], rtol=1e-6, atol=1e-12)
+    with pytest.raises(Exception):
+        s2 = P.Session(c, dss[0], _cfg(cfg))
+        try:
+            s2.set_tile_width(12)
+        finally:
+            s2.close()
