@@ -339,6 +339,17 @@ ppm_status  ppm_trace_eye(ppm_ctx* ctx, const ppm_scene* scene,
                           double r2, int32_t use_classic, int32_t max_depth,
                           int32_t pfilter, double* rgb_out);
 
+/* rtc (rtc.rs:17-47): trace_ray_classic (tracer.rs:221-259) for every pixel
+ * of the camera's screen, a Whitted ray tracer with no photon map: at each
+ * hit the direct light of every light (tracer.rs:263-290) plus the ambient
+ * radiance (camera.rs:40, "ambient" of the camera file), mirror reflection
+ * and refraction to depth 10.  The camera ray draws its blur / antialias
+ * offsets from the pixel's stream (seed, pass) as ppm_trace_eye does.
+ * rgb_out: yreso x xreso x 3 doubles, row-major. */
+ppm_status  ppm_trace_classic(ppm_ctx* ctx, const ppm_scene* scene,
+                              uint64_t seed, uint32_t pass,
+                              const double ambient[3], double* rgb_out);
+
 /* ---- PPM-PA passes (H8) -------------------------------------------------- */
 
 /* util/iterator.rb + util/averager2.rb in one device-resident loop: pass i

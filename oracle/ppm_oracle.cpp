@@ -23,7 +23,9 @@
 //   gather: uniform-carpet closed form;
 //   radius schedule: Gamma-function closed form;
 //   photon walk / eye tree: invariants (storage rule, albedo-0 and albedo-1
-//   boxes, hemisphere moments, channel frequencies).
+//   boxes, hemisphere moments, channel frequencies);
+//   trace_ray_classic: closed forms (ambient on a Lambertian, two parallel
+//   mirrors to the depth limit, glass slabs at 60 degrees).
 //   The rand 0.6 stream contents and the kdtree crate's tie order are
 //   unpinned by design (nothing in the reference fixes them).
 #include <algorithm>
@@ -835,6 +837,33 @@ static C trace_ray(const EyeCtx& ec, int m0, int l, uint32_t path, V o, V d, C w
   return mate.em * SR_HALF + s_bsdf(mate, cos, di, si, ti);
 }
 
+// trace_ray_classic (tracer.rs:221-259): the Whitted tree of rtc, with no
+// photon map and no random draw below the camera ray.  di = the direct
+// light of every light plus the ambient radiance; the mirror direction for
+// the reflection, the refraction about the normal (tdir's side decides the
+// child's medium, :247), the BSDF at cos1.
+static C trace_ray_classic(const Scene& sc, int m0, int l, V o, V d, C ambient) {
+  if (l >= 10) return C{0, 0, 0};  // tracer.rs:222
+  Isect is;
+  if (!calc_intersection(sc, o, d, is)) return C{0, 0, 0};
+  const Material& mate = sc.mats[is.mat];
+  Flt cos1;
+  V rdir = specular_reflection(is.n, d, cos1);
+  C di{0, 0, 0};
+  for (const Light& lt : sc.lights) di = di + radiance_from_light(sc, is.pos, is.n, lt);
+  di = di + ambient;
+  C si{0, 0, 0}, ti{0, 0, 0};
+  if (s_reflect(mate, cos1)) si = trace_ray_classic(sc, m0, l + 1, is.pos, rdir, ambient);
+  Flt eta = relative_ior_average(sc.mats[m0].ior, mate.ior);
+  V tdir;
+  Flt cos2;
+  if (specular_refraction(is.n, d, eta, tdir, cos2) && s_refract(mate, cos1)) {
+    int m02 = dot(tdir, is.n) < 0.0 ? is.mat : sc.air;
+    ti = trace_ray_classic(sc, m02, l + 1, is.pos, tdir, ambient);
+  }
+  return mate.em * SR_HALF + s_bsdf(mate, cos1, di, si, ti);
+}
+
 // Camera::generate_ray (camera.rs:58-75)
 static void generate_ray(const Camera& cam, int x, int y, Rng& rng, V& o, V& d) {
   V blur{0, 0, 0};
@@ -1194,6 +1223,23 @@ void orc_trace_eye(const orc_scene* s, const orc_map* m, uint64_t seed, uint32_t
       C c = eye_pixel(sc, m->kd, seed, pass, r2, uc != 0, maxd, pf, x, y, buf);
       rgb[3 * p] = c.r; rgb[3 * p + 1] = c.g; rgb[3 * p + 2] = c.b;
     }
+  }
+}
+
+// rtc (rtc.rs:17-47): trace_ray_classic for every pixel, the camera ray
+// drawn from the pixel's stream (seed, pass): row-major image.
+void orc_trace_classic(const orc_scene* s, uint64_t seed, uint32_t pass, const double ambient[3], double* rgb) {
+  const Scene& sc = s->sc;
+  const int W = sc.cam.xreso, H = sc.cam.yreso;
+  const C amb = cd(ambient);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t p = 0; p < (int64_t)W * H; ++p) {
+    int y = (int)(p / W), x = (int)(p % W);
+    Rng root(seed, (uint32_t)p, pass, TAG_EYE);
+    V o, d;
+    generate_ray(sc.cam, x, y, root, o, d);
+    C c = trace_ray_classic(sc, sc.air, 0, o, d, amb);
+    rgb[3 * p] = c.r; rgb[3 * p + 1] = c.g; rgb[3 * p + 2] = c.b;
   }
 }
 

@@ -9,6 +9,6 @@ from .scene import (Camera, Light, Material, Mesh, Scene, Shape, CONFIGS, two_sp
                     glass, lambertian, mirror, parallelogram, parallelogram_light, pinhole, plain, point_light,
                     polygon, read_camera, reference_room, simple, sphere, sun_light, ts)
 from .tracer import (Batch, Context, DeviceScene, PhotonMap, PhotonSet, PpmError, Session, build_photonmap, pass_cfg,
-                     photon_power, read_map, run_passes, trace_eye, trace_photons)
+                     photon_power, read_map, run_passes, trace_classic, trace_eye, trace_photons)
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -180,6 +180,7 @@ PROTOTYPES = {
     "ppm_radius_query": (st, [vp, vp, dp, C.c_uint64, C.c_double, C.c_int32, C.c_uint64, u32p, u64p]),
     "ppm_gather": (st, [vp, vp, dp, dp, C.c_uint64, C.c_double, C.c_int32, C.c_int32, dp]),
     "ppm_gather_noalloc": (st, [vp, vp, dp, dp, C.c_uint64, C.c_double, C.c_int32, C.c_int32, dp]),
+    "ppm_trace_classic": (st, [vp, vp, C.c_uint64, C.c_uint32, dp, dp]),
     "ppm_trace_eye": (st, [vp, vp, vp, C.c_uint64, C.c_uint32, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, dp]),
     "ppm_session_create": (st, [vp, vp, C.POINTER(PassCfg), C.POINTER(vp)]),

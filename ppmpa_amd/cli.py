@@ -6,6 +6,9 @@
       eye pass over the photon map read from stdin (read_map,
       photonmap.rs:31-74): pnm header then radiance (progressive camera) or
       RGB lines (rt.rs:47-55)
+  python -m ppmpa_amd.rtc [-h] <screen file> <scene file>             rtc.rs:17-47
+      the classic ray tracer (trace_ray_classic, tracer.rs:221-259), no
+      photons: pnm header then radiance (progressive camera) or RGB lines
   python -m ppmpa_amd.ppmpa [-nc|-h] <#photon> <radius> <camera file> <scene file>
       one in-process pass (ppmpa.rs:16-41): header + radiance lines
   python -m ppmpa_amd.iterate [-ppm|-exr <file>] <#photon> <#iteration> <radius> <screen> <scene>
@@ -96,6 +99,21 @@ def rt_main(argv=None):
     img = T.trace_eye(ctx, ds, pmap, a.seed, a.pass_, r * r, use_classic=True, max_depth=MAX_TRACE,
                       pfilter=cam.pfilter)
     sys.stdout.write(_header_and_body(img, cam, cfg))
+    return 0
+
+
+def rtc_main(argv=None):
+    ap = argparse.ArgumentParser(prog="rtc", description="classic ray tracer: direct light + ambient, no photon map")
+    ap.add_argument("screen")
+    ap.add_argument("scene")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--pass", dest="pass_", type=int, default=1)
+    a = ap.parse_args(argv)
+    sc, cam, cfg = _camera_scene(a.scene, a.screen)
+    ctx = _ctx()
+    ds = T.DeviceScene(ctx, sc)
+    img = T.trace_classic(ctx, ds, a.seed, a.pass_, cfg["ambient"])
+    sys.stdout.write(_header_and_body(img, cam, cfg))  # rtc.rs:25-27,32-39
     return 0
 
 

@@ -1480,6 +1480,42 @@ ppm_status ppm_trace_eye(ppm_ctx* c, const ppm_scene* sc, const ppm_map* m, uint
   return PPM_OK;
 }
 
+ppm_status ppm_trace_classic(ppm_ctx* c, const ppm_scene* sc, uint64_t seed, uint32_t pass, const double ambient[3],
+                             double* rgb_out) {
+  if (!c || !sc || !ambient || !rgb_out) return fail(c, PPM_E_INVALID, "null argument");
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(ambient[k])) return fail(c, PPM_E_INVALID, "bad ambient radiance");
+  EyeBuf eb;
+  PassState* st = nullptr;
+  hipError_t e = hipSuccess;
+  std::vector<float4> h;
+  PassState hs = host_state();
+  do {
+    if ((e = eye_alloc(eb, sc->buf.cam, CLASSIC_DEPTH, c->n_cu)) != hipSuccess) break;
+    if ((e = dalloc(&st, 1)) != hipSuccess) break;
+    hs.pass = pass;
+    if ((e = hipMemcpy(st, &hs, sizeof(hs), hipMemcpyHostToDevice)) != hipSuccess) break;
+    const float4 amb = make_float4((float)ambient[0], (float)ambient[1], (float)ambient[2], 0.0f);
+    if ((e = launch_classic(c->stream, sc->buf, st, eb, seed, amb, c->n_cu)) != hipSuccess) break;
+    h.resize(eb.n_pix);
+    if ((e = hipMemcpyAsync(h.data(), eb.pix, sizeof(float4) * eb.n_pix, hipMemcpyDeviceToHost, c->stream)) !=
+        hipSuccess)
+      break;
+    if ((e = hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, c->stream)) != hipSuccess) break;
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) break;
+  } while (0);
+  eye_free(eb);
+  dfree(st);
+  if (e != hipSuccess) return fail(c, e == hipErrorOutOfMemory ? PPM_E_OOM : PPM_E_HIP, hipGetErrorString(e));
+  if (hs.overflow) return fail(c, PPM_E_RANGE, "gather query buffer overflow");
+  for (size_t p = 0; p < h.size(); ++p) {
+    rgb_out[3 * p] = h[p].x;
+    rgb_out[3 * p + 1] = h[p].y;
+    rgb_out[3 * p + 2] = h[p].z;
+  }
+  return PPM_OK;
+}
+
 // ---------------------------------------------------------------------------
 // session
 

@@ -1500,7 +1500,13 @@ __device__ __forceinline__ uint64_t peer_mask(uint32_t key, bool pred) {
 // only on the tile: results are reproducible.  Deeper queries beyond a
 // pixel's sec_max slots go to the overflow region (atomic slots, added in
 // any order).
-template <int BV, bool BVH = (BV != 0)>
+// CL: the classic ray tracer's tree instead (trace_ray_classic,
+// tracer.rs:221-259): the mirror direction itself for the reflection (no
+// glossy draw), the refraction about the surface normal, the child's medium
+// by the side tdir leaves to (tracer.rs:247), and the BSDF at cos1; each
+// node's query then carries the direct light and the camera's ambient
+// instead of a photon estimate (k_shade_classic).
+template <int BV, bool CL = false, bool BVH = (BV != 0)>
 __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves, Blk blk) {
   __shared__ uint32_t s_nsec[4 * 64];  // per wave of the block and pixel of its tile
@@ -1581,25 +1587,30 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
         if ((int)depth < maxd && intersect<BV>(sc, cur.o, cur.d, is)) {
           const DevMaterial M = material(sc, is.mat);
           const DevMaterial Mm = material(sc, med);
-          Rng nr;
-          nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
-          if (depth == 0)
-            for (uint32_t j = 0; j < cam_draws; ++j) (void)nr.next();
           emit = cur.w * (c3(M.em) * SR_HALF_F);
           float cos1;
           V3 rdir = specular_reflection(is.n, cur.d, cos1);
-          const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
           V3 tdir = mk(0.0f, 0.0f, 0.0f);
           bool has_t = false;
           float cos2 = 0.0f;
-          if (spec_part) {
-            // tracer.rs:150-161
-            rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
-            const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
-            const V3 hvec = normalize_or(rdir - cur.d, is.n);
-            has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+          if constexpr (CL) {
+            // tracer.rs:232,246
+            has_t = specular_refraction(is.n, cur.d, relative_ior(Mm.ior_avg, M.ior_avg), tdir, cos2);
+          } else {
+            const bool spec_part = (M.surface == SURF_SIMPLE && M.diff != 1.0f) || M.surface == SURF_TS;
+            if (spec_part) {
+              // tracer.rs:150-161
+              Rng nr;
+              nr.init(seed, pixel, pass, TAG_EYE | (depth << 16) | path);
+              if (depth == 0)
+                for (uint32_t j = 0; j < cam_draws; ++j) (void)nr.next();
+              rdir = reflection_glossy(is.n, rdir, power_glossy(M), nr);
+              const float eta = relative_ior(Mm.ior_avg, M.ior_avg);
+              const V3 hvec = normalize_or(rdir - cur.d, is.n);
+              has_t = specular_refraction(hvec, cur.d, eta, tdir, cos2);
+            }
           }
-          const float cos0 = cos1 < cos2 ? cos1 : cos2;  // tracer.rs:173
+          const float cos0 = CL ? cos1 : (cos1 < cos2 ? cos1 : cos2);  // tracer.rs:173, :257
           const BsdfW W = bsdf_weights(M, cos0);
           const C3 wd = cur.w * W.wd;
           if (nonzero(wd)) {
@@ -1616,7 +1627,9 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
             const bool rc = surf_reflect(M, cos1) && nonzero(ws);
             const bool tc = has_t && surf_refract(M, cos1) && nonzero(wt);
             if (tc) {
-              const uint32_t tmed = is.inside_to_out ? sc.n_mats : is.mat;  // tracer.rs:164-167
+              // tracer.rs:164-167; classic: tracer.rs:247
+              const uint32_t tmed = CL ? (dot(tdir, is.n) < 0.0f ? is.mat : sc.n_mats)
+                                       : (is.inside_to_out ? sc.n_mats : is.mat);
               const EyeNode c{is.pos, tdir, wt, (depth + 1) | (tmed << 5) | ((path | (1u << depth)) << 16)};
               if (rc) {
                 pc = c;
@@ -1717,6 +1730,11 @@ template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_eye(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed, int uc,
                                              int maxd, uint32_t n_waves) {
   k_eye_body<BV>(sc, st, eb, seed, uc, maxd, n_waves, Blk{blockIdx.x, gridDim.x});
+}
+template <int BV>
+__global__ __launch_bounds__(256) void k_eye_classic(SceneView sc, PassState* st, EyeBuf eb, uint64_t seed,
+                                                     uint32_t n_waves) {
+  k_eye_body<BV, true>(sc, st, eb, seed, 1, CLASSIC_DEPTH, n_waves, Blk{blockIdx.x, gridDim.x});
 }
 
 // Queries kept in place per tile (primary + deeper), for the sort's offsets.
@@ -2287,6 +2305,67 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);
+  return hipGetLastError();
+}
+
+// ===========================================================================
+// The classic ray tracer (rtc.rs:17-47, trace_ray_classic): the eye trees of
+// k_eye_classic, then per query di = the direct light of every light
+// (tracer.rs:234-237) plus the camera's ambient (:238), no photon map;
+// res = w * di, summed per pixel as a pass's radiance.
+template <int BV, bool BVH = (BV != 0)>
+__global__ __launch_bounds__(256) void k_shade_classic(SceneView sc, const PassState* st, EyeBuf eb,
+                                                       const uint32_t* order, float4 ambient) {
+  scene_to_lds<BV>(sc);
+  const uint32_t nq = st->n_qsort;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < nq; base += stride) {
+    const uint32_t i = base + threadIdx.x;
+    if (i >= nq) continue;  // after the loop's last wave-wide step only
+    const uint32_t slot = order[i];
+    const Query qq = eb.q[slot];
+    const C3 di = direct_light<BV>(sc, mk(qq.a), mk(qq.b)) + C3{ambient.x, ambient.y, ambient.z};
+    eb.res[slot] = make_float4(qq.w.x * di.r, qq.w.y * di.g, qq.w.z * di.b, 0.0f);
+  }
+}
+
+hipError_t launch_classic(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
+                          float4 ambient, int n_cu) {
+  hipLaunchKernelGGL(k_tile_order, dim3((eb.n_tiles + 255) / 256), dim3(256), 0, s, eb, st);
+  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u), eb.estack_lanes / 64u);
+  if (n_waves < 1) n_waves = 1;
+  const uint32_t blocks = (n_waves + 3) / 4;
+  const SceneView v = make_view(sc);
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_eye_classic<2>, dim3(blocks), dim3(256), bvh_dyn_lds(sc), s, v, st, eb, seed, n_waves);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_eye_classic<1>, dim3(blocks), dim3(256), 0, s, v, st, eb, seed, n_waves);
+      break;
+    default:
+      hipLaunchKernelGGL(k_eye_classic<0>, dim3(blocks), dim3(256), 0, s, v, st, eb, seed, n_waves);
+  }
+  hipLaunchKernelGGL(k_tile_count, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb);
+  hipError_t e = launch_query_prep(s, sc, MapBuf{}, st, eb);
+  if (e != hipSuccess) return e;
+  const uint32_t sblocks = (uint32_t)n_cu * 8u;
+  switch (bvh_kind(sc)) {
+    case 2:
+      hipLaunchKernelGGL(k_shade_classic<2>, dim3(sblocks), dim3(256), bvh_dyn_lds(sc), s, v, (const PassState*)st,
+                         eb, eb.qvals[0], ambient);
+      break;
+    case 1:
+      hipLaunchKernelGGL(k_shade_classic<1>, dim3(sblocks), dim3(256), 0, s, v, (const PassState*)st, eb,
+                         eb.qvals[0], ambient);
+      break;
+    default:
+      hipLaunchKernelGGL(k_shade_classic<0>, dim3(sblocks), dim3(256), 0, s, v, (const PassState*)st, eb,
+                         eb.qvals[0], ambient);
+  }
+  hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
+  hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, (double*)nullptr,
+                     (double*)nullptr);
   return hipGetLastError();
 }
 

@@ -170,6 +170,11 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu);
+// One pass of the classic ray tracer (trace_ray_classic) into eb.pix:
+// k_eye_classic, the query sort, k_shade_classic (direct light + ambient),
+// the overflow queries and the per-pixel sum.
+hipError_t launch_classic(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
+                          float4 ambient, int n_cu);
 
 // Query order of the gather: per-tile offsets, Morton keys of the eye
 // queries, their sort (needs only the eye pass, not the photons).

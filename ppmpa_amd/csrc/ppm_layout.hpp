@@ -41,6 +41,7 @@ enum : uint32_t { LIGHT_POINT = 0, LIGHT_PARALLELOGRAM = 1, LIGHT_SUN = 2 };
 enum : int { FILTER_NONE = 0, FILTER_CONE = 1, FILTER_GAUSS = 2 };
 
 constexpr int MAX_DEPTH_LIMIT = 16;  // deepest MAX_TRACE the kernels accept
+constexpr int CLASSIC_DEPTH = 10;    // trace_ray_classic's own depth limit (tracer.rs:222)
 #ifndef PPM_LEAF
 #define PPM_LEAF 32
 #endif

@@ -315,6 +315,16 @@ def trace_eye(ctx, dscene, pmap, seed, pass_, r2, use_classic=True, max_depth=10
     return out
 
 
+def trace_classic(ctx, dscene, seed, pass_, ambient=(0.0, 0.0, 0.0)):
+    """rtc: trace_ray_classic for every pixel (rtc.rs:17-47, tracer.rs:221-259),
+    direct light plus ambient at every hit, no photon map; (H, W, 3)."""
+    cam = dscene.scene.camera
+    out = np.zeros((cam.yreso, cam.xreso, 3))
+    amb = np.ascontiguousarray(ambient, np.float64).reshape(3)
+    ctx.check(ctx.lib.ppm_trace_classic(ctx.h, dscene.h, int(seed), int(pass_), _dp(amb), _dp(out)))
+    return out
+
+
 def pass_cfg(seed, n_photon, radius0, alpha, max_depth, use_classic=True, pfilter=A.FILTER_NONE, first_pass=1,
              flags=0, pipelines=0):
     """ppm_pass_cfg; pipelines = passes in flight at once (0: automatic)."""

@@ -49,6 +49,7 @@ _PROTOS = {
     "orc_gather": (None, [vp, C.c_uint64, dp, dp, C.c_double, C.c_int, dp]),
     "orc_gather_f32set": (None, [vp, C.c_uint64, dp, dp, C.c_double, C.c_int, dp]),
     "orc_trace_eye": (None, [vp, vp, C.c_uint64, C.c_uint32, C.c_double, C.c_int, C.c_int, C.c_int, dp]),
+    "orc_trace_classic": (None, [vp, C.c_uint64, C.c_uint32, dp, dp]),
     "orc_radius_schedule": (None, [C.c_double, C.c_double, C.c_uint32, C.c_uint32, dp]),
     "orc_run_passes": (C.c_int64, [vp, C.POINTER(A.PassCfg), C.c_uint32, dp, dp, C.c_int, dp, dp]),
     "orc_max_threads": (C.c_int, []),
@@ -127,6 +128,14 @@ class OScene:
         cam = self.scene.camera
         out = np.zeros((cam.yreso, cam.xreso, 3))
         lib().orc_trace_eye(self.h, omap.h, seed, pass_, r2, int(uc), maxd, pf, _d(out))
+        return out
+
+    def trace_classic(self, seed, pass_, ambient=(0.0, 0.0, 0.0)):
+        """rtc: trace_ray_classic for every pixel (H, W, 3)."""
+        cam = self.scene.camera
+        out = np.zeros((cam.yreso, cam.xreso, 3))
+        amb = np.ascontiguousarray(ambient, np.float64)
+        lib().orc_trace_classic(self.h, seed, pass_, _d(amb), _d(out))
         return out
 
     def set_params(self, c1=None, q=None, light_scale=1.0):
