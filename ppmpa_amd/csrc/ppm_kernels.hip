@@ -1755,7 +1755,12 @@ __global__ __launch_bounds__(256) void k_tile_count(EyeBuf eb) {
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu) {
   hipLaunchKernelGGL(k_tile_order, dim3((eb.n_tiles + 255) / 256), dim3(256), 0, s, eb, st);
-  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u), eb.estack_lanes / 64u);
+  // 6 persistent waves per CU: the eye's 123 registers allow 4 waves per
+  // SIMD, and a smaller grid leaves room beside it for the photon branch
+  // and the other pipeline's kernels (16 per CU filled every slot: caustic
+  // 0.855 -> 0.816 ms per pass, mesh 95.6 -> 87.8 ms; 4 and 8: 0.815-0.834
+  // and 90.6-94.5 ms)
+  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU), eb.estack_lanes / 64u);
   if (n_waves < 1) n_waves = 1;
   const uint32_t blocks = (n_waves + 3) / 4;
   switch (bvh_kind(sc)) {
@@ -2332,7 +2337,7 @@ __global__ __launch_bounds__(256) void k_shade_classic(SceneView sc, const PassS
 hipError_t launch_classic(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                           float4 ambient, int n_cu) {
   hipLaunchKernelGGL(k_tile_order, dim3((eb.n_tiles + 255) / 256), dim3(256), 0, s, eb, st);
-  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * 16u), eb.estack_lanes / 64u);
+  uint32_t n_waves = std::min<uint32_t>(std::min<uint32_t>(eb.n_tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU), eb.estack_lanes / 64u);
   if (n_waves < 1) n_waves = 1;
   const uint32_t blocks = (n_waves + 3) / 4;
   const SceneView v = make_view(sc);

@@ -113,8 +113,10 @@ struct EyeBuf {
   uint32_t estack_depth = 0;  // pool entries per lane of a wave
 };
 
-// Eye waves per CU at most (the launch configurations a batch may tune to).
+// Eye waves per CU at most (the launch configurations a batch may tune to),
+// and the persistent eye waves per CU of a session's pass.
 constexpr uint32_t EYE_WAVES_PER_CU_MAX = 32;
+constexpr uint32_t EYE_WAVES_PER_CU = 6;
 
 // Next pass of the state (the first one: first_pass); a state that runs
 // every stride-th pass advances by stride.
