@@ -443,7 +443,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(const uint32_t* count, uin
 
 uint32_t trace_waves(uint64_t n_photon, int n_cu) {
   uint64_t w = (n_photon + 255) / 256;
-  const uint64_t maxw = (uint64_t)n_cu * 16u;
+  const uint64_t maxw = (uint64_t)n_cu * TRACE_WAVES_PER_CU;
   if (w > maxw) w = maxw;
   if (w > 8192) w = 8192;
   if (w < 1) w = 1;
@@ -2270,7 +2270,7 @@ hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m,
 hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, PassState* st, const EyeBuf& eb,
                          int pfilter, int use_classic, int n_cu, double* sum, double* sumsq, hipEvent_t* ev,
                          bool prepared) {
-  const uint32_t blocks = (uint32_t)n_cu * 8u;
+  const uint32_t blocks = (uint32_t)n_cu * SHADE_BLOCKS_PER_CU;
   hipError_t e;
   if (!prepared && (e = launch_query_prep(s, sc, m, st, eb)) != hipSuccess) return e;
   uint32_t* const* qv = eb.qvals;
