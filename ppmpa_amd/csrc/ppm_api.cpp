@@ -160,6 +160,24 @@ void dfree(T*& p) {
   p = nullptr;
 }
 
+// One-sweep sort state for up to cap keys (the look-back words count keys + 1
+// in 31 bits).
+hipError_t onesweep_alloc(OneSweep& o, uint64_t cap, int n_cu) {
+  if (cap >= 0x7FFFFFFFull) return hipErrorInvalidValue;
+  o.cap = cap;
+  o.tiles = (uint32_t)std::max<uint64_t>(1, (cap + OS_TILE - 1) / OS_TILE);
+  o.blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)n_cu * OS_BLOCKS_PER_CU, o.tiles));
+  hipError_t e;
+  if ((e = dalloc(&o.ctl, OS_CTL_WORDS)) != hipSuccess) return e;
+  if ((e = hipMemset(o.ctl, 0, sizeof(uint32_t) * OS_CTL_WORDS)) != hipSuccess) return e;
+  return dalloc(&o.look, (uint64_t)OS_MAX_DIGITS * o.tiles * 256u);
+}
+
+void onesweep_free(OneSweep& o) {
+  dfree(o.ctl);
+  dfree(o.look);
+}
+
 float4 f4(const double v[3], float w = 0.0f) { return make_float4((float)v[0], (float)v[1], (float)v[2], w); }
 
 bool finite3(const double v[3]) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
@@ -218,6 +236,7 @@ hipError_t map_alloc(MapBuf& m, uint64_t cap, int n_cu, bool grid) {
   if (grid) {
     m.cells_max = (uint32_t)std::min<uint64_t>(GRID_MAX_BUCKETS, next_pow2(std::max<uint64_t>(cap, 1024)));
     if ((e = dalloc(&m.cells, m.cells_max)) != hipSuccess) return e;
+    if ((e = onesweep_alloc(m.os, cap, n_cu)) != hipSuccess) return e;
   } else {
     const uint64_t leaves = (cap + LEAF - 1) / LEAF;
     m.lpad_max = next_pow2(leaves > 0 ? leaves : 1);
@@ -244,6 +263,7 @@ void map_free(MapBuf& m) {
   }
   dfree(m.ghist);
   dfree(m.totals);
+  onesweep_free(m.os);
 }
 
 hipError_t photons_alloc(PhotonBuf& pb, uint64_t cap, uint32_t max_regions) {
@@ -336,6 +356,7 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   }
   if ((e = dalloc(&eb.qghist, 256ull * eb.q_sort_blocks)) != hipSuccess) return e;
   if ((e = dalloc(&eb.qtotals, 256)) != hipSuccess) return e;
+  if ((e = onesweep_alloc(eb.qos, inplace + eb.ovf_cap, n_cu)) != hipSuccess) return e;
   if ((e = dalloc(&eb.res, inplace + eb.ovf_cap)) != hipSuccess) return e;
   eb.estack_lanes = 64u * std::max<uint32_t>(1, std::min<uint32_t>(tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU_MAX));
   eb.estack_depth = (uint32_t)std::max(1, std::min(max_depth, MAX_DEPTH_LIMIT));
@@ -357,6 +378,7 @@ void eye_free(EyeBuf& eb) {
   }
   dfree(eb.qghist);
   dfree(eb.qtotals);
+  onesweep_free(eb.qos);
   dfree(eb.res);
   dfree(eb.estack);
 }
@@ -1163,6 +1185,7 @@ ppm_status ppm_map_build_grid(ppm_ctx* c, const ppm_scene* sc, const ppm_photons
     m->m.cells_max = (uint32_t)std::min<uint64_t>(GRID_MAX_BUCKETS, next_pow2(std::max<uint64_t>(m->m.cap, 1024)));
     HIP_TRY(c, dalloc(&m->m.cells, m->m.cells_max));
   }
+  if (!m->m.os.ctl) HIP_TRY(c, onesweep_alloc(m->m.os, m->m.cap, c->n_cu));
   if (!m->hook_mat) {
     DevMaterial air;
     std::memset(&air, 0, sizeof(air));

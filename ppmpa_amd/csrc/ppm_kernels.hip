@@ -760,6 +760,234 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* kin, 
   k_rs_scatter_body(kin, vin, kout, vout, np, ghist, totals, shift, Blk{blockIdx.x, gridDim.x});
 }
 
+// --- one-sweep LSD radix sort (the session's per-pass sorts) ----------------
+// Look-back word of (tile, digit): 0 = not yet published; else bit 31 set
+// when the count is the inclusive prefix over tiles 0..tile, clear when it
+// is the tile's own count, and bits 0-30 = that count + 1.
+constexpr uint32_t OS_INCL = 1u << 31;
+constexpr uint32_t OS_COUNT = OS_INCL - 1u;
+constexpr uint32_t OS_SPIN_LIMIT = 1u << 22;  // waits of a look-back step before it gives up
+constexpr uint32_t OS_ERR_FLAG = 16u;         // PassState::overflow bit of a sort that gave up
+constexpr uint32_t OS_LOOK_STEP = 16;         // look-back words read at once
+
+__device__ __forceinline__ uint32_t os_load(uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void os_store(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v, uint32_t lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o);
+    if (lane >= (uint32_t)o) v += u;
+  }
+  return v;
+}
+
+// The kernel that writes a sort's keys clears its ctl words first (block 0).
+__device__ __forceinline__ void os_clear_ctl(uint32_t* ctl, Blk blk) {
+  if (ctl && blk.b == 0)
+    for (uint32_t i = threadIdx.x; i < OS_CTL_WORDS; i += blockDim.x) ctl[i] = 0u;
+}
+
+// Counts of every digit of the first n = min(*np, cap) keys into ctl, and
+// the look-back words of those keys' tiles cleared.  Each block counts in
+// LDS, lane l into copy l % OS_HIST_COPIES (copies OS_HIST_STRIDE words
+// apart, so that a bin's copies sit in different banks): query keys arrive
+// in screen tiles whose lanes mostly share their high digits, and 64 lanes
+// adding to one word serialise (31 us against 12 us for the grid's hashed
+// keys with one copy).  Then one global atomic add per nonzero bin.
+constexpr uint32_t OS_HIST_COPIES = 16;
+constexpr uint32_t OS_HIST_STRIDE = OS_MAX_DIGITS * 256u + 1u;
+__global__ __launch_bounds__(1024) void k_os_hist(const uint32_t* keys, const uint32_t* np, uint32_t cap,
+                                                  uint32_t digits, uint32_t* ctl, uint32_t* look, uint32_t tiles) {
+  __shared__ uint32_t h[OS_HIST_COPIES * OS_HIST_STRIDE];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < OS_HIST_COPIES * OS_HIST_STRIDE; i += blockDim.x) h[i] = 0u;
+  __syncthreads();
+  const uint32_t n = min(*np, cap);
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t* const mine = h + (t % OS_HIST_COPIES) * OS_HIST_STRIDE;
+  for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) {
+    const uint32_t k = keys[i];
+    for (uint32_t p = 0; p < digits; ++p) atomicAdd(&mine[p * 256u + ((k >> (8u * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < digits * 256u; i += blockDim.x) {
+    uint32_t c = 0;
+    for (uint32_t r = 0; r < OS_HIST_COPIES; ++r) c += h[r * OS_HIST_STRIDE + i];
+    if (c != 0u) atomicAdd(&ctl[i], c);
+  }
+  const uint32_t words = (n + OS_TILE - 1) / OS_TILE * 256u;
+  for (uint32_t p = 0; p < digits; ++p) {
+    uint32_t* lk = look + (uint64_t)p * tiles * 256u;
+    for (uint32_t i = blockIdx.x * blockDim.x + t; i < words; i += stride) lk[i] = 0u;
+  }
+}
+
+// One digit pass: kin/vin -> kout/vout, stable.  Persistent blocks take
+// tiles by ticket, so every tile a block waits on belongs to a block that
+// is already running.  Threads t < 256 own digit t for the tile's counts
+// and look-back.  hist = this digit's 256 counts; look = this digit's
+// words.
+__global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                           uint32_t* vout, const uint32_t* np, uint32_t cap,
+                                                           const uint32_t* hist, uint32_t* ticket, uint32_t* look,
+                                                           uint32_t shift, uint32_t* err) {
+  constexpr uint32_t W = OS_THREADS / 64, KPT = OS_KEYS_PER_THREAD;
+  static_assert(OS_THREADS >= 256 && OS_THREADS % 256 == 0, "digit owners are threads 0..255");
+  __shared__ uint32_t sk[OS_TILE];
+  __shared__ uint32_t sv[OS_TILE];
+  __shared__ uint32_t wc[W][256];  // per wave: keys of each digit so far, then the wave's offset in the tile
+  __shared__ uint32_t dbase[256];  // first output position of each digit
+  __shared__ uint32_t loc[256];    // first tile position of each digit
+  __shared__ uint32_t gout[256];   // output position of tile position 0 of each digit's run
+  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t s_tile;
+  const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+  const bool owner = t < 256u;
+  const uint32_t n = min(*np, cap);
+  const uint32_t n_tiles = (n + OS_TILE - 1) / OS_TILE;
+  // exclusive scan of 256 counts held by the owners (all threads call it)
+  auto owner_scan = [&](uint32_t c) -> uint32_t {
+    const uint32_t incl = wave_incl_sum(c, lane);
+    if (owner && lane == 63u) wsum[w] = incl;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (uint32_t k = 0; k < w && k < 4u; ++k) pre += wsum[k];
+    return pre + incl - c;
+  };
+  {
+    const uint32_t e = owner_scan(owner ? hist[t] : 0u);
+    if (owner) dbase[t] = e;
+  }
+  for (;;) {
+    __syncthreads();  // the last tile's LDS is read; wsum is free
+    if (t == 0) s_tile = atomicAdd(ticket, 1u);
+    for (uint32_t i = t; i < W * 256u; i += OS_THREADS) (&wc[0][0])[i] = 0u;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= n_tiles) return;
+    const uint32_t base = tile * OS_TILE;
+    const uint32_t tn = min(OS_TILE, n - base);
+    // wave w ranks tile positions wb + 64 k + lane, k = 0..KPT-1, in order
+    const uint32_t wb = w * 64u * KPT;
+    uint32_t key[KPT], val[KPT], rk[KPT];
+#pragma unroll
+    for (uint32_t k = 0; k < KPT; ++k) {
+      const uint32_t i = wb + k * 64u + lane;
+      key[k] = i < tn ? kin[base + i] : 0u;
+      val[k] = i < tn ? vin[base + i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < KPT; ++k) {
+      const bool valid = wb + k * 64u + lane < tn;
+      const uint32_t d = (key[k] >> shift) & 255u;
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        const uint64_t bb = __ballot(valid && ((d >> bit) & 1u));
+        peers &= ((d >> bit) & 1u) ? bb : ~bb;
+      }
+      const uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+      const uint32_t run = wc[w][d];
+      rk[k] = run + before;
+      if (valid && before == 0u) wc[w][d] = run + (uint32_t)__popcll(peers);
+    }
+    __syncthreads();
+    // digit owners: the tile's count, each wave's offset in it
+    uint32_t c = 0;
+    uint32_t* const lw = look + (uint64_t)tile * 256u + (owner ? t : 0u);
+    if (owner) {
+      for (uint32_t k = 0; k < W; ++k) {
+        const uint32_t v = wc[k][t];
+        wc[k][t] = c;
+        c += v;
+      }
+      os_store(lw, (tile == 0u ? OS_INCL : 0u) | (c + 1u));
+    }
+    const uint32_t lc = owner_scan(c);
+    if (owner) loc[t] = lc;
+    __syncthreads();
+    // the tile in digit order, in LDS
+#pragma unroll
+    for (uint32_t k = 0; k < KPT; ++k) {
+      if (wb + k * 64u + lane < tn) {
+        const uint32_t d = (key[k] >> shift) & 255u;
+        const uint32_t lp = loc[d] + wc[w][d] + rk[k];
+        sk[lp] = key[k];
+        sv[lp] = val[k];
+      }
+    }
+    // keys of digit t in tiles 0..tile-1: walk back until an inclusive
+    // count, OS_LOOK_STEP tiles per step (the loads cross XCDs, each costing
+    // about a microsecond)
+    if (owner) {
+      uint32_t ex = 0;
+      if (tile > 0u) {
+        uint32_t j = tile, spins = 0;  // tiles [0, j) are still to be counted
+        for (;;) {
+          uint32_t v[OS_LOOK_STEP];
+#pragma unroll
+          for (uint32_t q = 0; q < OS_LOOK_STEP; ++q)
+            v[q] = q < j ? os_load(look + (uint64_t)(j - 1u - q) * 256u + t) : 0u;
+          uint32_t used = 0;
+          bool found = false;
+#pragma unroll
+          for (uint32_t q = 0; q < OS_LOOK_STEP; ++q) {
+            if (found || used < q || q >= j || v[q] == 0u) continue;  // stop at the first word not yet published
+            ex += (v[q] & OS_COUNT) - 1u;
+            used = q + 1u;
+            found = (v[q] & OS_INCL) != 0u;
+          }
+          if (found) break;
+          j -= used;
+          if (used == 0u) {
+            if (++spins > OS_SPIN_LIMIT) {
+              atomicOr(err, OS_ERR_FLAG);
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        os_store(lw, OS_INCL | (ex + c + 1u));
+      }
+      gout[t] = dbase[t] + ex - loc[t];
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < tn; i += OS_THREADS) {
+      const uint32_t k = sk[i];
+      const uint32_t pos = gout[(k >> shift) & 255u] + i;
+      if (pos < n) {  // always, unless a look-back gave up
+        kout[pos] = k;
+        vout[pos] = sv[i];
+      }
+    }
+  }
+}
+
+// The sort of n = *np <= o.cap keys (written by a kernel that cleared
+// o.ctl): digits 8-bit digit passes, the result in keys[*cur] / vals[*cur].
+static hipError_t onesweep_sort(hipStream_t s, const OneSweep& o, uint32_t* keys[2], uint32_t* vals[2],
+                                const uint32_t* np, uint32_t key_bits, uint32_t* err, int* cur) {
+  const uint32_t digits = (key_bits + 7u) / 8u;
+  if (!o.ctl || !o.look || digits > OS_MAX_DIGITS) return hipErrorInvalidValue;
+  const uint32_t cap = (uint32_t)o.cap;
+  const uint32_t hblocks = std::max<uint32_t>(1u, std::min<uint32_t>(OS_HIST_BLOCKS, (cap + 4095u) / 4096u));
+  hipLaunchKernelGGL(k_os_hist, dim3(hblocks), dim3(1024), 0, s, keys[0], np, cap, digits, o.ctl, o.look, o.tiles);
+  const uint32_t blocks = o.blocks;
+  *cur = 0;
+  for (uint32_t p = 0; p < digits; ++p) {
+    hipLaunchKernelGGL(k_os_scatter, dim3(blocks), dim3(OS_THREADS), 0, s, keys[*cur], vals[*cur], keys[*cur ^ 1],
+                       vals[*cur ^ 1], np, cap, o.ctl + 256u * p, o.ctl + OS_TICKETS + 32u * p,
+                       o.look + (uint64_t)p * o.tiles * 256u, 8u * p, err);
+    *cur ^= 1;
+  }
+  return hipGetLastError();
+}
+
 // Leaves + tree.  The tree is a complete binary tree in heap order over
 // lpad = 2^depth leaves of 64 consecutive photons (empty leaves have empty
 // boxes).  k_build_leaves: one block per 64 leaves; each wave gathers 16
@@ -1054,7 +1282,9 @@ __global__ void k_grid_setup(PassState* st, float4 bmin, float4 bmax, uint32_t c
 
 __device__ __forceinline__ void k_grid_keys_body(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
                                                  uint32_t region_cap, const PassState* st, uint32_t* keys,
-                                                 uint32_t* vals, float4 bmin, uint32_t* nonfinite, Blk blk) {
+                                                 uint32_t* vals, float4 bmin, uint32_t* nonfinite, uint32_t* os_ctl,
+                                                 Blk blk) {
+  os_clear_ctl(os_ctl, blk);
   const uint32_t n = st->n_stored;
   const GridView g = grid_view(st, bmin);
   for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
@@ -1069,8 +1299,9 @@ __device__ __forceinline__ void k_grid_keys_body(const PhotonRec* ph, const uint
 
 __global__ __launch_bounds__(256) void k_grid_keys(const PhotonRec* ph, const uint32_t* offset, uint32_t n_regions,
                                                    uint32_t region_cap, const PassState* st, uint32_t* keys,
-                                                   uint32_t* vals, float4 bmin, uint32_t* nonfinite) {
-  k_grid_keys_body(ph, offset, n_regions, region_cap, st, keys, vals, bmin, nonfinite, Blk{blockIdx.x, gridDim.x});
+                                                   uint32_t* vals, float4 bmin, uint32_t* nonfinite, uint32_t* os_ctl) {
+  k_grid_keys_body(ph, offset, n_regions, region_cap, st, keys, vals, bmin, nonfinite, os_ctl,
+                   Blk{blockIdx.x, gridDim.x});
 }
 
 // Every bucket of the pass empty (a kernel, not a memset, so that a captured
@@ -1135,11 +1366,11 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
   if (kblocks > 8192) kblocks = 8192;
   if (kblocks == 0) kblocks = 1;
   hipLaunchKernelGGL(k_grid_keys, dim3(kblocks), dim3(256), 0, s, in.rec, in.offset, in.n_regions, in.region_cap,
-                     (const PassState*)st, m.keys[0], m.vals[0], bmin, &st->nonfinite);
+                     (const PassState*)st, m.keys[0], m.vals[0], bmin, &st->nonfinite, m.os.ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   int cur = 0;
-  e = radix_sort(s, m.keys, m.vals, &st->n_stored, m.ghist, m.totals, m.sort_blocks, GRID_KEY_BITS, &cur);
+  e = onesweep_sort(s, m.os, m.keys, m.vals, &st->n_stored, GRID_KEY_BITS, &st->overflow, &cur);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_grid_clear, dim3(grid_clear_blocks(m.cells_max)), dim3(256), 0, s, m.cells,
                      (const PassState*)st);
@@ -1538,8 +1769,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
     acc3[3u * lane] = 0.0f;
     acc3[3u * lane + 1u] = 0.0f;
     acc3[3u * lane + 2u] = 0.0f;
-    eb.q[t].a.w = __uint_as_float(NO_PIXEL);
-    bool alive = false;
+    bool alive = false, prim = false;
     uint32_t own = lane;  // the pixel (position in the tile) of the lane's node
     EyeNode cur;
     if (inside) {
@@ -1698,6 +1928,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
         }
       }
       if (want_q && slot != 0xFFFFFFFFu) eb.q[slot] = qq;
+      prim = prim || (want_q && depth == 0u);  // a root is its own lane's first node
       // pending refraction children onto the pool, in lane order
       const uint64_t pm = __ballot(push);
       if (pm) {
@@ -1722,7 +1953,7 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
     // the tile is done: each lane writes its pixel's emitted radiance and
     // in-place query count
     if (inside) eb.pix[my_pixel] = make_float4(acc3[3u * lane], acc3[3u * lane + 1u], acc3[3u * lane + 2u], 0.0f);
-    eb.sec_count[t] = inside ? nsec[lane] : 0u;
+    eb.sec_count[t] = inside ? nsec[lane] | (prim ? SEC_PRIMARY : 0u) : 0u;
   }
 }
 
@@ -1742,7 +1973,10 @@ __device__ __forceinline__ void k_tile_count_body(EyeBuf eb, Blk blk) {
   const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   const uint32_t tile = t >> 6;
   uint32_t c = 0;
-  if (t < eb.q_primary) c = (__float_as_uint(eb.q[t].a.w) != NO_PIXEL ? 1u : 0u) + eb.sec_count[t];
+  if (t < eb.q_primary) {
+    const uint32_t sc = eb.sec_count[t];
+    c = (sc >> 31) + (sc & ~SEC_PRIMARY);
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((threadIdx.x & 63u) == 0 && tile < eb.n_tiles) eb.tile_count[tile] = c;
@@ -1867,7 +2101,8 @@ __device__ __forceinline__ GatherAcc gather_query(const Node* nodes, const Photo
 // the position.
 __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
                                                     float fq, uint32_t lshift, uint32_t* keys, uint32_t* vals,
-                                                    Blk blk) {
+                                                    uint32_t* os_ctl, Blk blk) {
+  os_clear_ctl(os_ctl, blk);
   const uint32_t t = blk.b * blockDim.x + threadIdx.x;
   const uint32_t g = t >> 6, lane = lane_id();
   const uint64_t ovf_base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
@@ -1882,8 +2117,9 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     }
     return;
   }
-  const bool primary = __float_as_uint(eb.q[t].a.w) != NO_PIXEL;
-  const uint32_t nsec = eb.sec_count[t];
+  const uint32_t sc = eb.sec_count[t];
+  const bool primary = (sc & SEC_PRIMARY) != 0u;
+  const uint32_t nsec = sc & ~SEC_PRIMARY;
   const uint32_t c = (primary ? 1u : 0u) + nsec;
   uint32_t incl = c;
 #pragma unroll
@@ -1892,24 +2128,28 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     if (lane >= (uint32_t)o) incl += v;
   }
   uint32_t pos = eb.tile_off[g] + incl - c;
-  if (primary) {
-    const float4 a = eb.q[t].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
-    vals[pos] = t;
-    ++pos;
-  }
-  for (uint32_t k = 0; k < nsec; ++k) {
-    const uint32_t slot = eb.q_primary * (1u + k) + t;
-    const float4 a = eb.q[slot].a;
-    keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
-    vals[pos] = slot;
-    ++pos;
+  // slot q_primary * k + t for k = 0 (the primary, when there is one) ..
+  // nsec, four positions loaded at once (a lane behind glass has a dozen)
+  for (uint32_t k0 = primary ? 0u : 1u; k0 <= nsec; k0 += 4u) {
+    float4 a[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (k0 + j <= nsec) a[j] = eb.q[eb.q_primary * (k0 + j) + t].a;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+      if (k0 + j <= nsec) {
+        keys[pos] = morton_key(a[j].x, a[j].y, a[j].z, bmin, bscale, fq, lshift);
+        vals[pos] = eb.q_primary * (k0 + j) + t;
+        ++pos;
+      }
+    }
   }
 }
 
 __global__ __launch_bounds__(256) void k_query_keys(EyeBuf eb, const PassState* st, float4 bmin, float4 bscale,
-                                                    float fq, uint32_t lshift, uint32_t* keys, uint32_t* vals) {
-  k_query_keys_body(eb, st, bmin, bscale, fq, lshift, keys, vals, Blk{blockIdx.x, gridDim.x});
+                                                    float fq, uint32_t lshift, uint32_t* keys, uint32_t* vals,
+                                                    uint32_t* os_ctl) {
+  k_query_keys_body(eb, st, bmin, bscale, fq, lshift, keys, vals, os_ctl, Blk{blockIdx.x, gridDim.x});
 }
 
 // Shading of the queries in Morton order, one query per lane, 64 spatially
@@ -2215,18 +2455,23 @@ __device__ __forceinline__ void k_accumulate_body(EyeBuf eb, DevCamera cam, doub
   if (x >= cam.xreso || y >= cam.yreso) return;
   const uint32_t p = (uint32_t)y * (uint32_t)cam.xreso + (uint32_t)x;
   float4 c = eb.pix[p];
-  if (__float_as_uint(eb.q[t].a.w) != NO_PIXEL) {
-    const float4 r = eb.res[t];
-    c.x += r.x;
-    c.y += r.y;
-    c.z += r.z;
-  }
-  const uint32_t nsec = eb.sec_count[t];
-  for (uint32_t k = 0; k < nsec; ++k) {
-    const float4 r = eb.res[eb.q_primary * (1u + k) + t];
-    c.x += r.x;
-    c.y += r.y;
-    c.z += r.z;
+  const uint32_t sc = eb.sec_count[t];
+  const uint32_t nsec = sc & ~SEC_PRIMARY;
+  // results of slots q_primary * k + t, k = 0 (the primary, when there is
+  // one) .. nsec, added in that order, four loaded at once
+  for (uint32_t k0 = (sc & SEC_PRIMARY) != 0u ? 0u : 1u; k0 <= nsec; k0 += 4u) {
+    float4 r[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j)
+      if (k0 + j <= nsec) r[j] = eb.res[eb.q_primary * (k0 + j) + t];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+      if (k0 + j <= nsec) {
+        c.x += r[j].x;
+        c.y += r[j].y;
+        c.z += r[j].z;
+      }
+    }
   }
   if (sum) {
     const double r = c.x, gg = c.y, b = c.z;
@@ -2256,13 +2501,14 @@ hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m,
   const uint32_t kbits = fine ? QUERY_FINE_KEY_BITS : QUERY_KEY_BITS;
   const int first = (int)((kbits / 8u) & 1u);
   hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, st, sc.bmin,
-                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, lshift, eb.qkeys[first], eb.qvals[first]);
+                     morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, lshift, eb.qkeys[first], eb.qvals[first],
+                     eb.qos.ctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   uint32_t* qk[2] = {eb.qkeys[first], eb.qkeys[first ^ 1]};
   uint32_t* qv[2] = {eb.qvals[first], eb.qvals[first ^ 1]};
   int cur = 0;
-  e = radix_sort(s, qk, qv, &st->n_qsort, eb.qghist, eb.qtotals, eb.q_sort_blocks, kbits, &cur);
+  e = onesweep_sort(s, eb.qos, qk, qv, &st->n_qsort, kbits, &st->overflow, &cur);
   if (e == hipSuccess && (cur ^ first) != 0) e = hipErrorInvalidValue;  // the sorted order is read from qvals[0]
   return e;
 }
@@ -2653,7 +2899,7 @@ __global__ __launch_bounds__(256) void kb_grid_keys(const BatchItem* items, cons
                                                     uint32_t n_all) {
   BATCH_PROLOGUE(BK_MORTON)
   k_grid_keys_body(it.pb.rec, it.pb.offset, it.pb.n_regions, it.pb.region_cap, it.st, it.m.keys[0], it.m.vals[0],
-                   it.sc.bmin, &it.st->nonfinite, blk);
+                   it.sc.bmin, &it.st->nonfinite, nullptr, blk);
 }
 
 __global__ __launch_bounds__(256) void kb_grid_clear(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -2737,7 +2983,7 @@ __global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, con
   // batches sort on the coarse keys whatever their rooms' sizes
   k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, (float)QUERY_MORTON_MAX, QUERY_LEVEL_SHIFT,
                     QUERY_KEYS_FIRST != 0u ? it.eb.qkeys[1] : it.eb.qkeys[0],
-                    QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], blk);
+                    QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], nullptr, blk);
 }
 
 template <int BV, bool BVH = (BV != 0)>

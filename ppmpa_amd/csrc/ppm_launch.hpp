@@ -54,6 +54,31 @@ struct PhotonBuf {
   uint64_t cap = 0;          // records allocated
 };
 
+// One-sweep LSD radix sort of a pass's keys (grid buckets, query Morton
+// codes), 8-bit digits.  One histogram kernel counts every digit of every
+// key; then one kernel per digit, whose blocks take tiles of OS_TILE keys in
+// ticket order, rank each tile stably in LDS, find the earlier tiles' keys
+// of each digit by decoupled look-back over their published counts, and
+// write the tile out one digit run at a time.  ctl: the digit counts
+// [digit][256], then one ticket per digit pass on its own 128-byte line
+// (cleared by the kernel that writes the keys); look: [digit][tile][256]
+// look-back words (cleared by the histogram kernel).
+constexpr uint32_t OS_THREADS = 256;
+constexpr uint32_t OS_KEYS_PER_THREAD = 16;
+constexpr uint32_t OS_TILE = OS_THREADS * OS_KEYS_PER_THREAD;
+constexpr uint32_t OS_MAX_DIGITS = 4;
+constexpr uint32_t OS_TICKETS = OS_MAX_DIGITS * 256;  // ticket of digit pass p at OS_TICKETS + 32 p
+constexpr uint32_t OS_CTL_WORDS = OS_TICKETS + 32 * OS_MAX_DIGITS;
+constexpr uint32_t OS_BLOCKS_PER_CU = 2;  // persistent scatter blocks
+constexpr uint32_t OS_HIST_BLOCKS = 64;   // of 1024 threads
+struct OneSweep {
+  uint32_t* ctl = nullptr;
+  uint32_t* look = nullptr;
+  uint32_t tiles = 0;   // tiles of the capacity
+  uint32_t blocks = 0;  // persistent scatter blocks
+  uint64_t cap = 0;     // keys at most (< 2^31 - 1)
+};
+
 // A photon map in one of two forms.  Tree (ppm_map_build: any radius per
 // call): photons in leaf order under a heap-ordered box tree.  Grid (the
 // passes of a session or batch, whose queries share the pass radius): photons
@@ -72,6 +97,7 @@ struct MapBuf {
   uint32_t* ghist = nullptr;    // 256 * sort_blocks
   uint32_t* totals = nullptr;   // 256
   uint32_t sort_blocks = 0;
+  OneSweep os;                  // grid: the per-pass bucket sort
   int morton_bits = 10;         // per axis
 };
 
@@ -102,6 +128,7 @@ struct EyeBuf {
   uint32_t* qghist = nullptr;
   uint32_t* qtotals = nullptr;
   uint32_t q_sort_blocks = 0;
+  OneSweep qos;         // the query sort of sessions
   uint32_t q_fine = 0;  // 1: fine query sort keys (QUERY_FINE_*)
   float4* res = nullptr;
   // pending refraction nodes of the eye trees: wave w of the eye kernel

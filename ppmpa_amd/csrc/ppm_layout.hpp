@@ -161,6 +161,9 @@ struct alignas(16) Query {
   float4 w;
 };
 constexpr uint32_t NO_PIXEL = 0xFFFFFFFFu;
+// EyeBuf::sec_count[t]: the pixel's deeper in-place queries, with this bit
+// set when the pixel has its primary query in slot t.
+constexpr uint32_t SEC_PRIMARY = 1u << 31;
 
 // --- per-pass device state -----------------------------------------------------
 struct alignas(128) PassState {
