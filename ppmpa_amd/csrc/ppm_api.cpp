@@ -344,6 +344,7 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   eb.q_sort_blocks = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, (uint64_t)n_cu * 4));
   hipError_t e;
   if ((e = dalloc(&eb.q, inplace + eb.ovf_cap)) != hipSuccess) return e;
+  if ((e = dalloc(&eb.qpos, inplace + eb.ovf_cap)) != hipSuccess) return e;
   if ((e = dalloc(&eb.sec_count, eb.q_primary)) != hipSuccess) return e;
   if ((e = dalloc(&eb.pix, eb.n_pix)) != hipSuccess) return e;
   if ((e = dalloc(&eb.tile_count, tiles)) != hipSuccess) return e;
@@ -367,6 +368,7 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
 
 void eye_free(EyeBuf& eb) {
   dfree(eb.q);
+  dfree(eb.qpos);
   dfree(eb.sec_count);
   dfree(eb.pix);
   dfree(eb.tile_count);

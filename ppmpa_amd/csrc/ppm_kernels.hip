@@ -827,15 +827,27 @@ __global__ __launch_bounds__(1024) void k_os_hist(const uint32_t* keys, const ui
   }
 }
 
-// One digit pass: kin/vin -> kout/vout, stable.  Persistent blocks take
-// tiles by ticket, so every tile a block waits on belongs to a block that
-// is already running.  Threads t < 256 own digit t for the tile's counts
-// and look-back.  hist = this digit's 256 counts; look = this digit's
-// words.
+// Photon records moved with the last digit pass of the grid's sort: out[i]
+// = grid_rec(in[v_i]) for the sorted values v_i (indices into in), double
+// positions with them when in64 is given, and the GRID_READ_SLACK records
+// after the last one zeroed.
+struct OsRecords {
+  const PhotonRec* in = nullptr;
+  const double* in64 = nullptr;
+  PhotonRec* out = nullptr;
+  double* out64 = nullptr;
+};
+
+// One digit pass: kin/vin -> kout/vout, stable (REC: the records too).
+// Persistent blocks take tiles by ticket, so every tile a block waits on
+// belongs to a block that is already running.  Threads t < 256 own digit t
+// for the tile's counts and look-back.  hist = this digit's 256 counts;
+// look = this digit's words.
+template <bool REC>
 __global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
                                                            uint32_t* vout, const uint32_t* np, uint32_t cap,
                                                            const uint32_t* hist, uint32_t* ticket, uint32_t* look,
-                                                           uint32_t shift, uint32_t* err) {
+                                                           uint32_t shift, uint32_t* err, OsRecords rec) {
   constexpr uint32_t W = OS_THREADS / 64, KPT = OS_KEYS_PER_THREAD;
   static_assert(OS_THREADS >= 256 && OS_THREADS % 256 == 0, "digit owners are threads 0..255");
   __shared__ uint32_t sk[OS_TILE];
@@ -850,6 +862,8 @@ __global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, 
   const bool owner = t < 256u;
   const uint32_t n = min(*np, cap);
   const uint32_t n_tiles = (n + OS_TILE - 1) / OS_TILE;
+  if (REC && blockIdx.x == 0 && t < GRID_READ_SLACK)
+    rec.out[n + t] = PhotonRec{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
   // exclusive scan of 256 counts held by the owners (all threads call it)
   auto owner_scan = [&](uint32_t c) -> uint32_t {
     const uint32_t incl = wave_incl_sum(c, lane);
@@ -961,17 +975,29 @@ __global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, 
       const uint32_t k = sk[i];
       const uint32_t pos = gout[(k >> shift) & 255u] + i;
       if (pos < n) {  // always, unless a look-back gave up
+        const uint32_t v = sv[i];
         kout[pos] = k;
-        vout[pos] = sv[i];
+        vout[pos] = v;
+        if constexpr (REC) {
+          rec.out[pos] = grid_rec(rec.in[v]);
+          if (rec.in64) {
+            rec.out64[3 * (uint64_t)pos + 0] = rec.in64[3 * (uint64_t)v + 0];
+            rec.out64[3 * (uint64_t)pos + 1] = rec.in64[3 * (uint64_t)v + 1];
+            rec.out64[3 * (uint64_t)pos + 2] = rec.in64[3 * (uint64_t)v + 2];
+          }
+        }
       }
     }
   }
 }
 
 // The sort of n = *np <= o.cap keys (written by a kernel that cleared
-// o.ctl): digits 8-bit digit passes, the result in keys[*cur] / vals[*cur].
+// o.ctl): digits 8-bit digit passes, the result in keys[*cur] / vals[*cur]
+// (with rec, the last pass writes the values to last_vals instead and moves
+// the records).
 static hipError_t onesweep_sort(hipStream_t s, const OneSweep& o, uint32_t* keys[2], uint32_t* vals[2],
-                                const uint32_t* np, uint32_t key_bits, uint32_t* err, int* cur) {
+                                const uint32_t* np, uint32_t key_bits, uint32_t* err, int* cur,
+                                const OsRecords* rec = nullptr, uint32_t* last_vals = nullptr) {
   const uint32_t digits = (key_bits + 7u) / 8u;
   if (!o.ctl || !o.look || digits > OS_MAX_DIGITS) return hipErrorInvalidValue;
   const uint32_t cap = (uint32_t)o.cap;
@@ -979,10 +1005,17 @@ static hipError_t onesweep_sort(hipStream_t s, const OneSweep& o, uint32_t* keys
   hipLaunchKernelGGL(k_os_hist, dim3(hblocks), dim3(1024), 0, s, keys[0], np, cap, digits, o.ctl, o.look, o.tiles);
   const uint32_t blocks = o.blocks;
   *cur = 0;
+  if (rec && !last_vals) return hipErrorInvalidValue;
   for (uint32_t p = 0; p < digits; ++p) {
-    hipLaunchKernelGGL(k_os_scatter, dim3(blocks), dim3(OS_THREADS), 0, s, keys[*cur], vals[*cur], keys[*cur ^ 1],
-                       vals[*cur ^ 1], np, cap, o.ctl + 256u * p, o.ctl + OS_TICKETS + 32u * p,
-                       o.look + (uint64_t)p * o.tiles * 256u, 8u * p, err);
+    const bool last = p + 1u == digits;
+    if (rec && last)
+      hipLaunchKernelGGL(k_os_scatter<true>, dim3(blocks), dim3(OS_THREADS), 0, s, keys[*cur], vals[*cur],
+                         keys[*cur ^ 1], last_vals, np, cap, o.ctl + 256u * p, o.ctl + OS_TICKETS + 32u * p,
+                         o.look + (uint64_t)p * o.tiles * 256u, 8u * p, err, *rec);
+    else
+      hipLaunchKernelGGL(k_os_scatter<false>, dim3(blocks), dim3(OS_THREADS), 0, s, keys[*cur], vals[*cur],
+                         keys[*cur ^ 1], vals[*cur ^ 1], np, cap, o.ctl + 256u * p, o.ctl + OS_TICKETS + 32u * p,
+                         o.look + (uint64_t)p * o.tiles * 256u, 8u * p, err, OsRecords{});
     *cur ^= 1;
   }
   return hipGetLastError();
@@ -1332,7 +1365,8 @@ __global__ __launch_bounds__(256) void k_grid_cells(const uint32_t* keys, const 
 
 // Photon records (and double positions, if any) in bucket order, in the
 // grid's gather form (GridRec), and the GRID_READ_SLACK records after the
-// last one zeroed.
+// last one zeroed (the batch's grid build; a session's moves them with the
+// last digit pass of its sort).
 __device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const double* in64, const uint32_t* order,
                                                     PhotonRec* out, double* out64, uint32_t* src_idx,
                                                     const PassState* st, Blk blk) {
@@ -1351,11 +1385,6 @@ __device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const d
   }
 }
 
-__global__ __launch_bounds__(256) void k_grid_permute(const PhotonRec* in, const double* in64, const uint32_t* order,
-                                                      PhotonRec* out, double* out64, uint32_t* src_idx,
-                                                      const PassState* st) {
-  k_grid_permute_body(in, in64, order, out, out64, src_idx, st, Blk{blockIdx.x, gridDim.x});
-}
 
 hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin,
                              float4 bmax) {
@@ -1370,14 +1399,15 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   int cur = 0;
-  e = onesweep_sort(s, m.os, m.keys, m.vals, &st->n_stored, GRID_KEY_BITS, &st->overflow, &cur);
+  // the last digit pass also moves the records into bucket order (the
+  // source index of each sorted photon goes straight to src_idx)
+  const OsRecords rec{in.rec, in.rec64, m.sorted, m.sorted64};
+  e = onesweep_sort(s, m.os, m.keys, m.vals, &st->n_stored, GRID_KEY_BITS, &st->overflow, &cur, &rec, m.src_idx);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k_grid_clear, dim3(grid_clear_blocks(m.cells_max)), dim3(256), 0, s, m.cells,
                      (const PassState*)st);
   const uint32_t pblocks = grid_photon_blocks(m.cap);
   hipLaunchKernelGGL(k_grid_cells, dim3(pblocks), dim3(256), 0, s, m.keys[cur], (const PassState*)st, m.cells);
-  hipLaunchKernelGGL(k_grid_permute, dim3(pblocks), dim3(256), 0, s, in.rec, in.rec64, m.vals[cur], m.sorted,
-                     m.sorted64, m.src_idx, (const PassState*)st);
   return hipGetLastError();
 }
 
@@ -1927,7 +1957,10 @@ __device__ __forceinline__ void k_eye_body(SceneView sc, PassState* st, EyeBuf e
           if (rank == 0u) nsec[own] = min(base + (uint32_t)__popcll(pq), eb.sec_max);
         }
       }
-      if (want_q && slot != 0xFFFFFFFFu) eb.q[slot] = qq;
+      if (want_q && slot != 0xFFFFFFFFu) {
+        eb.q[slot] = qq;
+        eb.qpos[slot] = qq.a;
+      }
       prim = prim || (want_q && depth == 0u);  // a root is its own lane's first node
       // pending refraction children onto the pool, in lane order
       const uint64_t pm = __ballot(push);
@@ -2111,7 +2144,7 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     const uint32_t i = t - eb.n_tiles * 64u;
     if (i < n_ovf) {
       const uint32_t pos = st->n_qsort - n_ovf + i;
-      const float4 a = eb.q[ovf_base + i].a;
+      const float4 a = eb.qpos[ovf_base + i];
       keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
       vals[pos] = (uint32_t)(ovf_base + i);
     }
@@ -2134,7 +2167,7 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
     float4 a[4];
 #pragma unroll
     for (uint32_t j = 0; j < 4u; ++j)
-      if (k0 + j <= nsec) a[j] = eb.q[eb.q_primary * (k0 + j) + t].a;
+      if (k0 + j <= nsec) a[j] = eb.qpos[eb.q_primary * (k0 + j) + t];
 #pragma unroll
     for (uint32_t j = 0; j < 4u; ++j) {
       if (k0 + j <= nsec) {

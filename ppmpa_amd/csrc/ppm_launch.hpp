@@ -112,6 +112,7 @@ struct MapBuf {
 // query's weighted radiance.
 struct EyeBuf {
   Query* q = nullptr;
+  float4* qpos = nullptr;  // q[slot].a again, densely: the sort keys' only input
   uint32_t q_primary = 0;
   uint32_t sec_max = 0;
   uint32_t* sec_count = nullptr;
