@@ -1693,7 +1693,7 @@ static hipError_t enqueue_pass(ppm_session* s, uint32_t j, hipStream_t st, hipEv
   if ((e = launch_trace_photons(st, s->sbuf, p.st, p.pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
     return e;
   if (ev && (e = hipEventRecord(ev[1], st)) != hipSuccess) return e;
-  if ((e = launch_grid_build(st, p.map, p.st, p.pb, bmin, bmax)) != hipSuccess) return e;
+  if ((e = launch_grid_build(st, p.map, p.st, p.pb, bmin, bmax, 2ull * s->sbuf.n_emit_total)) != hipSuccess) return e;
   if (ev && (e = hipEventRecord(ev[2], st)) != hipSuccess) return e;
   if (ev) {
     if ((e = launch_eye(st, s->sbuf, p.st, p.eb, cfg.seed, cfg.use_classic, cfg.max_depth, n_cu)) != hipSuccess)
@@ -1946,7 +1946,7 @@ ppm_status ppm_session_backward(ppm_session* s, const double* grad_image, double
       if ((e = hipEventRecord(p0.evj, p0.s2)) != hipSuccess) break;
       if ((e = launch_trace_photons(q, s->sbuf, st, p0.pb, cfg.seed, cfg.use_classic, cfg.max_depth)) != hipSuccess)
         break;
-      if ((e = launch_grid_build(q, p0.map, st, p0.pb, bmin, bmax)) != hipSuccess) break;
+      if ((e = launch_grid_build(q, p0.map, st, p0.pb, bmin, bmax, 2ull * s->sbuf.n_emit_total)) != hipSuccess) break;
       if ((e = hipStreamWaitEvent(q, p0.evj, 0)) != hipSuccess) break;
       if ((e = launch_gather(q, s->sbuf, p0.map, st, p0.eb, cfg.pfilter, cfg.use_classic, c->n_cu, nullptr,
                              nullptr, nullptr, true)) != hipSuccess)
@@ -2098,17 +2098,19 @@ hipError_t batch_layout(ppm_batch* b) {
         1, std::min<uint64_t>(it.eb.q_sort_blocks, (3ull * it.eb.q_primary / 2 + 8191) / 8192));
     uint32_t cnt[BK_KINDS];
     cnt[BK_TRACE] = (it.pb.n_regions + 3) / 4;
-    cnt[BK_MORTON] = (uint32_t)std::min<uint64_t>(8192, std::max<uint64_t>(1, (it.m.cap + 255) / 256));
+    // the grid kernels over the photons the room is expected to store
+    const uint64_t expect = std::min<uint64_t>(it.m.cap, 2ull * s->sbuf.n_emit_total);
+    cnt[BK_MORTON] = grid_stride_blocks(expect);
     cnt[BK_PSORT] = it.psort_blocks;
     cnt[BK_SCAN] = 1;  // one block scans all 256 digit rows (kb_rs_scan)
-    cnt[BK_LEAVES] = grid_photon_blocks(it.m.cap);
-    cnt[BK_CLEAR] = grid_clear_blocks(it.m.cells_max);
+    cnt[BK_LEAVES] = grid_stride_blocks(expect);
+    cnt[BK_CLEAR] = grid_stride_blocks(std::min<uint64_t>(it.m.cells_max, next_pow2_u64(expect)));
     cnt[BK_EYE] = (it.eye_waves + 3) / 4;
     cnt[BK_PIX] = (it.eb.q_primary + 255) / 256;
-    cnt[BK_QKEYS] = (it.eb.q_primary + it.eb.ovf_cap + 255) / 256;
+    cnt[BK_QKEYS] = (it.eb.q_primary + 255) / 256 + OVF_BLOCKS;
     cnt[BK_QSORT] = it.qsort_blocks;
     cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, it.eb.q_primary / 128u));
-    cnt[BK_OVF] = (it.eb.ovf_cap + 255) / 256;
+    cnt[BK_OVF] = OVF_BLOCKS;
     for (int k = 0; k < BK_KINDS; ++k) b->htab[(size_t)k * (n + 1) + j + 1] = b->htab[(size_t)k * (n + 1) + j] + cnt[k];
     // the other pipelines: the same sizes on their own buffers, passes
     // first_pass + q, + q + K, ...

@@ -1387,13 +1387,12 @@ __device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const d
 
 
 hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin,
-                             float4 bmax) {
+                             float4 bmax, uint64_t expect) {
   if (!m.cells || m.cells_max == 0 || m.cells_max > GRID_MAX_BUCKETS || (m.cells_max & (m.cells_max - 1)) != 0)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_grid_setup, dim3(1), dim3(64), 0, s, st, bmin, bmax, m.cells_max);
-  uint32_t kblocks = (uint32_t)((m.cap + 255) / 256);
-  if (kblocks > 8192) kblocks = 8192;
-  if (kblocks == 0) kblocks = 1;
+  const uint64_t items = expect != 0 ? std::min<uint64_t>(expect, m.cap) : m.cap;
+  const uint32_t kblocks = grid_stride_blocks(items);
   hipLaunchKernelGGL(k_grid_keys, dim3(kblocks), dim3(256), 0, s, in.rec, in.offset, in.n_regions, in.region_cap,
                      (const PassState*)st, m.keys[0], m.vals[0], bmin, &st->nonfinite, m.os.ctl);
   hipError_t e = hipGetLastError();
@@ -1404,10 +1403,9 @@ hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const Phot
   const OsRecords rec{in.rec, in.rec64, m.sorted, m.sorted64};
   e = onesweep_sort(s, m.os, m.keys, m.vals, &st->n_stored, GRID_KEY_BITS, &st->overflow, &cur, &rec, m.src_idx);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_grid_clear, dim3(grid_clear_blocks(m.cells_max)), dim3(256), 0, s, m.cells,
-                     (const PassState*)st);
-  const uint32_t pblocks = grid_photon_blocks(m.cap);
-  hipLaunchKernelGGL(k_grid_cells, dim3(pblocks), dim3(256), 0, s, m.keys[cur], (const PassState*)st, m.cells);
+  hipLaunchKernelGGL(k_grid_clear, dim3(grid_stride_blocks(std::min<uint64_t>(m.cells_max, next_pow2_u64(items)))),
+                     dim3(256), 0, s, m.cells, (const PassState*)st);
+  hipLaunchKernelGGL(k_grid_cells, dim3(kblocks), dim3(256), 0, s, m.keys[cur], (const PassState*)st, m.cells);
   return hipGetLastError();
 }
 
@@ -2140,9 +2138,10 @@ __device__ __forceinline__ void k_query_keys_body(EyeBuf eb, const PassState* st
   const uint32_t g = t >> 6, lane = lane_id();
   const uint64_t ovf_base = (uint64_t)eb.q_primary * (1u + eb.sec_max);
   if (g >= eb.n_tiles) {
+    // the threads past the tiles key the overflow queries, striding
     const uint32_t n_ovf = min(st->n_queries, eb.ovf_cap);
-    const uint32_t i = t - eb.n_tiles * 64u;
-    if (i < n_ovf) {
+    const uint32_t stride = blk.n * blockDim.x - eb.n_tiles * 64u;
+    for (uint32_t i = t - eb.n_tiles * 64u; i < n_ovf; i += stride) {
       const uint32_t pos = st->n_qsort - n_ovf + i;
       const float4 a = eb.qpos[ovf_base + i];
       keys[pos] = morton_key(a.x, a.y, a.z, bmin, bscale, fq, lshift);
@@ -2462,14 +2461,15 @@ hipError_t launch_adjoint(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pa
 
 // Overflow results into their pixels (float atomics; few per pass).
 __device__ __forceinline__ void k_overflow_add_body(EyeBuf eb, const PassState* st, Blk blk) {
-  const uint32_t i = blk.b * blockDim.x + threadIdx.x;
-  if (i >= min(st->n_queries, eb.ovf_cap)) return;
-  const uint64_t slot = (uint64_t)eb.q_primary * (1u + eb.sec_max) + i;
-  const uint32_t p = __float_as_uint(eb.q[slot].a.w);
-  const float4 r = eb.res[slot];
-  atomicAdd(&eb.pix[p].x, r.x);
-  atomicAdd(&eb.pix[p].y, r.y);
-  atomicAdd(&eb.pix[p].z, r.z);
+  const uint32_t n = min(st->n_queries, eb.ovf_cap);
+  for (uint32_t i = blk.b * blockDim.x + threadIdx.x; i < n; i += blk.n * blockDim.x) {
+    const uint64_t slot = (uint64_t)eb.q_primary * (1u + eb.sec_max) + i;
+    const uint32_t p = __float_as_uint(eb.q[slot].a.w);
+    const float4 r = eb.res[slot];
+    atomicAdd(&eb.pix[p].x, r.x);
+    atomicAdd(&eb.pix[p].y, r.y);
+    atomicAdd(&eb.pix[p].z, r.z);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_overflow_add(EyeBuf eb, const PassState* st) {
@@ -2533,7 +2533,7 @@ hipError_t launch_query_prep(hipStream_t s, const SceneBuf& sc, const MapBuf& m,
   const uint32_t lshift = fine ? QUERY_FINE_LEVEL_SHIFT : QUERY_LEVEL_SHIFT;
   const uint32_t kbits = fine ? QUERY_FINE_KEY_BITS : QUERY_KEY_BITS;
   const int first = (int)((kbits / 8u) & 1u);
-  hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, st, sc.bmin,
+  hipLaunchKernelGGL(k_query_keys, dim3((eb.q_primary + 255) / 256 + OVF_BLOCKS), dim3(256), 0, s, eb, st, sc.bmin,
                      morton_scale(sc.bmin, sc.bmax, qmax), (float)qmax, lshift, eb.qkeys[first], eb.qvals[first],
                      eb.qos.ctl);
   hipError_t e = hipGetLastError();
@@ -2587,7 +2587,7 @@ hipError_t launch_gather(hipStream_t s, const SceneBuf& sc, const MapBuf& m, Pas
     }
   }
   if (ev && (e = hipEventRecord(ev[1], s)) != hipSuccess) return e;
-  hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
+  hipLaunchKernelGGL(k_overflow_add, dim3(OVF_BLOCKS), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, sum, sumsq);
   return hipGetLastError();
 }
@@ -2647,7 +2647,7 @@ hipError_t launch_classic(hipStream_t s, const SceneBuf& sc, PassState* st, cons
       hipLaunchKernelGGL(k_shade_classic<0>, dim3(sblocks), dim3(256), 0, s, v, (const PassState*)st, eb,
                          eb.qvals[0], ambient);
   }
-  hipLaunchKernelGGL(k_overflow_add, dim3((eb.ovf_cap + 255) / 256), dim3(256), 0, s, eb, (const PassState*)st);
+  hipLaunchKernelGGL(k_overflow_add, dim3(OVF_BLOCKS), dim3(256), 0, s, eb, (const PassState*)st);
   hipLaunchKernelGGL(k_accumulate, dim3((eb.q_primary + 255) / 256), dim3(256), 0, s, eb, sc.cam, (double*)nullptr,
                      (double*)nullptr);
   return hipGetLastError();

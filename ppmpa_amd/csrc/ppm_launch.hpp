@@ -193,17 +193,27 @@ constexpr uint32_t QUERY_FINE_LEVEL_SHIFT = 30;
 constexpr uint32_t QUERY_FINE_KEY_BITS = 32;
 constexpr uint32_t QUERY_FINE_PIXELS = 1u << 20;
 
-// Blocks of the per-photon and per-bucket grid kernels.
-inline uint32_t grid_photon_blocks(uint64_t cap) {
-  const uint64_t b = (cap + 255) / 256;
+// Blocks of 256 threads for a grid-stride kernel over about `items`
+// elements, four per thread.  The grid kernels are sized from the photons a
+// pass is expected to store (twice those emitted) rather than from the
+// capacity (emitted times the depth limit): in a batch the capacity-sized
+// grids summed to over a million mostly idle blocks per round.
+inline uint32_t grid_stride_blocks(uint64_t items) {
+  const uint64_t b = (items + 1023) / 1024;
   return (uint32_t)(b < 1 ? 1 : (b > 16384 ? 16384 : b));
 }
-inline uint32_t grid_clear_blocks(uint32_t cells_max) {
-  const uint32_t b = (cells_max + 255) / 256;
-  return b < 1 ? 1 : (b > 4096 ? 4096 : b);
+// Blocks for the overflow queries (a grid-stride loop: there are rarely many).
+constexpr uint32_t OVF_BLOCKS = 16;
+inline uint64_t next_pow2_u64(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
 }
 
-hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax);
+// expect: the photons the pass is expected to store, which sizes the grid
+// kernels (0: the set's capacity).
+hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin, float4 bmax,
+                             uint64_t expect = 0);
 
 hipError_t launch_eye(hipStream_t s, const SceneBuf& sc, PassState* st, const EyeBuf& eb, uint64_t seed,
                       int use_classic, int max_depth, int n_cu);
