@@ -341,7 +341,6 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
   eb.n_pix = (uint32_t)cam.xreso * (uint32_t)cam.yreso;
   eb.ovf_cap = 2u * eb.n_pix + 1024u;
   const uint64_t inplace = (uint64_t)eb.q_primary * (1u + eb.sec_max);
-  eb.q_sort_blocks = (uint32_t)std::min<uint64_t>(1024, std::max<uint64_t>(1, (uint64_t)n_cu * 4));
   hipError_t e;
   if ((e = dalloc(&eb.q, inplace + eb.ovf_cap)) != hipSuccess) return e;
   if ((e = dalloc(&eb.qpos, inplace + eb.ovf_cap)) != hipSuccess) return e;
@@ -355,8 +354,6 @@ hipError_t eye_alloc(EyeBuf& eb, const DevCamera& cam, int max_depth, int n_cu) 
     if ((e = dalloc(&eb.qkeys[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
     if ((e = dalloc(&eb.qvals[i], inplace + eb.ovf_cap)) != hipSuccess) return e;
   }
-  if ((e = dalloc(&eb.qghist, 256ull * eb.q_sort_blocks)) != hipSuccess) return e;
-  if ((e = dalloc(&eb.qtotals, 256)) != hipSuccess) return e;
   if ((e = onesweep_alloc(eb.qos, inplace + eb.ovf_cap, n_cu)) != hipSuccess) return e;
   if ((e = dalloc(&eb.res, inplace + eb.ovf_cap)) != hipSuccess) return e;
   eb.estack_lanes = 64u * std::max<uint32_t>(1, std::min<uint32_t>(tiles, (uint32_t)n_cu * EYE_WAVES_PER_CU_MAX));
@@ -378,8 +375,6 @@ void eye_free(EyeBuf& eb) {
     dfree(eb.qkeys[i]);
     dfree(eb.qvals[i]);
   }
-  dfree(eb.qghist);
-  dfree(eb.qtotals);
   onesweep_free(eb.qos);
   dfree(eb.res);
   dfree(eb.estack);
@@ -2088,27 +2083,31 @@ hipError_t batch_layout(ppm_batch* b) {
     it.pfilter = s->cfg.pfilter;
     it.trace_per_wave = (uint32_t)(((uint64_t)s->sbuf.n_emit_total + it.pb.n_regions - 1) / it.pb.n_regions);
     it.eye_waves = std::max(1u, std::min(std::min(it.eb.n_tiles, n_cu * ew), it.eb.estack_lanes / 64u));
-    // sort blocks from the expected key counts (about 2 stored photons per
-    // emitted one, 1.5 queries per pixel), 8192 keys per block: a stable
-    // radix sort gives the same order for any block count, and a batch's
-    // sorts together need no more blocks than fill the GPU
-    it.psort_blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(it.m.sort_blocks, (2ull * s->sbuf.n_emit_total + 8191) / 8192));
-    it.qsort_blocks = (uint32_t)std::max<uint64_t>(
-        1, std::min<uint64_t>(it.eb.q_sort_blocks, (3ull * it.eb.q_primary / 2 + 8191) / 8192));
+    // the sorts' blocks from the expected key counts (about 2 stored photons
+    // per emitted one, 1.5 queries per pixel): the histogram blocks stride
+    // and the digit passes' persistent blocks take tiles until none is left,
+    // so any count sorts every key, and a batch's sorts together need no
+    // more blocks than fill the GPU
+    const uint64_t pkeys = std::min<uint64_t>(it.m.cap, 2ull * s->sbuf.n_emit_total);
+    const uint64_t qkeys = 3ull * it.eb.q_primary / 2;
+    auto hist_blocks = [](uint64_t k) { return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(OS_HIST_BLOCKS, k / 16384)); };
+    auto os_blocks = [](const OneSweep& o, uint64_t k) {
+      return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(o.blocks, (k + OS_TILE - 1) / OS_TILE));
+    };
     uint32_t cnt[BK_KINDS];
     cnt[BK_TRACE] = (it.pb.n_regions + 3) / 4;
     // the grid kernels over the photons the room is expected to store
     const uint64_t expect = std::min<uint64_t>(it.m.cap, 2ull * s->sbuf.n_emit_total);
     cnt[BK_MORTON] = grid_stride_blocks(expect);
-    cnt[BK_PSORT] = it.psort_blocks;
-    cnt[BK_SCAN] = 1;  // one block scans all 256 digit rows (kb_rs_scan)
+    cnt[BK_PHIST] = hist_blocks(pkeys);
+    cnt[BK_POS] = os_blocks(it.m.os, pkeys);
     cnt[BK_LEAVES] = grid_stride_blocks(expect);
     cnt[BK_CLEAR] = grid_stride_blocks(std::min<uint64_t>(it.m.cells_max, next_pow2_u64(expect)));
     cnt[BK_EYE] = (it.eye_waves + 3) / 4;
     cnt[BK_PIX] = (it.eb.q_primary + 255) / 256;
     cnt[BK_QKEYS] = (it.eb.q_primary + 255) / 256 + OVF_BLOCKS;
-    cnt[BK_QSORT] = it.qsort_blocks;
+    cnt[BK_QHIST] = hist_blocks(qkeys);
+    cnt[BK_QOS] = os_blocks(it.eb.qos, qkeys);
     cnt[BK_SHADE] = std::max(1u, std::min(n_cu * sb, it.eb.q_primary / 128u));
     cnt[BK_OVF] = OVF_BLOCKS;
     for (int k = 0; k < BK_KINDS; ++k) b->htab[(size_t)k * (n + 1) + j + 1] = b->htab[(size_t)k * (n + 1) + j] + cnt[k];

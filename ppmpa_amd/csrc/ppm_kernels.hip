@@ -656,32 +656,6 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* ghist, uint32_t* tot
   k_rs_scan_body(ghist, totals, G, Blk{blockIdx.x, gridDim.x});
 }
 
-// The same 256 scans by one block: each wave scans digit rows w, w + W, ...
-// 64 counts at a time, carrying the row's running total.  The batch uses it
-// with one block per scene: 256 blocks per scene, each paying its scene
-// lookup and item load for a few hundred counts, took 580 us per launch
-// over 64 scenes.
-__device__ __forceinline__ void k_rs_scan_rows_body(uint32_t* ghist, uint32_t* totals, uint32_t G) {
-  const uint32_t lane = lane_id(), nw = blockDim.x >> 6;
-  for (uint32_t d = threadIdx.x >> 6; d < 256u; d += nw) {
-    uint32_t* row = ghist + (uint64_t)d * G;
-    uint32_t run = 0;
-    for (uint32_t base = 0; base < G; base += 64u) {
-      const uint32_t i = base + lane;
-      const uint32_t v = i < G ? row[i] : 0u;
-      uint32_t incl = v;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t u = __shfl_up(incl, o);
-        if (lane >= (uint32_t)o) incl += u;
-      }
-      if (i < G) row[i] = run + incl - v;
-      run += __shfl(incl, 63);
-    }
-    if (lane == 0) totals[d] = run;
-  }
-}
-
 // Stable scatter.  Each round moves 256 keys: lanes with equal digits are
 // matched with 8 ballots, ranked by mbcnt, and placed after the same digit's
 // keys of earlier waves and earlier rounds.
@@ -801,16 +775,17 @@ __device__ __forceinline__ void os_clear_ctl(uint32_t* ctl, Blk blk) {
 // keys with one copy).  Then one global atomic add per nonzero bin.
 constexpr uint32_t OS_HIST_COPIES = 16;
 constexpr uint32_t OS_HIST_STRIDE = OS_MAX_DIGITS * 256u + 1u;
-__global__ __launch_bounds__(1024) void k_os_hist(const uint32_t* keys, const uint32_t* np, uint32_t cap,
-                                                  uint32_t digits, uint32_t* ctl, uint32_t* look, uint32_t tiles) {
+__device__ __forceinline__ void k_os_hist_body(const uint32_t* keys, const uint32_t* np, uint32_t cap,
+                                               uint32_t digits, uint32_t* ctl, uint32_t* look, uint32_t tiles,
+                                               Blk blk) {
   __shared__ uint32_t h[OS_HIST_COPIES * OS_HIST_STRIDE];
   const uint32_t t = threadIdx.x;
   for (uint32_t i = t; i < OS_HIST_COPIES * OS_HIST_STRIDE; i += blockDim.x) h[i] = 0u;
   __syncthreads();
   const uint32_t n = min(*np, cap);
-  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t stride = blk.n * blockDim.x;
   uint32_t* const mine = h + (t % OS_HIST_COPIES) * OS_HIST_STRIDE;
-  for (uint32_t i = blockIdx.x * blockDim.x + t; i < n; i += stride) {
+  for (uint32_t i = blk.b * blockDim.x + t; i < n; i += stride) {
     const uint32_t k = keys[i];
     for (uint32_t p = 0; p < digits; ++p) atomicAdd(&mine[p * 256u + ((k >> (8u * p)) & 255u)], 1u);
   }
@@ -823,8 +798,13 @@ __global__ __launch_bounds__(1024) void k_os_hist(const uint32_t* keys, const ui
   const uint32_t words = (n + OS_TILE - 1) / OS_TILE * 256u;
   for (uint32_t p = 0; p < digits; ++p) {
     uint32_t* lk = look + (uint64_t)p * tiles * 256u;
-    for (uint32_t i = blockIdx.x * blockDim.x + t; i < words; i += stride) lk[i] = 0u;
+    for (uint32_t i = blk.b * blockDim.x + t; i < words; i += stride) lk[i] = 0u;
   }
+}
+
+__global__ __launch_bounds__(1024) void k_os_hist(const uint32_t* keys, const uint32_t* np, uint32_t cap,
+                                                  uint32_t digits, uint32_t* ctl, uint32_t* look, uint32_t tiles) {
+  k_os_hist_body(keys, np, cap, digits, ctl, look, tiles, Blk{blockIdx.x, gridDim.x});
 }
 
 // Photon records moved with the last digit pass of the grid's sort: out[i]
@@ -844,10 +824,10 @@ struct OsRecords {
 // for the tile's counts and look-back.  hist = this digit's 256 counts;
 // look = this digit's words.
 template <bool REC>
-__global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
-                                                           uint32_t* vout, const uint32_t* np, uint32_t cap,
-                                                           const uint32_t* hist, uint32_t* ticket, uint32_t* look,
-                                                           uint32_t shift, uint32_t* err, OsRecords rec) {
+__device__ __forceinline__ void k_os_scatter_body(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                  uint32_t* vout, const uint32_t* np, uint32_t cap,
+                                                  const uint32_t* hist, uint32_t* ticket, uint32_t* look,
+                                                  uint32_t shift, uint32_t* err, OsRecords rec, Blk blk) {
   constexpr uint32_t W = OS_THREADS / 64, KPT = OS_KEYS_PER_THREAD;
   static_assert(OS_THREADS >= 256 && OS_THREADS % 256 == 0, "digit owners are threads 0..255");
   __shared__ uint32_t sk[OS_TILE];
@@ -862,7 +842,7 @@ __global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, 
   const bool owner = t < 256u;
   const uint32_t n = min(*np, cap);
   const uint32_t n_tiles = (n + OS_TILE - 1) / OS_TILE;
-  if (REC && blockIdx.x == 0 && t < GRID_READ_SLACK)
+  if (REC && blk.b == 0 && t < GRID_READ_SLACK)
     rec.out[n + t] = PhotonRec{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
   // exclusive scan of 256 counts held by the owners (all threads call it)
   auto owner_scan = [&](uint32_t c) -> uint32_t {
@@ -989,6 +969,15 @@ __global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, 
       }
     }
   }
+}
+
+template <bool REC>
+__global__ __launch_bounds__(OS_THREADS) void k_os_scatter(const uint32_t* kin, const uint32_t* vin, uint32_t* kout,
+                                                           uint32_t* vout, const uint32_t* np, uint32_t cap,
+                                                           const uint32_t* hist, uint32_t* ticket, uint32_t* look,
+                                                           uint32_t shift, uint32_t* err, OsRecords rec) {
+  k_os_scatter_body<REC>(kin, vin, kout, vout, np, cap, hist, ticket, look, shift, err, rec,
+                         Blk{blockIdx.x, gridDim.x});
 }
 
 // The sort of n = *np <= o.cap keys (written by a kernel that cleared
@@ -1362,29 +1351,6 @@ __device__ __forceinline__ void k_grid_cells_body(const uint32_t* keys, const Pa
 __global__ __launch_bounds__(256) void k_grid_cells(const uint32_t* keys, const PassState* st, uint2* cells) {
   k_grid_cells_body(keys, st, cells, Blk{blockIdx.x, gridDim.x});
 }
-
-// Photon records (and double positions, if any) in bucket order, in the
-// grid's gather form (GridRec), and the GRID_READ_SLACK records after the
-// last one zeroed (the batch's grid build; a session's moves them with the
-// last digit pass of its sort).
-__device__ __forceinline__ void k_grid_permute_body(const PhotonRec* in, const double* in64, const uint32_t* order,
-                                                    PhotonRec* out, double* out64, uint32_t* src_idx,
-                                                    const PassState* st, Blk blk) {
-  const uint32_t n = st->n_stored;
-  const uint32_t t = blk.b * blockDim.x + threadIdx.x;
-  if (t < GRID_READ_SLACK) out[n + t] = PhotonRec{make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
-  for (uint32_t i = t; i < n; i += blk.n * blockDim.x) {
-    const uint32_t src = order[i];
-    out[i] = grid_rec(in[src]);
-    src_idx[i] = src;
-    if (in64) {
-      out64[3 * (uint64_t)i + 0] = in64[3 * (uint64_t)src + 0];
-      out64[3 * (uint64_t)i + 1] = in64[3 * (uint64_t)src + 1];
-      out64[3 * (uint64_t)i + 2] = in64[3 * (uint64_t)src + 2];
-    }
-  }
-}
-
 
 hipError_t launch_grid_build(hipStream_t s, MapBuf& m, PassState* st, const PhotonBuf& in, float4 bmin,
                              float4 bmax, uint64_t expect) {
@@ -2932,7 +2898,7 @@ __global__ __launch_bounds__(256) void kb_grid_keys(const BatchItem* items, cons
                                                     uint32_t n_all) {
   BATCH_PROLOGUE(BK_MORTON)
   k_grid_keys_body(it.pb.rec, it.pb.offset, it.pb.n_regions, it.pb.region_cap, it.st, it.m.keys[0], it.m.vals[0],
-                   it.sc.bmin, &it.st->nonfinite, nullptr, blk);
+                   it.sc.bmin, &it.st->nonfinite, it.m.os.ctl, blk);
 }
 
 __global__ __launch_bounds__(256) void kb_grid_clear(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -2950,45 +2916,43 @@ __global__ __launch_bounds__(256) void kb_grid_cells(const BatchItem* items, con
   k_grid_cells_body(it.m.keys[GRID_SORTED], it.st, it.m.cells, blk);
 }
 
-__global__ __launch_bounds__(256) void kb_grid_permute(const BatchItem* items, const uint32_t* tab, uint32_t n,
-                                                       uint32_t n_all) {
-  BATCH_PROLOGUE(BK_LEAVES)
-  k_grid_permute_body(it.pb.rec, it.pb.rec64, it.m.vals[GRID_SORTED], it.m.sorted, it.m.sorted64, it.m.src_idx, it.st,
-                      blk);
-}
-
-// Radix-sort digit pass `p` (shift 8p) of the photon keys (which = 0, from
-// buffer 0) or the query keys (which = 1, from buffer QUERY_KEYS_FIRST).
-__global__ __launch_bounds__(RS_THREADS) void kb_rs_hist(const BatchItem* items, const uint32_t* tab, uint32_t n,
-                                                         uint32_t n_all, int which, uint32_t p) {
-  BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
-  // selects, not a runtime index into the item's arrays (which would put the
-  // whole item in scratch memory)
-  const bool odd = ((p + (which ? QUERY_KEYS_FIRST : 0u)) & 1u) != 0;
-  const uint32_t* keys = which ? (odd ? it.eb.qkeys[1] : it.eb.qkeys[0]) : (odd ? it.m.keys[1] : it.m.keys[0]);
-  k_rs_hist_body(keys, which ? &it.st->n_qsort : &it.st->n_stored, which ? it.eb.qghist : it.m.ghist, 8u * p, blk);
-}
-
-__global__ __launch_bounds__(1024) void kb_rs_scan(const BatchItem* items, const uint32_t* tab, uint32_t n,
+// The sorts' histograms: the photon keys (which = 0, in buffer 0) or the
+// query keys (which = 1, in buffer QUERY_KEYS_FIRST); selects, not a runtime
+// index into the item's arrays (which would put the whole item in scratch
+// memory).
+__global__ __launch_bounds__(1024) void kb_os_hist(const BatchItem* items, const uint32_t* tab, uint32_t n,
                                                    uint32_t n_all, int which) {
-  BATCH_PROLOGUE(BK_SCAN)
-  (void)blk;
-  k_rs_scan_rows_body(which ? it.eb.qghist : it.m.ghist, which ? it.eb.qtotals : it.m.totals,
-                      which ? it.qsort_blocks : it.psort_blocks);
+  BATCH_PROLOGUE(which ? BK_QHIST : BK_PHIST)
+  if (which)
+    k_os_hist_body(QUERY_KEYS_FIRST != 0u ? it.eb.qkeys[1] : it.eb.qkeys[0], &it.st->n_qsort, (uint32_t)it.eb.qos.cap,
+                   QUERY_KEY_BITS / 8u, it.eb.qos.ctl, it.eb.qos.look, it.eb.qos.tiles, blk);
+  else
+    k_os_hist_body(it.m.keys[0], &it.st->n_stored, (uint32_t)it.m.os.cap, GRID_KEY_BITS / 8u, it.m.os.ctl,
+                   it.m.os.look, it.m.os.tiles, blk);
 }
 
-__global__ __launch_bounds__(RS_THREADS) void kb_rs_scatter(const BatchItem* items, const uint32_t* tab,
-                                                            uint32_t n, uint32_t n_all, int which, uint32_t p) {
-  BATCH_PROLOGUE(which ? BK_QSORT : BK_PSORT)
+// Digit pass p of a sort; REC: the last pass of the photon sort, which also
+// moves the records into bucket order and writes the sorted source indices
+// to src_idx.
+template <bool REC>
+__global__ __launch_bounds__(OS_THREADS) void kb_os_scatter(const BatchItem* items, const uint32_t* tab, uint32_t n,
+                                                            uint32_t n_all, int which, uint32_t p) {
+  BATCH_PROLOGUE(which ? BK_QOS : BK_POS)
   const bool odd = ((p + (which ? QUERY_KEYS_FIRST : 0u)) & 1u) != 0;
-  if (which)
-    k_rs_scatter_body(odd ? it.eb.qkeys[1] : it.eb.qkeys[0], odd ? it.eb.qvals[1] : it.eb.qvals[0],
-                      odd ? it.eb.qkeys[0] : it.eb.qkeys[1], odd ? it.eb.qvals[0] : it.eb.qvals[1], &it.st->n_qsort,
-                      it.eb.qghist, it.eb.qtotals, 8u * p, blk);
-  else
-    k_rs_scatter_body(odd ? it.m.keys[1] : it.m.keys[0], odd ? it.m.vals[1] : it.m.vals[0],
-                      odd ? it.m.keys[0] : it.m.keys[1], odd ? it.m.vals[0] : it.m.vals[1], &it.st->n_stored,
-                      it.m.ghist, it.m.totals, 8u * p, blk);
+  if (!REC && which) {  // (one body per kernel: each instance has its own LDS)
+    k_os_scatter_body<false>(odd ? it.eb.qkeys[1] : it.eb.qkeys[0], odd ? it.eb.qvals[1] : it.eb.qvals[0],
+                             odd ? it.eb.qkeys[0] : it.eb.qkeys[1], odd ? it.eb.qvals[0] : it.eb.qvals[1],
+                             &it.st->n_qsort, (uint32_t)it.eb.qos.cap, it.eb.qos.ctl + 256u * p,
+                             it.eb.qos.ctl + OS_TICKETS + 32u * p, it.eb.qos.look + (uint64_t)p * it.eb.qos.tiles * 256u,
+                             8u * p, &it.st->overflow, OsRecords{}, blk);
+  } else {
+    const OsRecords rec = REC ? OsRecords{it.pb.rec, it.pb.rec64, it.m.sorted, it.m.sorted64} : OsRecords{};
+    k_os_scatter_body<REC>(odd ? it.m.keys[1] : it.m.keys[0], odd ? it.m.vals[1] : it.m.vals[0],
+                           odd ? it.m.keys[0] : it.m.keys[1],
+                           REC ? it.m.src_idx : (odd ? it.m.vals[0] : it.m.vals[1]), &it.st->n_stored,
+                           (uint32_t)it.m.os.cap, it.m.os.ctl + 256u * p, it.m.os.ctl + OS_TICKETS + 32u * p,
+                           it.m.os.look + (uint64_t)p * it.m.os.tiles * 256u, 8u * p, &it.st->overflow, rec, blk);
+  }
 }
 
 __global__ __launch_bounds__(256) void kb_tile_order(const BatchItem* items, const uint32_t* tab, uint32_t n,
@@ -3016,7 +2980,7 @@ __global__ __launch_bounds__(256) void kb_query_keys(const BatchItem* items, con
   // batches sort on the coarse keys whatever their rooms' sizes
   k_query_keys_body(it.eb, it.st, it.sc.bmin, it.mscale, (float)QUERY_MORTON_MAX, QUERY_LEVEL_SHIFT,
                     QUERY_KEYS_FIRST != 0u ? it.eb.qkeys[1] : it.eb.qkeys[0],
-                    QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], nullptr, blk);
+                    QUERY_KEYS_FIRST != 0u ? it.eb.qvals[1] : it.eb.qvals[0], it.eb.qos.ctl, blk);
 }
 
 template <int BV, bool BVH = (BV != 0)>
@@ -3047,7 +3011,7 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   // blocks[k] = total blocks of kind k over the first n scenes.  The eye
   // branch (tile order, eye trace, query sort) runs on s2 beside the photon
   // branch (trace, grid build): the two touch disjoint buffers and counters.
-  const dim3 T256(256), T1024(1024), T64(64), TRS(RS_THREADS);
+  const dim3 T256(256), T1024(1024), T64(64), TOS(OS_THREADS);
   hipError_t e;
   hipLaunchKernelGGL(kb_pass_begin, dim3(n), T64, 0, s, items, n);
   if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
@@ -3066,11 +3030,9 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   hipLaunchKernelGGL(kb_tile_count, dim3(blocks[BK_PIX]), T256, 0, s2, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_scan_tiles, dim3(n), T1024, 0, s2, items, n);
   hipLaunchKernelGGL(kb_query_keys, dim3(blocks[BK_QKEYS]), T256, 0, s2, items, tab, n, n_all);
-  for (uint32_t p = 0; p < QUERY_KEY_BITS / 8u; ++p) {
-    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);
-    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s2, items, tab, n, n_all, 1);
-    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_QSORT]), TRS, 0, s2, items, tab, n, n_all, 1, p);
-  }
+  hipLaunchKernelGGL(kb_os_hist, dim3(blocks[BK_QHIST]), T1024, 0, s2, items, tab, n, n_all, 1);
+  for (uint32_t p = 0; p < QUERY_KEY_BITS / 8u; ++p)
+    hipLaunchKernelGGL(kb_os_scatter<false>, dim3(blocks[BK_QOS]), TOS, 0, s2, items, tab, n, n_all, 1, p);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipEventRecord(join, s2)) != hipSuccess) return e;
   switch ((bvh ? (dyn != 0u ? 2 : 1) : 0)) {
@@ -3086,14 +3048,14 @@ hipError_t launch_batch_pass(hipStream_t s, hipStream_t s2, hipEvent_t fork, hip
   hipLaunchKernelGGL(kb_scan_photons, dim3(n), T1024, 0, s, items, n);
   hipLaunchKernelGGL(kb_grid_setup, dim3(n), T64, 0, s, items, n);
   hipLaunchKernelGGL(kb_grid_keys, dim3(blocks[BK_MORTON]), T256, 0, s, items, tab, n, n_all);
-  for (uint32_t p = 0; p < GRID_KEY_BITS / 8u; ++p) {
-    hipLaunchKernelGGL(kb_rs_hist, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
-    hipLaunchKernelGGL(kb_rs_scan, dim3(blocks[BK_SCAN]), T1024, 0, s, items, tab, n, n_all, 0);
-    hipLaunchKernelGGL(kb_rs_scatter, dim3(blocks[BK_PSORT]), TRS, 0, s, items, tab, n, n_all, 0, p);
-  }
+  // the last digit pass also moves the records into bucket order
+  hipLaunchKernelGGL(kb_os_hist, dim3(blocks[BK_PHIST]), T1024, 0, s, items, tab, n, n_all, 0);
+  for (uint32_t p = 0; p + 1u < GRID_KEY_BITS / 8u; ++p)
+    hipLaunchKernelGGL(kb_os_scatter<false>, dim3(blocks[BK_POS]), TOS, 0, s, items, tab, n, n_all, 0, p);
+  hipLaunchKernelGGL(kb_os_scatter<true>, dim3(blocks[BK_POS]), TOS, 0, s, items, tab, n, n_all, 0,
+                     GRID_KEY_BITS / 8u - 1u);
   hipLaunchKernelGGL(kb_grid_clear, dim3(blocks[BK_CLEAR]), T256, 0, s, items, tab, n, n_all);
   hipLaunchKernelGGL(kb_grid_cells, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
-  hipLaunchKernelGGL(kb_grid_permute, dim3(blocks[BK_LEAVES]), T256, 0, s, items, tab, n, n_all);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
   switch ((bvh ? (dyn != 0u ? 2 : 1) : 0)) {

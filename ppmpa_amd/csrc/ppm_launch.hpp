@@ -126,9 +126,6 @@ struct EyeBuf {
   uint32_t* tile_order = nullptr;   // the eye kernel's tile order (k_tile_order)
   uint32_t* qkeys[2] = {nullptr, nullptr};
   uint32_t* qvals[2] = {nullptr, nullptr};
-  uint32_t* qghist = nullptr;
-  uint32_t* qtotals = nullptr;
-  uint32_t q_sort_blocks = 0;
   OneSweep qos;         // the query sort of sessions
   uint32_t q_fine = 0;  // 1: fine query sort keys (QUERY_FINE_*)
   float4* res = nullptr;
@@ -299,8 +296,6 @@ struct BatchItem {
   int32_t uc, maxd, pfilter;
   uint32_t trace_per_wave;   // photons per tracing wave
   uint32_t eye_waves;        // persistent eye waves
-  uint32_t psort_blocks;     // blocks of the photon-key sort (<= m.sort_blocks)
-  uint32_t qsort_blocks;     // blocks of the query-key sort (<= eb.q_sort_blocks)
 };
 
 // Launch kinds of a batched pass; tab[k * (n_all + 1) + s] is the first block
@@ -308,16 +303,17 @@ struct BatchItem {
 enum BatchKind : int {
   BK_TRACE = 0,  // photon tracing waves / 4
   BK_MORTON,     // photon grid keys
-  BK_PSORT,      // photon sort blocks (hist, scatter)
-  BK_SCAN,       // one block per scene for a sort's scan of its 256 digit rows
-  BK_LEAVES,     // grid cell ranges and record permutation (photons / 256)
+  BK_PHIST,      // photon sort histogram (blocks of 1024)
+  BK_POS,        // photon sort digit passes (persistent blocks)
+  BK_LEAVES,     // grid cell ranges
   BK_CLEAR,      // grid bucket clear
   BK_EYE,        // eye waves / 4
   BK_PIX,        // primary slots / 256 (tile counts, accumulation)
-  BK_QKEYS,      // (primary + overflow slots) / 256
-  BK_QSORT,      // query sort blocks
+  BK_QKEYS,      // primary slots / 256 + the overflow blocks
+  BK_QHIST,      // query sort histogram (blocks of 1024)
+  BK_QOS,        // query sort digit passes (persistent blocks)
   BK_SHADE,      // shade blocks
-  BK_OVF,        // overflow slots / 256
+  BK_OVF,        // overflow query blocks
   BK_KINDS
 };
 
