@@ -306,13 +306,27 @@ __device__ __forceinline__ DevMaterial material(const SceneView& sc, uint32_t m)
   return sc.mats[m];
 }
 
-// Slab test of a node box: the ray (o, 1/d) enters the box at tn within
-// [0, tmax].  An axis whose two plane distances are NaN (ray in the plane of
-// a flat box) does not constrain: fminf/fmaxf drop NaN operands.
-__device__ __forceinline__ bool slab(const Node& b, V3 o, V3 inv, float tmax, float& tn) {
-  const float x0 = (b.lo.x - o.x) * inv.x, x1 = (b.hi.x - o.x) * inv.x;
-  const float y0 = (b.lo.y - o.y) * inv.y, y1 = (b.hi.y - o.y) * inv.y;
-  const float z0 = (b.lo.z - o.z) * inv.z, z1 = (b.hi.z - o.z) * inv.z;
+// Slab test of a node box: the ray o + t d, given as inv = 1/d and noi =
+// -o/d, enters the box at tn within [0, tmax].  Each plane distance is one
+// FMA, b inv + noi, whose rounding amounts to about an ulp of the
+// coordinates, far inside the boxes' padding.  An axis whose two plane
+// distances are NaN (a ray in the plane of a flat box) does not constrain:
+// fminf/fmaxf drop NaN operands.
+// 1/d for the slab tests, each component of d first kept at least 1e-20
+// from zero: with an infinite 1/d, b/d - o/d would be inf - inf = NaN and
+// leave that axis unconstrained for the whole walk (such rays walked large
+// parts of a 327k-triangle tree); a finite 1e20 puts the axis's planes
+// at +-huge distances, as the exact difference did.  Primitive tests use d.
+__device__ __forceinline__ V3 slab_inv(V3 d) {
+  const float m = 1e-20f;
+  return mk(1.0f / (fabsf(d.x) < m ? copysignf(m, d.x) : d.x), 1.0f / (fabsf(d.y) < m ? copysignf(m, d.y) : d.y),
+            1.0f / (fabsf(d.z) < m ? copysignf(m, d.z) : d.z));
+}
+
+__device__ __forceinline__ bool slab(const Node& b, V3 inv, V3 noi, float tmax, float& tn) {
+  const float x0 = fmaf(b.lo.x, inv.x, noi.x), x1 = fmaf(b.hi.x, inv.x, noi.x);
+  const float y0 = fmaf(b.lo.y, inv.y, noi.y), y1 = fmaf(b.hi.y, inv.y, noi.y);
+  const float z0 = fmaf(b.lo.z, inv.z, noi.z), z1 = fmaf(b.hi.z, inv.z, noi.z);
   const float t0 = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fmaxf(fminf(z0, z1), 0.0f));
   const float t1 = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fminf(fmaxf(z0, z1), tmax));
   tn = t0;
@@ -361,11 +375,12 @@ __device__ __forceinline__ uint32_t trail_pop(uint32_t& trail, uint32_t node) {
 template <bool CACHED>
 __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
   if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
-  const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const V3 inv = slab_inv(d);
+  const V3 noi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node<CACHED>(sc, 0), o, inv, h.t, tn)) return;
+  if (!slab(bvh_node<CACHED>(sc, 0), inv, noi, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
@@ -374,8 +389,8 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
     } else {
       const uint32_t l = 2u * node + 1u;
       float tl, tr;
-      const bool hl = slab(bvh_node<CACHED>(sc, l), o, inv, h.t, tl);
-      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), o, inv, h.t, tr);
+      const bool hl = slab(bvh_node<CACHED>(sc, l), inv, noi, h.t, tl);
+      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), inv, noi, h.t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -390,7 +405,7 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node<CACHED>(sc, node), o, inv, h.t, tn)) {
+      if (slab(bvh_node<CACHED>(sc, node), inv, noi, h.t, tn)) {
         more = true;
         break;
       }
@@ -653,12 +668,13 @@ __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, 
 template <bool CACHED>
 __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
   if (sc.n_bprims == 0) return false;
-  const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+  const V3 inv = slab_inv(d);
+  const V3 noi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const float t_occ = sqrtf(fmaxf(sq_l - LIGHT_SLACK_F, 0.0f)) * 1.001f + 1e-4f;
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node<CACHED>(sc, 0), o, inv, any ? t_occ : INF_F, tn)) return false;
+  if (!slab(bvh_node<CACHED>(sc, 0), inv, noi, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
@@ -669,8 +685,8 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
       const uint32_t l = 2u * node + 1u;
       const float tmax = any ? t_occ : INF_F;
       float tl, tr;
-      const bool hl = slab(bvh_node<CACHED>(sc, l), o, inv, tmax, tl);
-      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), o, inv, tmax, tr);
+      const bool hl = slab(bvh_node<CACHED>(sc, l), inv, noi, tmax, tl);
+      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), inv, noi, tmax, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -685,7 +701,7 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node<CACHED>(sc, node), o, inv, any ? t_occ : INF_F, tn)) {
+      if (slab(bvh_node<CACHED>(sc, node), inv, noi, any ? t_occ : INF_F, tn)) {
         more = true;
         break;
       }
