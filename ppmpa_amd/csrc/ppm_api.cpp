@@ -380,6 +380,19 @@ void eye_free(EyeBuf& eb) {
   dfree(eb.estack);
 }
 
+// What the overflow flags of a pass state say (device-side bits 1 photon
+// region, 2 overflow queries, 4 a query's cells past the grid's span, 8 the
+// eye's pending pool, 16 a sort's look-back that gave up waiting).
+std::string overflow_text(uint32_t f) {
+  static const char* what[] = {"a photon region's capacity", "the overflow query slots",
+                               "a query's cells past the grid's span", "the eye's pending-node pool",
+                               "a sort's look-back wait (results of that pass are wrong)"};
+  std::string t;
+  for (int b = 0; b < 5; ++b)
+    if (f & (1u << b)) t += (t.empty() ? "" : ", ") + std::string(what[b]);
+  return t.empty() ? "flags " + std::to_string(f) : t;
+}
+
 PassState host_state() {
   PassState s;
   std::memset(&s, 0, sizeof(s));
@@ -1995,7 +2008,7 @@ ppm_status ppm_session_image(ppm_session* s, double* mean_out, double* var_out, 
   if (passes_done) *passes_done = s->passes_done;
   for (const PassState& x : h) {
     if (x.overflow)
-      return fail(c, PPM_E_RANGE, "capacity exceeded during a pass (flags " + std::to_string(x.overflow) + ")");
+      return fail(c, PPM_E_RANGE, "capacity exceeded during a pass: " + overflow_text(x.overflow));
     if (x.nonfinite) return fail(c, PPM_E_NONFINITE, "non-finite photon position (photonmap.rs:26)");
   }
   std::vector<double> sum, sq;
