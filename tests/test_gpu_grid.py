@@ -95,6 +95,8 @@ def _check_radiance(m, om, q, nrm, r2, pfs=(0, 1, 2), min_nonzero=1):
     (5000, 400, 0.05, False, 3),
     (20000, 500, 0.02, True, 4),
     (4096 + 3, 300, 0.04, False, 7),
+    # about 300 tiles of the sort: its look-back over many earlier tiles
+    (1_200_000, 200, 0.01, True, 11),
 ])
 def test_grid_sets_and_radiance(n, nq, r, clustered, seed):
     say("grid sets+radiance", n, nq, r, clustered, seed)
