@@ -216,30 +216,28 @@ struct Hit {
   int32_t obj;
 };
 
-// Nearest-hit update: the earlier of two equal distances is the shape that
-// comes first in the scene, as the stable sort of tracer.rs:335 keeps it.
-__device__ __forceinline__ void take(Hit& h, float t, int32_t idx) {
-  if (t >= NEARLY0_F && (t < h.t || (t == h.t && idx < h.obj))) {
-    h.t = t;
-    h.obj = idx;
-  }
+// Nearest-hit update (when ok): the earlier of two equal distances is the
+// shape that comes first in the scene, as the stable sort of tracer.rs:335
+// keeps it.  Written as selects: the tests of the walks' inner loops then
+// issue no branches.
+__device__ __forceinline__ void take(Hit& h, float t, int32_t idx, bool ok = true) {
+  const bool better = ok & (t >= NEARLY0_F) & ((t < h.t) | ((t == h.t) & (idx < h.obj)));
+  h.t = better ? t : h.t;
+  h.obj = better ? idx : h.obj;
 }
 
 // distance_sphere (geometry.rs:179-193) of the sphere (c, radius^2 = rr).
+// The reference offers both roots t0 -+ t2 (one when t2 = 0); of the two,
+// the farther can only be taken when the nearer lies short of NEARLY0, so
+// one take of that root gives what taking both in turn gives.
 __device__ __forceinline__ void sphere_hit(V3 c, float rr, int32_t idx, V3 o, V3 d, Hit& h) {
 #pragma clang fp contract(off)
   V3 oc = c - o;
   float t0 = dot(oc, d);
   float t1 = rr - (dot(oc, oc) - t0 * t0);
-  if (t1 > 0.0f) {
-    float t2 = __builtin_amdgcn_sqrtf(t1);  // v_sqrt_f32, ~1 ulp
-    if (t2 == 0.0f) {
-      take(h, t0, idx);
-    } else {
-      take(h, t0 - t2, idx);
-      take(h, t0 + t2, idx);
-    }
-  }
+  const float t2 = __builtin_amdgcn_sqrtf(fmaxf(t1, 0.0f));  // v_sqrt_f32, ~1 ulp
+  const float tn = t0 - t2;
+  take(h, tn >= NEARLY0_F ? tn : t0 + t2, idx, t1 > 0.0f);
 }
 
 // distance_plain (geometry.rs:170-177), distance_sphere (:179-193) and
@@ -259,19 +257,19 @@ __device__ __forceinline__ void shape_hit(const DevShape& s, int32_t idx, V3 o, 
       break;
     case SHAPE_POLYGON:
     case SHAPE_PARALLELOGRAM: {
+      // computed whatever det is, and taken only for det != 0
       V3 d1 = mk(s.d1), d2 = mk(s.d2);
       V3 re2 = cross(d, d2);
       float det = dot(re2, d1);
-      if (det != 0.0f) {
-        V3 pp = o - mk(s.p);
-        V3 te1 = cross(pp, d1);
-        float inv = __builtin_amdgcn_rcpf(det);
-        float u = dot(re2, pp) * inv;
-        float v = dot(te1, d) * inv;
-        float t = dot(te1, d2) * inv;
-        float lim = s.kind == SHAPE_POLYGON ? 1.0f : 2.0f;
-        if (!(u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f || u + v > lim)) take(h, t, idx);
-      }
+      V3 pp = o - mk(s.p);
+      V3 te1 = cross(pp, d1);
+      float inv = __builtin_amdgcn_rcpf(det);
+      float u = dot(re2, pp) * inv;
+      float v = dot(te1, d) * inv;
+      float t = dot(te1, d2) * inv;
+      float lim = s.kind == SHAPE_POLYGON ? 1.0f : 2.0f;
+      const bool out = (u < 0.0f) | (u > 1.0f) | (v < 0.0f) | (v > 1.0f) | (u + v > lim);
+      take(h, t, idx, (det != 0.0f) & !out);
       break;
     }
     default:
@@ -333,13 +331,17 @@ __device__ __forceinline__ bool slab(const Node& b, V3 inv, V3 noi, float tmax, 
   return t0 <= t1;
 }
 
-// BVH node i and the nearest-hit test of primitive j: from LDS when
-// scene_to_lds put the tree there, else from global memory.
-template <bool CACHED>
+// BVH node i: from the LDS copy (NL: scene_to_lds put the nodes there), else
+// from global memory.  The walks are instantiated for each source behind one
+// uniform test, so a node load is an LDS or a global load, not a flat load
+// through a selected pointer with its 64-bit address arithmetic.
+template <bool NL>
 __device__ __forceinline__ Node bvh_node(const SceneView& sc, uint32_t i) {
-  if (CACHED && (sc.lds & 4u)) return bvh_lds_nodes()[i];
+  if (NL) return bvh_lds_nodes()[i];
   return sc.bvh[i];
 }
+// The nearest-hit test of primitive j: from its LDS compact sphere when
+// scene_to_lds put the primitives there, else from global memory.
 template <bool CACHED>
 __device__ __forceinline__ void bprim_hit(const SceneView& sc, uint32_t j, V3 o, V3 d, Hit& h) {
   if (CACHED && (sc.lds & 8u)) {
@@ -372,15 +374,14 @@ __device__ __forceinline__ uint32_t trail_pop(uint32_t& trail, uint32_t node) {
 // nearer child first, the farther one deferred on the bit trail).  A node is
 // entered while its box starts no later than the best hit, so equal
 // distances still reach the lower shape index.
-template <bool CACHED>
-__device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
-  if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
+template <bool CACHED, bool NL>
+__device__ __forceinline__ void bvh_nearest_walk(const SceneView& sc, V3 o, V3 d, Hit& h) {
   const V3 inv = slab_inv(d);
   const V3 noi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node<CACHED>(sc, 0), inv, noi, h.t, tn)) return;
+  if (!slab(bvh_node<NL>(sc, 0), inv, noi, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
@@ -389,8 +390,8 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
     } else {
       const uint32_t l = 2u * node + 1u;
       float tl, tr;
-      const bool hl = slab(bvh_node<CACHED>(sc, l), inv, noi, h.t, tl);
-      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), inv, noi, h.t, tr);
+      const bool hl = slab(bvh_node<NL>(sc, l), inv, noi, h.t, tl);
+      const bool hr = slab(bvh_node<NL>(sc, l + 1u), inv, noi, h.t, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -405,13 +406,21 @@ __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node<CACHED>(sc, node), inv, noi, h.t, tn)) {
+      if (slab(bvh_node<NL>(sc, node), inv, noi, h.t, tn)) {
         more = true;
         break;
       }
     }
     if (!more) break;
   }
+}
+template <bool CACHED>
+__device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
+  if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
+  if (CACHED && (sc.lds & 4u))
+    bvh_nearest_walk<CACHED, true>(sc, o, d, h);
+  else
+    bvh_nearest_walk<CACHED, false>(sc, o, d, h);
 }
 
 // calc_intersection (tracer.rs:306-335): the nearest hit over every shape.
@@ -665,16 +674,15 @@ __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, 
 // occluder; sets any when some primitive is hit.  Once a hit is known only
 // boxes entered before t_occ (beyond which no hit can be an occluder) are
 // searched.
-template <bool CACHED>
-__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
-  if (sc.n_bprims == 0) return false;
+template <bool CACHED, bool NL>
+__device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
   const V3 inv = slab_inv(d);
   const V3 noi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const float t_occ = sqrtf(fmaxf(sq_l - LIGHT_SLACK_F, 0.0f)) * 1.001f + 1e-4f;
   const uint32_t first_leaf = (1u << sc.bvh_depth) - 1u;
   uint32_t node = 0, trail = 0;
   float tn;
-  if (!slab(bvh_node<CACHED>(sc, 0), inv, noi, any ? t_occ : INF_F, tn)) return false;
+  if (!slab(bvh_node<NL>(sc, 0), inv, noi, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
       const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
@@ -685,8 +693,8 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
       const uint32_t l = 2u * node + 1u;
       const float tmax = any ? t_occ : INF_F;
       float tl, tr;
-      const bool hl = slab(bvh_node<CACHED>(sc, l), inv, noi, tmax, tl);
-      const bool hr = slab(bvh_node<CACHED>(sc, l + 1u), inv, noi, tmax, tr);
+      const bool hl = slab(bvh_node<NL>(sc, l), inv, noi, tmax, tl);
+      const bool hr = slab(bvh_node<NL>(sc, l + 1u), inv, noi, tmax, tr);
       if (hl && hr) {
         const bool left_first = tl <= tr;
         node = left_first ? l : l + 1u;
@@ -701,7 +709,7 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
     bool more = false;
     while (trail != 0u) {
       node = trail_pop(trail, node);
-      if (slab(bvh_node<CACHED>(sc, node), inv, noi, any ? t_occ : INF_F, tn)) {
+      if (slab(bvh_node<NL>(sc, node), inv, noi, any ? t_occ : INF_F, tn)) {
         more = true;
         break;
       }
@@ -709,6 +717,12 @@ __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, floa
     if (!more) break;
   }
   return false;
+}
+template <bool CACHED>
+__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
+  if (sc.n_bprims == 0) return false;
+  if (CACHED && (sc.lds & 4u)) return bvh_shadow_walk<CACHED, true>(sc, o, d, sq_l, any);
+  return bvh_shadow_walk<CACHED, false>(sc, o, d, sq_l, any);
 }
 
 // Linear shapes that can meet a segment from p to a point of light L's
