@@ -174,6 +174,7 @@ __device__ __forceinline__ void copy16(float4* dst, const float4* src, uint32_t 
 // cost it occupancy (mesh 105.8 -> 98.6 ms per pass without it).
 constexpr uint32_t BVH_LDS_NODES = 255;  // leaf level <= 7: 128 leaves
 constexpr uint32_t BVH_LDS_PRIMS = 512;
+static_assert(BVH_LEAF_SMALL * ((BVH_LDS_NODES + 1u) / 2u) <= BVH_LDS_PRIMS, "every leaf slot of a small tree in LDS");
 extern __shared__ float4 g_dyn_lds[];
 __device__ __forceinline__ Node* bvh_lds_nodes() { return reinterpret_cast<Node*>(g_dyn_lds); }
 __device__ __forceinline__ float4* bvh_lds_sph() { return g_dyn_lds + 2u * BVH_LDS_NODES; }
@@ -196,12 +197,21 @@ __device__ __forceinline__ void scene_to_lds(SceneView& sc) {
     nodes = n_nodes <= BVH_LDS_NODES;
     prims = sc.n_bprims <= BVH_LDS_PRIMS;
     if (nodes) copy16((float4*)bvh_lds_nodes(), (const float4*)sc.bvh, n_nodes * (uint32_t)(sizeof(Node) / 16));
+    // every slot of the tree's leaves: the slots past the last primitive
+    // hold w = -2 (no primitive), so a leaf's tests need no bound
+    const uint32_t slots = nodes ? (BVH_LEAF_SMALL << sc.bvh_depth) : sc.n_bprims;
     if (prims)
-      for (uint32_t j = threadIdx.x; j < sc.n_bprims; j += blockDim.x) {
-        const DevShape s = sc.bprims[j];
-        bvh_lds_sph()[j] = s.kind == SHAPE_SPHERE ? make_float4(s.p.x, s.p.y, s.p.z, s.radius * s.radius)
-                                                  : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        bvh_lds_idx()[j] = (int32_t)__float_as_uint(s.p.w);
+      for (uint32_t j = threadIdx.x; j < slots; j += blockDim.x) {
+        float4 c = make_float4(0.0f, 0.0f, 0.0f, -2.0f);
+        int32_t idx = 0;
+        if (j < sc.n_bprims) {
+          const DevShape s = sc.bprims[j];
+          c = s.kind == SHAPE_SPHERE ? make_float4(s.p.x, s.p.y, s.p.z, s.radius * s.radius)
+                                     : make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+          idx = (int32_t)__float_as_uint(s.p.w);
+        }
+        bvh_lds_sph()[j] = c;
+        bvh_lds_idx()[j] = idx;
       }
   }
   __syncthreads();
@@ -230,14 +240,15 @@ __device__ __forceinline__ void take(Hit& h, float t, int32_t idx, bool ok = tru
 // The reference offers both roots t0 -+ t2 (one when t2 = 0); of the two,
 // the farther can only be taken when the nearer lies short of NEARLY0, so
 // one take of that root gives what taking both in turn gives.
-__device__ __forceinline__ void sphere_hit(V3 c, float rr, int32_t idx, V3 o, V3 d, Hit& h) {
+// Taken only when ok.
+__device__ __forceinline__ void sphere_hit(V3 c, float rr, int32_t idx, V3 o, V3 d, Hit& h, bool ok = true) {
 #pragma clang fp contract(off)
   V3 oc = c - o;
   float t0 = dot(oc, d);
   float t1 = rr - (dot(oc, oc) - t0 * t0);
   const float t2 = __builtin_amdgcn_sqrtf(fmaxf(t1, 0.0f));  // v_sqrt_f32, ~1 ulp
   const float tn = t0 - t2;
-  take(h, tn >= NEARLY0_F ? tn : t0 + t2, idx, t1 > 0.0f);
+  take(h, tn >= NEARLY0_F ? tn : t0 + t2, idx, ok & (t1 > 0.0f));
 }
 
 // distance_plain (geometry.rs:170-177), distance_sphere (:179-193) and
@@ -355,6 +366,32 @@ __device__ __forceinline__ void bprim_hit(const SceneView& sc, uint32_t j, V3 o,
   shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
 }
 
+// The nearest-hit tests of small-tree leaf `leaf`, all from the LDS copies
+// (its BVH_LEAF_SMALL slots, padded past the last primitive): the slots'
+// loads are issued together and the sphere tests run without branches.
+__device__ __forceinline__ void leaf_hit_lds(const SceneView& sc, uint32_t leaf, V3 o, V3 d, Hit& h) {
+  const uint32_t base = leaf * BVH_LEAF_SMALL;
+  float4 c[BVH_LEAF_SMALL];
+  int32_t id[BVH_LEAF_SMALL];
+  bool other = false;
+#pragma unroll
+  for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k) {
+    c[k] = bvh_lds_sph()[base + k];
+    id[k] = bvh_lds_idx()[base + k];
+  }
+#pragma unroll
+  for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k) {
+    sphere_hit(mk(c[k].x, c[k].y, c[k].z), c[k].w, id[k], o, d, h, c[k].w >= 0.0f);
+    other |= c[k].w == -1.0f;
+  }
+  if (other)  // primitives of another kind, from global memory
+    for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k)
+      if (c[k].w == -1.0f) {
+        const DevShape s = sc.bprims[base + k];
+        shape_hit(s, (int32_t)__float_as_uint(s.p.w), o, d, h);
+      }
+}
+
 // The BVH is a complete binary tree in heap order (children of n: 2n+1,
 // 2n+2), and a depth-first walk only ever defers the sibling of the child it
 // enters.  So instead of a stack, a bit trail (Laine, "Restart trail for
@@ -384,9 +421,13 @@ __device__ __forceinline__ void bvh_nearest_walk(const SceneView& sc, V3 o, V3 d
   if (!slab(bvh_node<NL>(sc, 0), inv, noi, h.t, tn)) return;
   while (true) {
     if (node >= first_leaf) {
-      const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
-      const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
-      for (uint32_t j = base; j < end; ++j) bprim_hit<CACHED>(sc, j, o, d, h);
+      if (NL) {
+        leaf_hit_lds(sc, node - first_leaf, o, d, h);
+      } else {
+        const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
+        const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
+        for (uint32_t j = base; j < end; ++j) bprim_hit<CACHED>(sc, j, o, d, h);
+      }
     } else {
       const uint32_t l = 2u * node + 1u;
       float tl, tr;
@@ -417,7 +458,7 @@ __device__ __forceinline__ void bvh_nearest_walk(const SceneView& sc, V3 o, V3 d
 template <bool CACHED>
 __device__ __forceinline__ void bvh_nearest(const SceneView& sc, V3 o, V3 d, Hit& h) {
   if (sc.n_bprims == 0) return;  // a scene without a BVH in a batch that has one
-  if (CACHED && (sc.lds & 4u))
+  if (CACHED && (sc.lds & 12u) == 12u)  // nodes and primitives in LDS
     bvh_nearest_walk<CACHED, true>(sc, o, d, h);
   else
     bvh_nearest_walk<CACHED, false>(sc, o, d, h);
@@ -669,6 +710,23 @@ __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, 
   }
   return occludes(sc.bprims[j], p, d, sq_l, any);
 }
+// bprim_occludes over small-tree leaf `leaf` from the LDS copies, as
+// leaf_hit_lds; true at the first occluder.
+__device__ __forceinline__ bool leaf_occludes_lds(const SceneView& sc, uint32_t leaf, V3 p, V3 d, float sq_l,
+                                                  bool& any) {
+  const uint32_t base = leaf * BVH_LEAF_SMALL;
+  float4 c[BVH_LEAF_SMALL];
+#pragma unroll
+  for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k) c[k] = bvh_lds_sph()[base + k];
+#pragma unroll
+  for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k) {
+    Hit h{INF_F, -1};
+    sphere_hit(mk(c[k].x, c[k].y, c[k].z), c[k].w, 0, p, d, h, c[k].w >= 0.0f);
+    if (occluder_hit(h, p, d, sq_l, any)) return true;
+    if (c[k].w == -1.0f && occludes(sc.bprims[base + k], p, d, sq_l, any)) return true;
+  }
+  return false;
+}
 
 // Any-hit form of bvh_nearest for the shadow test: true at the first
 // occluder; sets any when some primitive is hit.  Once a hit is known only
@@ -685,10 +743,14 @@ __device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d,
   if (!slab(bvh_node<NL>(sc, 0), inv, noi, any ? t_occ : INF_F, tn)) return false;
   while (true) {
     if (node >= first_leaf) {
-      const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
-      const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
-      for (uint32_t j = base; j < end; ++j)
-        if (bprim_occludes<CACHED>(sc, j, o, d, sq_l, any)) return true;
+      if (NL) {
+        if (leaf_occludes_lds(sc, node - first_leaf, o, d, sq_l, any)) return true;
+      } else {
+        const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
+        const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
+        for (uint32_t j = base; j < end; ++j)
+          if (bprim_occludes<CACHED>(sc, j, o, d, sq_l, any)) return true;
+      }
     } else {
       const uint32_t l = 2u * node + 1u;
       const float tmax = any ? t_occ : INF_F;
@@ -721,7 +783,7 @@ __device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d,
 template <bool CACHED>
 __device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
   if (sc.n_bprims == 0) return false;
-  if (CACHED && (sc.lds & 4u)) return bvh_shadow_walk<CACHED, true>(sc, o, d, sq_l, any);
+  if (CACHED && (sc.lds & 12u) == 12u) return bvh_shadow_walk<CACHED, true>(sc, o, d, sq_l, any);
   return bvh_shadow_walk<CACHED, false>(sc, o, d, sq_l, any);
 }
 
