@@ -2235,11 +2235,11 @@ ppm_status ppm_batch_create(ppm_ctx* c, uint32_t n, const ppm_scene* const* scen
   ppm_batch* b = new ppm_batch();
   b->ctx = c;
   b->n_pass.assign(n_passes, n_passes + n);
-  // pipelines: the smallest cfgs[i].pipelines, 2 where it is 0 (automatic)
-  // three rounds in flight by default: a full round keeps the GPU busy, and a
-  // third pipeline fills the tails of each stage, where the rooms with the
-  // most work are still running (64-room workload: 4.27 s with two, 3.98 s
-  // with three, 4.18 s with four; a session gains nothing past two)
+  // pipelines: the smallest cfgs[i].pipelines, 3 where it is 0 (automatic):
+  // three rounds in flight by default, since a full round keeps the GPU busy
+  // and a third pipeline fills the tails of each stage, where the rooms with
+  // the most work are still running (64-room workload: 3.19 s with two,
+  // 3.11 s with three, 3.18 s with four; a session gains nothing past two)
   b->K = 4;
   for (uint32_t i = 0; i < n; ++i) b->K = std::min<uint32_t>(b->K, cfgs[i].pipelines ? cfgs[i].pipelines : 3u);
   if (b->K == 0) b->K = 1;
