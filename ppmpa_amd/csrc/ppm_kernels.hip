@@ -2789,10 +2789,13 @@ hipError_t launch_grid_shade(hipStream_t s, const SceneBuf& sc, const MapBuf& m,
 namespace ppm {
 
 // Test hook: nearest hit of host-given rays (calc_intersection,
-// tracer.rs:306-335): t and scene index of the shape, or (inf, -1).
+// tracer.rs:306-335): t and scene index of the shape, or (inf, -1).  The
+// shapes are read as the pass kernels read them: a small tree, the shapes
+// tested one by one and the materials from their LDS copies.
 template <int BV, bool BVH = (BV != 0)>
 __global__ __launch_bounds__(256) void k_intersect(SceneView sc, const float4* o, const float4* d, uint32_t n,
                                                    float* t_out, int32_t* obj_out) {
+  scene_to_lds<BV>(sc);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const Hit h = nearest<BV>(sc, mk(o[i]), mk(d[i]));
@@ -2805,8 +2808,8 @@ hipError_t launch_intersect(hipStream_t s, const SceneBuf& sc, const float4* o, 
   if (n == 0) return hipSuccess;
   switch (bvh_kind(sc)) {
     case 2:
-      hipLaunchKernelGGL(k_intersect<2>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,
-                       obj_out);
+      hipLaunchKernelGGL(k_intersect<2>, dim3((n + 255) / 256), dim3(256), bvh_dyn_lds(sc), s, make_view(sc), o, d,
+                         n, t_out, obj_out);
       break;
     case 1:
       hipLaunchKernelGGL(k_intersect<1>, dim3((n + 255) / 256), dim3(256), 0, s, make_view(sc), o, d, n, t_out,

@@ -59,11 +59,15 @@ def test_bvh_layout():
     assert P.DeviceScene(c, sc2).stats() == (6, 8 + 80, 5)
 
 
-@pytest.mark.parametrize("subdiv,walls", [(3, "quad"), (4, "plane")])
+# subdivision 2: 320 triangles, 64 spheres and the light quad, a small tree
+# walked from its LDS copies (triangles and the quad read from global memory
+# beside the LDS spheres, leaf slots past the last shape padded)
+@pytest.mark.parametrize("subdiv,walls", [(3, "quad"), (4, "plane"), (2, "quad")])
 def test_bvh_equals_linear(subdiv, walls):
     say("bvh == linear", subdiv, walls)
     sc = P.mesh_scene(64, 48, subdiv=subdiv, n_spheres=64, walls=walls)
     on, off = _pair(sc)
+    assert (on.stats()[1] <= 512) == (subdiv == 2)
     o, d = _rays(40000, subdiv)
     t1, i1 = on.intersect(o, d)
     t0, i0 = off.intersect(o, d)
@@ -114,11 +118,14 @@ def test_bvh_full_mesh():
     assert (i2 >= 70).mean() > 0.5
 
 
-def test_bvh_pass_identical():
+@pytest.mark.parametrize("subdiv", [3, 2])
+def test_bvh_pass_identical(subdiv):
     """Whole passes (photon trace, eye rays, shadow rays) give the same image
-    through the BVH as through the one-by-one test."""
-    say("bvh pass identical")
-    sc = P.mesh_scene(48, 32, subdiv=3, n_spheres=24)
+    through the BVH as through the one-by-one test: a large tree read from
+    global memory, and a small tree of triangles, spheres and the light quad
+    walked from its LDS copies."""
+    say("bvh pass identical", subdiv)
+    sc = P.mesh_scene(48, 32, subdiv=subdiv, n_spheres=24)
     on, off = _pair(sc)
     c = ctx()
     cfg = P.pass_cfg(seed=3, n_photon=30000, radius0=0.02, alpha=2.0 / 3.0, max_depth=8)
