@@ -711,9 +711,9 @@ __device__ __forceinline__ bool bprim_occludes(const SceneView& sc, uint32_t j, 
   return occludes(sc.bprims[j], p, d, sq_l, any);
 }
 // bprim_occludes over small-tree leaf `leaf` from the LDS copies, as
-// leaf_hit_lds; true at the first occluder.
+// leaf_hit_lds; true at the first occluder, whose slot goes to occ.
 __device__ __forceinline__ bool leaf_occludes_lds(const SceneView& sc, uint32_t leaf, V3 p, V3 d, float sq_l,
-                                                  bool& any) {
+                                                  bool& any, uint32_t& occ) {
   const uint32_t base = leaf * BVH_LEAF_SMALL;
   float4 c[BVH_LEAF_SMALL];
 #pragma unroll
@@ -722,18 +722,21 @@ __device__ __forceinline__ bool leaf_occludes_lds(const SceneView& sc, uint32_t 
   for (uint32_t k = 0; k < BVH_LEAF_SMALL; ++k) {
     Hit h{INF_F, -1};
     sphere_hit(mk(c[k].x, c[k].y, c[k].z), c[k].w, 0, p, d, h, c[k].w >= 0.0f);
-    if (occluder_hit(h, p, d, sq_l, any)) return true;
-    if (c[k].w == -1.0f && occludes(sc.bprims[base + k], p, d, sq_l, any)) return true;
+    if (occluder_hit(h, p, d, sq_l, any) || (c[k].w == -1.0f && occludes(sc.bprims[base + k], p, d, sq_l, any))) {
+      occ = base + k;
+      return true;
+    }
   }
   return false;
 }
 
 // Any-hit form of bvh_nearest for the shadow test: true at the first
-// occluder; sets any when some primitive is hit.  Once a hit is known only
-// boxes entered before t_occ (beyond which no hit can be an occluder) are
-// searched.
+// occluder, whose primitive slot goes to occ; sets any when some primitive
+// is hit.  Once a hit is known only boxes entered before t_occ (beyond which
+// no hit can be an occluder) are searched.
 template <bool CACHED, bool NL>
-__device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
+__device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any,
+                                                uint32_t& occ) {
   const V3 inv = slab_inv(d);
   const V3 noi = mk(-o.x * inv.x, -o.y * inv.y, -o.z * inv.z);
   const float t_occ = sqrtf(fmaxf(sq_l - LIGHT_SLACK_F, 0.0f)) * 1.001f + 1e-4f;
@@ -744,12 +747,15 @@ __device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d,
   while (true) {
     if (node >= first_leaf) {
       if (NL) {
-        if (leaf_occludes_lds(sc, node - first_leaf, o, d, sq_l, any)) return true;
+        if (leaf_occludes_lds(sc, node - first_leaf, o, d, sq_l, any, occ)) return true;
       } else {
         const uint32_t base = (node - first_leaf) * sc.bvh_leaf;
         const uint32_t end = min(base + sc.bvh_leaf, sc.n_bprims);
         for (uint32_t j = base; j < end; ++j)
-          if (bprim_occludes<CACHED>(sc, j, o, d, sq_l, any)) return true;
+          if (bprim_occludes<CACHED>(sc, j, o, d, sq_l, any)) {
+            occ = j;
+            return true;
+          }
       }
     } else {
       const uint32_t l = 2u * node + 1u;
@@ -780,11 +786,17 @@ __device__ __forceinline__ bool bvh_shadow_walk(const SceneView& sc, V3 o, V3 d,
   }
   return false;
 }
+// occ: the primitive that occluded the lane's previous shadow ray from the
+// same point (the samples of one light go to nearby points), tested before
+// the walk; the walk's first occluder replaces it.  An occluder found either
+// way makes the same decision: the ray is shadowed, and some primitive is
+// hit.
 template <bool CACHED>
-__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any) {
+__device__ __forceinline__ bool bvh_shadow(const SceneView& sc, V3 o, V3 d, float sq_l, bool& any, uint32_t& occ) {
   if (sc.n_bprims == 0) return false;
-  if (CACHED && (sc.lds & 12u) == 12u) return bvh_shadow_walk<CACHED, true>(sc, o, d, sq_l, any);
-  return bvh_shadow_walk<CACHED, false>(sc, o, d, sq_l, any);
+  if (occ < sc.n_bprims && bprim_occludes<CACHED>(sc, occ, o, d, sq_l, any)) return true;
+  if (CACHED && (sc.lds & 12u) == 12u) return bvh_shadow_walk<CACHED, true>(sc, o, d, sq_l, any, occ);
+  return bvh_shadow_walk<CACHED, false>(sc, o, d, sq_l, any, occ);
 }
 
 // Linear shapes that can meet a segment from p to a point of light L's
@@ -979,7 +991,7 @@ __device__ __forceinline__ void shadow_need(const SceneView& sc, const DevLight&
 template <int BV, bool BVH = (BV != 0)>
 __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 ld, uint64_t cand, uint64_t need_o,
                                            uint64_t need_s, uint64_t wu_o, uint64_t wu_s, const LightSelf& ls,
-                                           float& cos0sq, float& sqd) {
+                                           uint32_t& occ, float& cos0sq, float& sqd) {
   V3 ld3;
   if (!normalize(ld, ld3)) return false;
   float c0 = dot(n, ld3);
@@ -998,7 +1010,7 @@ __device__ __forceinline__ bool sample_lit(const SceneView& sc, V3 p, V3 n, V3 l
   }
   for (uint32_t i = CAND_BITS; i < sc.n_lin; ++i)
     if (occludes(lin_shape(sc, i), p, ld3, sq_l, any)) return false;
-  if (BVH && bvh_shadow<BV == 2>(sc, p, ld3, sq_l, any)) return false;
+  if (BVH && bvh_shadow<BV == 2>(sc, p, ld3, sq_l, any, occ)) return false;
   // no occluder; lit iff the ray meets something at all (calc_intersection
   // returns None otherwise, tracer.rs:284)
   for (uint32_t i = 0; i < min(sc.n_lin, CAND_BITS) && !any; ++i) {
@@ -1069,11 +1081,12 @@ __device__ C3 direct_light(const SceneView& sc, V3 p, V3 n) {
       if (__ballot(((need_s >> i) & 1ull) != 0)) wu_s |= 1ull << i;
     }
     C3 prev{0.0f, 0.0f, 0.0f};
+    uint32_t occ = ~0u;  // the BVH primitive that shadowed the last sample
     for (int s = 0; s < 25; ++s) {
       const V3 ld = tss_point(sc, li, s) - p;
       if (!(dot(ln, ld) < 0.0f)) continue;  // light.rs:129-131
       float csq, sqd;
-      if (!sample_lit<BV>(sc, p, n, ld, cand, need_o, need_s, wu_o, wu_s, ls, csq, sqd)) continue;
+      if (!sample_lit<BV>(sc, p, n, ld, cand, need_o, need_s, wu_o, wu_s, ls, occ, csq, sqd)) continue;
       sum = sum + prev * csq;
       prev = light_radiance(L, sqd);
     }
