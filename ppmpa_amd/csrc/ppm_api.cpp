@@ -616,11 +616,37 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
   std::vector<DevShape> lin;
   double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
   std::vector<char> large(shapes.size(), 0);
-  for (size_t j = 0; j < bounded.size(); ++j) {
+  auto wide = [&](size_t j) {
     const double* lo = &boxes[6 * j];
     const double* hi = lo + 3;
     const double side = std::max({hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]});
-    if (side > 0.25 * size && bounded.size() > 1) {
+    return side > 0.25 * size && bounded.size() > 1;
+  };
+  size_t n_wide = 0;
+  for (size_t j = 0; j < bounded.size(); ++j) n_wide += wide(j) ? 1 : 0;
+  // Beside a small tree (one that fits the kernels' LDS copy), a
+  // ParallelogramLight's own panel (a parallelogram with exactly its corner
+  // and sides) stays in the list as well: there the shadow test knows that a
+  // ray to one of the light's samples meets it, and no walk of the tree has
+  // to reach the panel at the end of every shadow ray (64 rooms of 2-512
+  // spheres: 3.11 -> 2.91 s).  A large tree keeps it: the 327,744-shape
+  // mesh scene got slower in every stage without it (79.4 -> 83.1 ms per
+  // pass).
+  const bool small_tree = bounded.size() - n_wide <= BVH_SMALL_PRIMS;
+  auto lamp = [&](const DevShape& sh) {
+    if (!small_tree) return false;
+    if (sh.kind != SHAPE_PARALLELOGRAM) return false;
+    for (const DevLight& l : sc->lights)
+      if (l.kind == LIGHT_PARALLELOGRAM && sh.p.x == l.pos.x && sh.p.y == l.pos.y && sh.p.z == l.pos.z &&
+          sh.d1.x == l.d1.x && sh.d1.y == l.d1.y && sh.d1.z == l.d1.z && sh.d2.x == l.d2.x && sh.d2.y == l.d2.y &&
+          sh.d2.z == l.d2.z)
+        return true;
+    return false;
+  };
+  for (size_t j = 0; j < bounded.size(); ++j) {
+    const double* lo = &boxes[6 * j];
+    const double* hi = lo + 3;
+    if (wide(j) || lamp(shapes[bounded[j]])) {
       large[bounded[j]] = 1;
       continue;
     }

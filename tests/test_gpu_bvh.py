@@ -57,6 +57,10 @@ def test_bvh_layout():
     # planes stay one by one, and so does the 0.25 m light quad: the bounded
     # shapes of this scene span 0.92 m, and it is wider than a quarter of that
     assert P.DeviceScene(c, sc2).stats() == (6, 8 + 80, 5)
+    # beside a small tree the light's own panel stays in the list too: 320
+    # triangles and 64 spheres in the tree, the walls and the panel one by one
+    sc3 = P.mesh_scene(32, 24, subdiv=2, n_spheres=64)
+    assert P.DeviceScene(c, sc3).stats() == (6, 320 + 64, 7)
 
 
 # subdivision 2: 320 triangles, 64 spheres and the light quad, a small tree
