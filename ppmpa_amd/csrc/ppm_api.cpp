@@ -624,17 +624,16 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
   };
   size_t n_wide = 0;
   for (size_t j = 0; j < bounded.size(); ++j) n_wide += wide(j) ? 1 : 0;
-  // Beside a small tree (one that fits the kernels' LDS copy), a
-  // ParallelogramLight's own panel (a parallelogram with exactly its corner
-  // and sides) stays in the list as well: there the shadow test knows that a
-  // ray to one of the light's samples meets it, and no walk of the tree has
-  // to reach the panel at the end of every shadow ray (64 rooms of 2-512
-  // spheres: 3.11 -> 2.91 s).  A large tree keeps it: the 327,744-shape
-  // mesh scene got slower in every stage without it (79.4 -> 83.1 ms per
-  // pass).
+  // A ParallelogramLight's own panel (a parallelogram with exactly its
+  // corner and sides) stays in the list as well: there the shadow test knows
+  // that a ray to one of the light's samples meets it, and no walk of the
+  // tree has to reach the panel at the end of every shadow ray (64 rooms of
+  // 2-512 spheres: 3.11 -> 2.91 s; the 327,744-shape mesh scene 80.2 ->
+  // 78.5 ms per pass).  Beside a large tree the panel's centre still bounds
+  // the Morton box: without it the box, and with it the tree over the mesh,
+  // changed, and that scene got slower in every stage (79.4 -> 83.1 ms).
   const bool small_tree = bounded.size() - n_wide <= BVH_SMALL_PRIMS;
   auto lamp = [&](const DevShape& sh) {
-    if (!small_tree) return false;
     if (sh.kind != SHAPE_PARALLELOGRAM) return false;
     for (const DevLight& l : sc->lights)
       if (l.kind == LIGHT_PARALLELOGRAM && sh.p.x == l.pos.x && sh.p.y == l.pos.y && sh.p.z == l.pos.z &&
@@ -646,7 +645,14 @@ static ppm_status scene_layout_bvh(ppm_ctx* c, ppm_scene* sc, const std::vector<
   for (size_t j = 0; j < bounded.size(); ++j) {
     const double* lo = &boxes[6 * j];
     const double* hi = lo + 3;
-    if (wide(j) || lamp(shapes[bounded[j]])) {
+    const bool is_lamp = !wide(j) && lamp(shapes[bounded[j]]);
+    if (is_lamp && !small_tree)
+      for (int a = 0; a < 3; ++a) {
+        const double cc = 0.5 * (lo[a] + hi[a]);
+        cmin[a] = std::min(cmin[a], cc);
+        cmax[a] = std::max(cmax[a], cc);
+      }
+    if (wide(j) || is_lamp) {
       large[bounded[j]] = 1;
       continue;
     }

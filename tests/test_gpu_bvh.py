@@ -45,9 +45,10 @@ def test_bvh_layout():
     c = ctx()
     ds = P.DeviceScene(c, sc)
     n_lin, n_bvh, depth = ds.stats()
-    # the five 1 m walls (wider than a quarter of the scene) are tested one by
-    # one; the light quad, the spheres and the triangles are in the BVH
-    assert n_lin == 5 and n_bvh == sc.n_shapes - 5
+    # the five 1 m walls (wider than a quarter of the scene) and the light's
+    # own panel are tested one by one; the spheres and the triangles are in
+    # the BVH
+    assert n_lin == 6 and n_bvh == sc.n_shapes - 6
     per_leaf = 4 if n_bvh <= 512 else 1  # small trees: 4 shapes per leaf, fitting the LDS copy whole
     assert (1 << depth) >= -(-n_bvh // per_leaf) > (1 << depth) // 2
     small = P.DeviceScene(c, P.caustic_scene(32, 24))
@@ -57,8 +58,8 @@ def test_bvh_layout():
     # planes stay one by one, and so does the 0.25 m light quad: the bounded
     # shapes of this scene span 0.92 m, and it is wider than a quarter of that
     assert P.DeviceScene(c, sc2).stats() == (6, 8 + 80, 5)
-    # beside a small tree the light's own panel stays in the list too: 320
-    # triangles and 64 spheres in the tree, the walls and the panel one by one
+    # a small tree: 320 triangles and 64 spheres in it, the walls and the
+    # light's own panel one by one
     sc3 = P.mesh_scene(32, 24, subdiv=2, n_spheres=64)
     assert P.DeviceScene(c, sc3).stats() == (6, 320 + 64, 7)
 
@@ -110,7 +111,7 @@ def test_bvh_full_mesh():
     assert sc.n_shapes == 327680 + 64 + 6
     on, off = _pair(sc)
     n_lin, n_bvh, depth = on.stats()
-    assert n_lin == 5 and n_bvh == sc.n_shapes - 5 and depth == 19  # one shape per leaf: 2^19 >= 327,745
+    assert n_lin == 6 and n_bvh == sc.n_shapes - 6 and depth == 19  # one shape per leaf: 2^19 >= 327,744
     o, d = _rays(4096, 5)
     t1, i1 = on.intersect(o, d)
     t0, i0 = off.intersect(o, d)
